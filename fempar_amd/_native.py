@@ -1,0 +1,117 @@
+"""ctypes binding of libfemhip.so (the C-ABI declared in include/femhip.h).
+
+The library is loaded from the package directory only; there is no CPU
+fallback: without the built HIP library (or without a GPU at context creation)
+every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfemhip.so")
+
+OK, ERR_INVALID, ERR_HIP, ERR_OOM, ERR_NOT_CONVERGED = 0, 1, 2, 3, 4
+ERR_ZERO_DIAGONAL, ERR_STATE, ERR_ADMISSION, ERR_NO_DEVICE = 5, 6, 7, 8
+
+BC_ALL_BOUNDARY, BC_X0_FACE, BC_NONE = 0, 1, 2
+PHYS_POISSON, PHYS_ELASTICITY = 0, 1
+K_CONST, K_SIN3, K_CELL = 0, 1, 2
+F_CONST, F_MANUFACTURED, F_CELL_SIN = 0, 1, 2
+G_ZERO, G_LINEAR = 0, 1
+PREC_IDENTITY, PREC_JACOBI = 0, 1
+PRECISION_FP64, PRECISION_FP32_REFINED = 0, 1
+
+# iterative_solver_parameters.f90:37-46, in the C-ABI's numbering
+STOPPING_CRITERIA = ["RES_NRMGIVEN_RHS_NRMGIVEN", "RES_NRMGIVEN_RES_NRMGIVEN", "DELTA_RHS", "DELTA_DELTA",
+                     "RES_RES", "RES_RHS", "DELTA_RHS_AND_RES_RES", "DELTA_RHS_AND_RES_RHS",
+                     "DELTA_DELTA_AND_RES_RES", "DELTA_DELTA_AND_RES_RHS"]
+
+EXPORTS = [
+    "femhip_ctx_create", "femhip_ctx_destroy", "femhip_last_error",
+    "femhip_space_create_structured", "femhip_space_create_from_dofs", "femhip_space_info",
+    "femhip_space_get_dofs", "femhip_space_interpolate_dirichlet", "femhip_space_destroy",
+    "femhip_operator_create", "femhip_operator_info", "femhip_operator_get_pattern",
+    "femhip_operator_set_physics", "femhip_operator_compute", "femhip_operator_get_values",
+    "femhip_operator_apply", "femhip_operator_destroy",
+    "femhip_solver_create", "femhip_solver_solve", "femhip_solver_get_solution", "femhip_solver_destroy",
+    "femhip_solver_profile_spmv",
+]
+
+
+class FemhipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"femhip error {code}: {msg}")
+        self.code = code
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("n", C.c_int32 * 3), ("limits", C.c_double * 6), ("order", C.c_int32),
+                ("num_components", C.c_int32), ("dirichlet", C.c_int32)]
+
+
+class Physics(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("k_kind", C.c_int32), ("k0", C.c_double),
+                ("k_cell", C.POINTER(C.c_double)), ("f_kind", C.c_int32), ("f0", C.c_double),
+                ("g_kind", C.c_int32), ("E", C.c_double), ("nu", C.c_double), ("body", C.c_double * 3)]
+
+
+class CgParams(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("max_num_iterations", C.c_int32),
+                ("stopping_criterion", C.c_int32), ("preconditioner", C.c_int32), ("precision", C.c_int32),
+                ("inner_rtol", C.c_double), ("max_outer", C.c_int32)]
+
+
+class CgResult(C.Structure):
+    _fields_ = [("num_iterations", C.c_int32), ("did_converge", C.c_int32), ("outer_iterations", C.c_int32),
+                ("status", C.c_int32), ("error_estimate", C.c_double), ("rhs_convergence_test", C.c_double),
+                ("true_residual", C.c_double), ("rhs_norm", C.c_double), ("solve_ms", C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load libfemhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FemhipError(ERR_INVALID, f"{LIB_PATH} is missing: run fempar_amd.build.build() (no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    P, I32, I64, D = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    pP = C.POINTER(C.c_void_p)
+    sig = {
+        "femhip_ctx_create": [I32, pP], "femhip_ctx_destroy": [P], "femhip_last_error": [],
+        "femhip_space_create_structured": [P, C.POINTER(MeshDesc), pP],
+        "femhip_space_create_from_dofs": [P, C.POINTER(MeshDesc), P, I32, I32, pP],
+        "femhip_space_info": [P, C.POINTER(I64), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)],
+        "femhip_space_get_dofs": [P, P], "femhip_space_interpolate_dirichlet": [P, C.POINTER(Physics), P],
+        "femhip_space_destroy": [P],
+        "femhip_operator_create": [P, P, I32, pP],
+        "femhip_operator_info": [P, C.POINTER(I32), C.POINTER(I64), C.POINTER(I64)],
+        "femhip_operator_get_pattern": [P, P, P], "femhip_operator_set_physics": [P, C.POINTER(Physics)],
+        "femhip_operator_compute": [P, P, C.POINTER(D)], "femhip_operator_get_values": [P, P, P],
+        "femhip_operator_apply": [P, P, P], "femhip_operator_destroy": [P],
+        "femhip_solver_create": [P, P, C.POINTER(CgParams), pP],
+        "femhip_solver_solve": [P, P, P, C.POINTER(CgResult)], "femhip_solver_get_solution": [P, P],
+        "femhip_solver_destroy": [P],
+        "femhip_solver_profile_spmv": [P, I32, C.POINTER(D), C.POINTER(D)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name == "femhip_last_error" else C.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != OK:
+        msg = load().femhip_last_error().decode(errors="replace")
+        raise FemhipError(rc, msg)
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)

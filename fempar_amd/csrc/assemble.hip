@@ -1,0 +1,368 @@
+// assemble.hip -- cell-wise Galerkin integration + deterministic coloured
+// scatter-add into the CSR values and RHS on gfx950.
+//
+// Mirrors the integrate_galerkin plugins
+//   Poisson:    Sources/Tests/Serial/test_poisson/modules/poisson_discrete_integration.f90:62-141
+//               (coefficient k at the quadrature point as in the heterogeneous
+//               par_test_pb_bddc_poisson plugin)
+//   Elasticity: Sources/Tests/Par/par_test_pb_bddc_linear_elasticity/modules/
+//               sbm_irreducible_discrete_integration.i90:64-130
+// followed by impose_strong_dirichlet_bcs_conforming
+// (Integration/FESpace/sbm_fe_cell_iterator.i90:1610-1637) and the sparse
+// assembler's insert into a fixed CSR pattern
+// (LinearAlgebra/SparseMatrix/csr_sparse_matrix.f90:1179-1211, 1369-1389).
+//
+// On a uniform mesh every cell is a translate of the first one, so the element
+// matrix factors as  elmat_c = sum_q k_c(x_q) * M_q  with M_q = detJ w_q G_q^T G_q
+// fixed per space (nq small nd x nd matrices), and elvec_c = sum_q f_c(x_q) Nw_q.
+// Cells are processed colour by colour (2^dim colours by cell-index parity):
+// cells of one colour share no DOF, so every (row, col) slot is updated by at
+// most one thread per launch -- no atomics, and the result is deterministic.
+#include <hip/hip_runtime.h>
+
+#include "femhip_internal.h"
+
+namespace femhip {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct MeshDev {
+  int dim;
+  int n[3];
+  double lim[6];
+};
+
+struct PhysDev {
+  int kind, k_kind, f_kind;
+  double k0, f0;
+  const double *k_cell;
+};
+
+__device__ __forceinline__ double vcoord(const MeshDev &m, int d, int i) {
+  // coordinates i/n mapped to the domain limits (uniform_hex_mesh_generator.f90:608-612, 633-635)
+  double c = (double)i / (double)m.n[d];
+  return (m.lim[2 * d + 1] - m.lim[2 * d]) * c + m.lim[2 * d];
+}
+
+__device__ double k_at(const PhysDev &ph, int dim, const double x[3], int64_t cell) {
+  if (ph.k_kind == FEMHIP_K_SIN3) {
+    double v = sin(2 * kPi * x[0]) * sin(2 * kPi * x[1]);
+    if (dim == 3) v *= sin(2 * kPi * x[2]);
+    return 1.0 + 0.5 * v;
+  }
+  if (ph.k_kind == FEMHIP_K_CELL) return ph.k_cell[cell];
+  return ph.k0;
+}
+
+__device__ double f_at(const PhysDev &ph, int dim, const double x[3], int64_t cell) {
+  if (ph.f_kind == FEMHIP_F_MANUFACTURED) {
+    double su[3] = {1, 1, 1}, cu[3] = {0, 0, 0};
+    for (int k = 0; k < dim; ++k) {
+      su[k] = sin(kPi * x[k]);
+      cu[k] = cos(kPi * x[k]);
+    }
+    double u = su[0] * su[1] * su[2];
+    double kk = k_at(ph, dim, x, cell);
+    double f = kk * dim * kPi * kPi * u;
+    if (ph.k_kind == FEMHIP_K_SIN3) {
+      double s2[3] = {1, 1, 1}, c2[3] = {0, 0, 0};
+      for (int k = 0; k < dim; ++k) {
+        s2[k] = sin(2 * kPi * x[k]);
+        c2[k] = cos(2 * kPi * x[k]);
+      }
+      double dot = kPi * c2[0] * s2[1] * s2[2] * kPi * cu[0] * su[1] * su[2] +
+                   kPi * s2[0] * c2[1] * s2[2] * kPi * su[0] * cu[1] * su[2];
+      if (dim == 3) dot += kPi * s2[0] * s2[1] * c2[2] * kPi * su[0] * su[1] * cu[2];
+      f -= dot;
+    }
+    return f;
+  }
+  if (ph.f_kind == FEMHIP_F_CELL_SIN) {
+    double u = 1.0;
+    for (int k = 0; k < dim; ++k) u *= sin(kPi * x[k]);
+    return ph.f0 * k_at(ph, dim, x, cell) * dim * kPi * kPi * u;
+  }
+  return ph.f0;
+}
+
+__global__ void k_fill_zero(double *__restrict__ a, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) a[i] = 0.0;
+}
+
+// position of column c in the sorted row segment col[b, e); -1 if absent
+__device__ __forceinline__ int64_t find_col(const int32_t *__restrict__ col, int64_t b, int64_t e, int32_t c) {
+  int64_t lo = b, hi = e - 1;
+  while (lo <= hi) {
+    int64_t mid = (lo + hi) >> 1;
+    int32_t v = col[mid];
+    if (v == c) return mid;
+    if (v < c) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+// Cell (i,j,k) of colour `color` with index t inside the colour.
+__device__ __forceinline__ int64_t color_cell(const MeshDev &m, int color, int64_t t, int ijk[3]) {
+  int64_t rem = t;
+  int64_t cell = 0, mul = 1;
+  for (int d = 0; d < m.dim; ++d) {
+    int b = (color >> d) & 1;
+    int cnt = (m.n[d] - b + 1) / 2;
+    int a = (int)(rem % cnt);
+    rem /= cnt;
+    ijk[d] = 2 * a + b;
+    cell += mul * ijk[d];
+    mul *= m.n[d];
+  }
+  for (int d = m.dim; d < 3; ++d) ijk[d] = 0;
+  return cell;
+}
+
+// Poisson: ND local DOFs, NQ quadrature points, CPB cells per block, one
+// thread per (cell, local row).
+template <int DIM, int ND, int NQ, int CPB>
+__global__ __launch_bounds__(CPB *ND) void k_assemble_poisson(
+    MeshDev m, PhysDev ph, int color, int64_t ncolor_cells, const int32_t *__restrict__ dofs,
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ tab,
+    const double *__restrict__ fixed, double *__restrict__ val, double *__restrict__ rhs) {
+  // tab layout: M[NQ][ND][ND] | Nw[NQ][ND] | G[NQ][2^DIM]
+  const double *M = tab;
+  const double *Nw = tab + NQ * ND * ND;
+  const double *G = Nw + NQ * ND;
+  __shared__ double s_k[CPB][NQ];
+  __shared__ double s_f[CPB][NQ];
+  __shared__ int64_t s_cell[CPB];
+  const int64_t base = (int64_t)blockIdx.x * CPB;
+
+  for (int t = threadIdx.x; t < CPB * NQ; t += blockDim.x) {
+    int lc = t / NQ, q = t % NQ;
+    int64_t tc = base + lc;
+    if (tc >= ncolor_cells) continue;
+    int ijk[3];
+    int64_t cell = color_cell(m, color, tc, ijk);
+    if (q == 0) s_cell[lc] = cell;
+    // x_q = sum_v N_v(q) x_v (sbm_cell_map.i90:673-687)
+    double x[3] = {0.0, 0.0, 0.0};
+    for (int v = 0; v < (1 << DIM); ++v) {
+      double gv = G[q * (1 << DIM) + v];
+      for (int d = 0; d < DIM; ++d) x[d] += vcoord(m, d, ijk[d] + ((v >> d) & 1)) * gv;
+    }
+    s_k[lc][q] = k_at(ph, DIM, x, cell);
+    s_f[lc][q] = f_at(ph, DIM, x, cell);
+  }
+  __syncthreads();
+
+  const int lc = threadIdx.x / ND, i = threadIdx.x % ND;
+  if (base + lc >= ncolor_cells) return;
+  const int64_t cell = s_cell[lc];
+  const int32_t *cd = dofs + cell * ND;
+  const int32_t gi = cd[i];
+  if (gi < 0) return;   // fixed row: dropped (coo append skips ia < 1)
+  const int64_t rb = rowptr[gi], re = rowptr[gi + 1];
+  double bacc = 0.0;
+  for (int q = 0; q < NQ; ++q) bacc += s_f[lc][q] * Nw[q * ND + i];
+#pragma unroll 4
+  for (int j = 0; j < ND; ++j) {
+    double e = 0.0;
+    for (int q = 0; q < NQ; ++q) e += s_k[lc][q] * M[(q * ND + i) * ND + j];
+    const int32_t gj = cd[j];
+    if (gj >= 0) {
+      int64_t pos = find_col(col, rb, re, gj);
+      if (pos >= 0) val[pos] += e;
+    } else if (fixed) {
+      bacc -= e * fixed[-gj - 1];
+    }
+  }
+  rhs[gi] += bacc;
+}
+
+// Elasticity with constant (lambda, mu) and body force: the element matrix and
+// vector are the same for every cell; only the scatter runs per cell.
+template <int ND, int CPB>
+__global__ __launch_bounds__(CPB *ND) void k_assemble_const(
+    MeshDev m, int color, int64_t ncolor_cells, const int32_t *__restrict__ dofs,
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ tab,
+    const double *__restrict__ fixed, double *__restrict__ val, double *__restrict__ rhs) {
+  const double *E = tab;          // elmat, row-major [ND][ND]
+  const double *F = tab + ND * ND;
+  const int64_t tc = (int64_t)blockIdx.x * CPB + threadIdx.x / ND;
+  const int i = threadIdx.x % ND;
+  if (tc >= ncolor_cells) return;
+  int ijk[3];
+  const int64_t cell = color_cell(m, color, tc, ijk);
+  const int32_t *cd = dofs + cell * ND;
+  const int32_t gi = cd[i];
+  if (gi < 0) return;
+  const int64_t rb = rowptr[gi], re = rowptr[gi + 1];
+  double bacc = F[i];
+  for (int j = 0; j < ND; ++j) {
+    const double e = E[i * ND + j];
+    const int32_t gj = cd[j];
+    if (gj >= 0) {
+      int64_t pos = find_col(col, rb, re, gj);
+      if (pos >= 0) val[pos] += e;
+    } else if (fixed) {
+      bacc -= e * fixed[-gj - 1];
+    }
+  }
+  rhs[gi] += bacc;
+}
+
+__global__ void k_spmv_csr(int32_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                           const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y) {
+  int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double acc = 0.0;
+  for (int32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) acc += val[k] * x[col[k]];
+  y[r] = acc;
+}
+
+MeshDev mesh_dev(const femhip_mesh_desc_t &d) {
+  MeshDev m{};
+  m.dim = d.dim;
+  for (int k = 0; k < 3; ++k) m.n[k] = k < d.dim ? d.n[k] : 1;
+  for (int k = 0; k < 6; ++k) m.lim[k] = d.limits[k];
+  return m;
+}
+
+template <int DIM, int ND, int NQ, int CPB>
+void launch_poisson(femhip_operator &op, const MeshDev &m, const PhysDev &ph, const double *d_fixed) {
+  femhip_space &s = *op.space;
+  for (int color = 0; color < (1 << DIM); ++color) {
+    int64_t cnt = 1;
+    for (int d = 0; d < DIM; ++d) cnt *= (m.n[d] - ((color >> d) & 1) + 1) / 2;
+    if (cnt == 0) continue;
+    int64_t blocks = (cnt + CPB - 1) / CPB;
+    hipLaunchKernelGGL((k_assemble_poisson<DIM, ND, NQ, CPB>), dim3((unsigned)blocks), dim3(CPB * ND), 0,
+                       op.ctx->stream, m, ph, color, cnt, s.d_dofs.p, op.rowptr.p, op.col.p, op.tab.p, d_fixed,
+                       op.val.p, op.rhs.p);
+  }
+}
+
+template <int ND, int CPB>
+void launch_const(femhip_operator &op, const MeshDev &m, const double *d_fixed) {
+  femhip_space &s = *op.space;
+  for (int color = 0; color < (1 << m.dim); ++color) {
+    int64_t cnt = 1;
+    for (int d = 0; d < m.dim; ++d) cnt *= (m.n[d] - ((color >> d) & 1) + 1) / 2;
+    if (cnt == 0) continue;
+    int64_t blocks = (cnt + CPB - 1) / CPB;
+    hipLaunchKernelGGL((k_assemble_const<ND, CPB>), dim3((unsigned)blocks), dim3(CPB * ND), 0, op.ctx->stream, m,
+                       color, cnt, s.d_dofs.p, op.rowptr.p, op.col.p, op.tab.p, d_fixed, op.val.p, op.rhs.p);
+  }
+}
+
+}  // namespace
+
+// Element tables for the operator's physics (host, once per set_physics).
+void upload_physics(femhip_operator &op) {
+  const RefFE &fe = op.space->fe;
+  const femhip_physics_t &ph = op.phys;
+  const int nd = fe.nd, nq = fe.nq, nn = fe.nnode, dim = fe.dim;
+  std::vector<double> t;
+  if (ph.kind == FEMHIP_PHYS_POISSON) {
+    FH_CHECK(fe.ncomp == 1, FEMHIP_ERR_INVALID, "Poisson plugin needs a scalar FE space");
+    t.assign((size_t)(nq * nd * nd + nq * nd + nq * (1 << dim)), 0.0);
+    for (int q = 0; q < nq; ++q) {
+      double factor = fe.detj[q] * fe.qw[q];
+      for (int i = 0; i < nd; ++i)
+        for (int j = 0; j < nd; ++j) {
+          double dot = 0.0;
+          for (int k = 0; k < 3; ++k) dot += fe.grad[j][k][q] * fe.grad[i][k][q];
+          t[(size_t)((q * nd + i) * nd + j)] = factor * dot;
+        }
+      for (int i = 0; i < nd; ++i) t[(size_t)(nq * nd * nd + q * nd + i)] = factor * fe.shp[i][q];
+      for (int v = 0; v < (1 << dim); ++v) t[(size_t)(nq * nd * nd + nq * nd + q * (1 << dim) + v)] = fe.gshp[v][q];
+    }
+    op.tab_kind = 0;
+    if (ph.k_kind == FEMHIP_K_CELL) {
+      FH_CHECK(ph.k_cell != nullptr, FEMHIP_ERR_INVALID, "k_cell is NULL for FEMHIP_K_CELL");
+      op.k_cell.alloc((size_t)op.space->ncells);
+      FH_HIP(hipMemcpy(op.k_cell.p, ph.k_cell, sizeof(double) * (size_t)op.space->ncells, hipMemcpyHostToDevice));
+    }
+  } else if (ph.kind == FEMHIP_PHYS_ELASTICITY) {
+    FH_CHECK(fe.ncomp == dim && dim == 3, FEMHIP_ERR_INVALID, "elasticity plugin needs a 3-D vector FE space");
+    const double E = ph.E, nu = ph.nu;
+    const double lambda = (nu * E) / ((1 + nu) * (1 - 2 * nu));
+    const double mu = E / (2 * (1 + nu));
+    t.assign((size_t)(nd * nd + nd), 0.0);
+    for (int q = 0; q < nq; ++q) {
+      double dV = fe.detj[q] * fe.qw[q];
+      for (int i = 0; i < nd; ++i) {
+        int ci = i / nn, ai = i % nn;
+        for (int j = 0; j < nd; ++j) {
+          int cj = j / nn, aj = j % nn;
+          // grad(phi_i) : sigma(phi_j),  sigma = lambda tr(eps) I + 2 mu eps
+          double gi[3], gj[3];
+          for (int k = 0; k < 3; ++k) {
+            gi[k] = fe.grad[ai][k][q];
+            gj[k] = fe.grad[aj][k][q];
+          }
+          double divdiv = gi[ci] * gj[cj];
+          double epseps = 0.5 * ((ci == cj ? gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2] : 0.0) + gi[cj] * gj[ci]);
+          t[(size_t)(i * nd + j)] += dV * (lambda * divdiv + 2 * mu * epseps);
+        }
+        t[(size_t)(nd * nd + i)] += dV * ph.body[ci] * fe.shp[ai][q];
+      }
+    }
+    op.tab_kind = 1;
+  } else {
+    throw Error{FEMHIP_ERR_INVALID, "unknown physics kind"};
+  }
+  op.tab.alloc(t.size());
+  FH_HIP(hipMemcpy(op.tab.p, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+}
+
+void assemble(femhip_operator &op, const double *fixed_host, float *ms) {
+  femhip_space &s = *op.space;
+  hipStream_t st = op.ctx->stream;
+  const double *d_fixed = nullptr;
+  if (fixed_host && s.n_fixed > 0) {
+    op.fixed.alloc((size_t)s.n_fixed);
+    FH_HIP(hipMemcpyAsync(op.fixed.p, fixed_host, sizeof(double) * (size_t)s.n_fixed, hipMemcpyHostToDevice, st));
+    d_fixed = op.fixed.p;
+  }
+  hipEvent_t e0, e1;
+  FH_HIP(hipEventCreate(&e0));
+  FH_HIP(hipEventCreate(&e1));
+  FH_HIP(hipEventRecord(e0, st));
+  int zb = (int)std::min<int64_t>(4096, (op.nnz_full + 255) / 256 + 1);
+  hipLaunchKernelGGL(k_fill_zero, dim3(zb), dim3(256), 0, st, op.val.p, op.nnz_full);
+  hipLaunchKernelGGL(k_fill_zero, dim3(std::max(1, std::min(1024, (op.n + 255) / 256))), dim3(256), 0, st, op.rhs.p,
+                     (int64_t)op.n);
+  MeshDev m = mesh_dev(s.mesh);
+  PhysDev ph{op.phys.kind, op.phys.k_kind, op.phys.f_kind, op.phys.k0, op.phys.f0, op.k_cell.p};
+  const RefFE &fe = s.fe;
+  if (op.tab_kind == 0) {
+    if (fe.dim == 2 && fe.nd == 4) launch_poisson<2, 4, 4, 64>(op, m, ph, d_fixed);
+    else if (fe.dim == 2 && fe.nd == 9) launch_poisson<2, 9, 9, 28>(op, m, ph, d_fixed);
+    else if (fe.dim == 3 && fe.nd == 8) launch_poisson<3, 8, 8, 32>(op, m, ph, d_fixed);
+    else if (fe.dim == 3 && fe.nd == 27) launch_poisson<3, 27, 27, 8>(op, m, ph, d_fixed);
+    else throw Error{FEMHIP_ERR_INVALID, "unsupported Poisson element"};
+  } else {
+    if (fe.nd == 24) launch_const<24, 10>(op, m, d_fixed);
+    else throw Error{FEMHIP_ERR_INVALID, "unsupported elasticity element"};
+  }
+  FH_HIP(hipGetLastError());
+  FH_HIP(hipEventRecord(e1, st));
+  FH_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  FH_HIP(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
+void spmv_full(femhip_operator &op, const double *d_x, double *d_y) {
+  if (op.n == 0) return;
+  hipLaunchKernelGGL(k_spmv_csr, dim3((op.n + 255) / 256), dim3(256), 0, op.ctx->stream, op.n, op.rowptr.p,
+                     op.col.p, op.val.p, d_x, d_y);
+  FH_HIP(hipGetLastError());
+}
+
+}  // namespace femhip

@@ -1,0 +1,133 @@
+// femhip_internal.h -- shared internal declarations of libfemhip (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "femhip.h"
+
+namespace femhip {
+
+// ---------------------------------------------------------------- errors ---
+void set_error(const std::string &msg);
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define FH_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      throw ::femhip::Error{FEMHIP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+#define FH_CHECK(cond, code, msg)                                                     \
+  do {                                                                                \
+    if (!(cond)) throw ::femhip::Error{(code), (msg)};                                \
+  } while (0)
+
+// ------------------------------------------------------- device buffers ---
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error{FEMHIP_ERR_OOM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+    }
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// ------------------------------------------------------------ reference FE ---
+constexpr int kMaxNd = 81;   // Q2 vector 3D upper bound of local DOFs
+constexpr int kMaxNq = 27;   // 3x3x3 Gauss points
+constexpr int kMaxNf = 27;   // n-faces of a hexahedron incl. the cell
+
+struct RefFE {
+  int dim = 0, order = 0, ncomp = 1;
+  int np1 = 0, nnode = 0, nd = 0, nq1 = 0, nq = 0, nnf = 0;
+  int nf_type[kMaxNf], nf_anchor[kMaxNf];   // polytope n-faces (sorted)
+  int own_ptr[kMaxNf + 1], own_lst[kMaxNd]; // own DOFs per n-face (reference order)
+  double qp[3][kMaxNq], qw[kMaxNq];
+  double shp[kMaxNd][kMaxNq];               // scalar shape values
+  double gshp[8][kMaxNq];                   // geometric Q1 shape values
+  double detj[kMaxNq];
+  double grad[kMaxNd][3][kMaxNq];           // physical gradients of scalar shapes (first cell map)
+};
+void build_reffe(RefFE &fe, int dim, int order, int ncomp, const double h[3]);
+void gauss_legendre(int n, double *x, double *w);
+
+// --------------------------------------------------------------- context ---
+}  // namespace femhip
+
+struct femhip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+};
+
+struct femhip_space {
+  femhip_ctx *ctx = nullptr;
+  femhip_mesh_desc_t mesh{};
+  femhip::RefFE fe;
+  int64_t ncells = 0;
+  int32_t n_free = 0, n_fixed = 0;
+  std::vector<int32_t> dofs;          // host copy (cells x nd), signed 1-based
+  femhip::DevBuf<int32_t> d_dofs;     // device: free -> row (0-based), fixed -> -(fixed index)-1
+};
+
+struct femhip_operator {
+  femhip_ctx *ctx = nullptr;
+  femhip_space *space = nullptr;
+  int symmetric_storage = 0;
+  int32_t n = 0;
+  int64_t nnz_full = 0, nnz_upper = 0;
+  // full CSR (solver format), 0-based on device
+  std::vector<int32_t> h_rowptr, h_col;     // host copies for the pattern API
+  femhip::DevBuf<int32_t> rowptr, col;
+  femhip::DevBuf<double> val, rhs;
+  femhip::DevBuf<float> val32;              // FP32 copy for the refined solver (lazy)
+  // upper-triangle view: index into the full arrays
+  std::vector<int64_t> h_upper_map;
+  femhip::DevBuf<int64_t> upper_map;
+  // physics
+  femhip_physics_t phys{};
+  bool has_physics = false;
+  femhip::DevBuf<double> k_cell;
+  femhip::DevBuf<double> tab;               // element tables (per-q matrices etc.)
+  int tab_kind = 0;
+  bool computed = false;
+  femhip::DevBuf<double> fixed;             // fixed values on device
+};
+
+namespace femhip {
+
+// host setup (space.cpp)
+void number_structured(femhip_space &s);
+void build_pattern(femhip_operator &op);
+
+// assembly (assemble.hip)
+void assemble(femhip_operator &op, const double *fixed_host, float *ms);
+void upload_physics(femhip_operator &op);
+void spmv_full(femhip_operator &op, const double *d_x, double *d_y);
+
+}  // namespace femhip

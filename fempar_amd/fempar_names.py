@@ -1,0 +1,396 @@
+"""Host-side mirror of FEMPAR's interfaces for the one-subdomain hot path.
+
+Names, argument meaning and error behaviour follow the Fortran API
+(Sources/Lib/...), so driver code reads like the reference's
+test_poisson_driver.f90:
+
+    tria = serial_triangulation_t(); tria.create(env, plist)
+    fe_space = serial_fe_space_t(); fe_space.create(tria, order=1)
+    op = fe_affine_operator_t()
+    op.create('CSR', [True], fe_space, poisson_discrete_integration_t(...))
+    op.compute()
+    solver = iterative_linear_solver_t(); solver.create(env)
+    solver.set_type_from_string('CG'); solver.set_parameters_from_pl(plist)
+    solver.set_operators(op.get_tangent(), jacobi_preconditioner_t(op))
+    x = solver.solve(op.get_translation())
+
+Everything below is plumbing over the C-ABI (include/femhip.h); the work runs
+in libfemhip's HIP kernels.  A failing call raises FemhipError, the analogue of
+the reference's check/mcheck abort.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from ._native import FemhipError
+
+# ParameterList keys (uniform_hex_mesh_generator_parameters.f90:31-35,
+# iterative_solver_parameters.f90:76-83)
+STRUCT_HEX_MESH_GENERATOR_NUM_DIMS = "STRUCT_HEX_MESH_GENERATOR_NUM_DIMS"
+STRUCT_HEX_MESH_GENERATOR_NUM_CELLS_X_DIM = "STRUCT_HEX_MESH_GENERATOR_NUM_CELLS_X_DIM"
+STRUCT_HEX_MESH_GENERATOR_DOMAIN_LIMITS = "STRUCT_HEX_MESH_GENERATOR_DOMAIN_LIMITS"
+ILS_TYPE = "ILS_TYPE"
+ILS_RTOL = "ILS_RTOL"
+ILS_ATOL = "ILS_ATOL"
+ILS_STOPPING_CRITERIUM = "ILS_STOPPING_CRITERIUM"
+ILS_MAX_NUM_ITERATIONS = "ILS_MAX_NUM_ITERATIONS"
+# keys of this implementation's refined mode (no reference counterpart)
+ILS_PRECISION = "FEMHIP_PRECISION"          # "FP64" | "FP32_REFINED"
+ILS_INNER_RTOL = "FEMHIP_INNER_RTOL"
+ILS_MAX_OUTER = "FEMHIP_MAX_OUTER"
+
+# defaults of iterative_solver_parameters.f90:166-178
+DEFAULT_RTOL = 1.0e-6
+DEFAULT_ATOL = 0.0
+DEFAULT_MAX_NUM_ITERATIONS = 1000
+DEFAULT_CG_STOPPING_CRITERIA = "RES_RES"
+
+cg_name = "CG"
+
+
+class environment_t:
+    """One GPU per process: owns the femhip context (device + stream)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = N.load()
+        h = C.c_void_p()
+        N.check(self._lib.femhip_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def free(self):
+        if getattr(self, "h", None):
+            self._lib.femhip_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class serial_triangulation_t:
+    """Struct-hex triangulation (sbm_serial_triangulation.i90:61-118); only the
+    generator parameters are kept, the mesh is implicit."""
+
+    def create(self, environment: environment_t, parameter_list: dict):
+        self.environment = environment
+        self.num_dims = int(parameter_list.get(STRUCT_HEX_MESH_GENERATOR_NUM_DIMS, 3))
+        n = list(parameter_list.get(STRUCT_HEX_MESH_GENERATOR_NUM_CELLS_X_DIM, [10, 10, 10]))
+        self.num_cells_x_dim = (n + [1, 1, 1])[:3]
+        lim = list(parameter_list.get(STRUCT_HEX_MESH_GENERATOR_DOMAIN_LIMITS, [0.0, 1.0, 0.0, 1.0, 0.0, 1.0]))
+        self.domain_limits = (lim + [0.0] * 6)[:6]
+        return self
+
+    def get_num_dims(self):
+        return self.num_dims
+
+
+class serial_fe_space_t:
+    """serial_fe_space_t on a struct-hex triangulation: strong Dirichlet and
+    first-touch DOF numbering (sbm_serial_fe_space.i90:189-244, 525-715, 2623-2922)."""
+
+    def create(self, triangulation: serial_triangulation_t, order: int = 1, num_components: int = 1,
+               dirichlet: str = "boundary", lst_dofs_gids=None, num_free_dofs=None, num_fixed_dofs=None):
+        self.triangulation = triangulation
+        env = triangulation.environment
+        self._lib = N.load()
+        d = N.MeshDesc()
+        d.dim = triangulation.num_dims
+        for i in range(3):
+            d.n[i] = triangulation.num_cells_x_dim[i]
+        for i in range(6):
+            d.limits[i] = triangulation.domain_limits[i]
+        d.order, d.num_components = order, num_components
+        d.dirichlet = {"boundary": N.BC_ALL_BOUNDARY, "x0": N.BC_X0_FACE, "none": N.BC_NONE}[dirichlet]
+        self.desc = d
+        h = C.c_void_p()
+        if lst_dofs_gids is None:
+            N.check(self._lib.femhip_space_create_structured(env.h, C.byref(d), C.byref(h)))
+        else:
+            arr = np.ascontiguousarray(lst_dofs_gids, dtype=np.int32)
+            N.check(self._lib.femhip_space_create_from_dofs(env.h, C.byref(d), N.ptr(arr), int(num_free_dofs),
+                                                            int(num_fixed_dofs), C.byref(h)))
+        self.h = h
+        nc, nd, nq, nf, nx = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        N.check(self._lib.femhip_space_info(h, C.byref(nc), C.byref(nd), C.byref(nq), C.byref(nf), C.byref(nx)))
+        self.num_cells, self.num_dofs_x_cell, self.num_quadrature_points = nc.value, nd.value, nq.value
+        self.num_free_dofs, self.num_fixed_dofs = nf.value, nx.value
+        return self
+
+    def get_num_dofs(self):
+        return self.num_free_dofs
+
+    def get_num_fixed_dofs(self):
+        return self.num_fixed_dofs
+
+    def get_lst_dofs_gids(self):
+        out = np.empty((self.num_cells, self.num_dofs_x_cell), dtype=np.int32)
+        N.check(self._lib.femhip_space_get_dofs(self.h, N.ptr(out)))
+        return out
+
+    def interpolate_dirichlet_values(self, discrete_integration):
+        out = np.zeros(max(1, self.num_fixed_dofs))
+        N.check(self._lib.femhip_space_interpolate_dirichlet(self.h, C.byref(discrete_integration.physics()),
+                                                             N.ptr(out)))
+        return out[: self.num_fixed_dofs]
+
+    def free(self):
+        if getattr(self, "h", None):
+            self._lib.femhip_space_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class poisson_discrete_integration_t:
+    """-div(k grad u) = f (poisson_discrete_integration.f90:62-141 with k(x_q))."""
+
+    def __init__(self, k="const", k0=1.0, k_cell=None, f="const", f0=0.0, g="zero"):
+        self.k, self.k0, self.k_cell, self.f, self.f0, self.g = k, k0, k_cell, f, f0, g
+
+    def physics(self):
+        p = N.Physics()
+        p.kind = N.PHYS_POISSON
+        p.k_kind = {"const": N.K_CONST, "sin3": N.K_SIN3, "cell": N.K_CELL}[self.k]
+        p.k0 = self.k0
+        self._kc = None
+        if self.k_cell is not None:
+            self._kc = np.ascontiguousarray(self.k_cell, dtype=np.float64)
+            p.k_cell = self._kc.ctypes.data_as(C.POINTER(C.c_double))
+        p.f_kind = {"const": N.F_CONST, "manufactured": N.F_MANUFACTURED, "cell_sin": N.F_CELL_SIN}[self.f]
+        p.f0 = self.f0
+        p.g_kind = {"zero": N.G_ZERO, "linear": N.G_LINEAR}[self.g]
+        return p
+
+
+class linear_elasticity_discrete_integration_t:
+    """Irreducible linear elasticity (sbm_irreducible_discrete_integration.i90:64-130)."""
+
+    def __init__(self, E=1.0, nu=0.2, body=(0.0, 0.0, -1.0)):
+        self.E, self.nu, self.body = E, nu, body
+
+    def physics(self):
+        p = N.Physics()
+        p.kind = N.PHYS_ELASTICITY
+        p.E, p.nu = self.E, self.nu
+        for i in range(3):
+            p.body[i] = self.body[i]
+        return p
+
+
+class csr_matrix_view:
+    """Host copy of csr_sparse_matrix_t's irp/ja/val (1-based like the reference)."""
+
+    def __init__(self, irp, ja, val, symmetric_storage):
+        self.irp, self.ja, self.val, self.symmetric_storage = irp, ja, val, symmetric_storage
+
+    def get_num_rows(self):
+        return len(self.irp) - 1
+
+    def get_nnz(self):
+        return len(self.ja)
+
+
+class fe_affine_operator_t:
+    """fe_affine_operator_t with a CSR sparse_assembler_t (fe_operator.f90:174-183)."""
+
+    def create(self, sparse_matrix_storage_format, diagonal_blocks_symmetric_storage, fe_space: serial_fe_space_t,
+               discrete_integration, diagonal_blocks_symmetric=None, diagonal_blocks_sign=None):
+        if sparse_matrix_storage_format != "CSR":
+            raise FemhipError(N.ERR_INVALID, "only CSR storage is supported")
+        self.fe_space = fe_space
+        self.discrete_integration = discrete_integration
+        self.symmetric_storage = bool(diagonal_blocks_symmetric_storage[0])
+        self._lib = N.load()
+        h = C.c_void_p()
+        N.check(self._lib.femhip_operator_create(fe_space.triangulation.environment.h, fe_space.h,
+                                                 int(self.symmetric_storage), C.byref(h)))
+        self.h = h
+        n, nnz, nnzf = C.c_int32(), C.c_int64(), C.c_int64()
+        N.check(self._lib.femhip_operator_info(h, C.byref(n), C.byref(nnz), C.byref(nnzf)))
+        self.n, self.nnz, self.nnz_full = n.value, nnz.value, nnzf.value
+        self._phys = discrete_integration.physics()
+        N.check(self._lib.femhip_operator_set_physics(h, C.byref(self._phys)))
+        self.fixed_values = None
+        self.last_compute_ms = None
+        return self
+
+    def set_fixed_dof_values(self, values):
+        self.fixed_values = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+
+    def compute(self):
+        ms = C.c_double()
+        N.check(self._lib.femhip_operator_compute(self.h, N.ptr(self.fixed_values), C.byref(ms)))
+        self.last_compute_ms = ms.value
+
+    def get_pattern(self):
+        irp = np.empty(self.n + 1, dtype=np.int32)
+        ja = np.empty(self.nnz, dtype=np.int32)
+        N.check(self._lib.femhip_operator_get_pattern(self.h, N.ptr(irp), N.ptr(ja)))
+        return irp, ja
+
+    def get_matrix(self) -> csr_matrix_view:
+        irp, ja = self.get_pattern()
+        val = np.empty(self.nnz)
+        N.check(self._lib.femhip_operator_get_values(self.h, N.ptr(val), None))
+        return csr_matrix_view(irp, ja, val, self.symmetric_storage)
+
+    def get_translation(self):
+        rhs = np.empty(self.n)
+        N.check(self._lib.femhip_operator_get_values(self.h, None, N.ptr(rhs)))
+        return rhs
+
+    def get_tangent(self):
+        return self
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n)
+        N.check(self._lib.femhip_operator_apply(self.h, N.ptr(x), N.ptr(y)))
+        return y
+
+    def free(self):
+        if getattr(self, "h", None):
+            self._lib.femhip_operator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class identity_operator_t:
+    """.identity. operator (operator.f90:39-180) used as M."""
+
+    kind = N.PREC_IDENTITY
+
+    def __init__(self, op=None):
+        self.op = op
+
+
+class jacobi_preconditioner_t:
+    """Jacobi preconditioner M = diag(A)^-1 (jacobi_preconditioner.f90:136-450).
+    Unlike the reference, it also works on a serial FE space."""
+
+    kind = N.PREC_JACOBI
+
+    def __init__(self, op=None):
+        self.op = op
+
+
+class iterative_linear_solver_t:
+    """iterative_linear_solver_t restricted to 'CG' (iterative_linear_solver.f90:62-241)."""
+
+    def create(self, environment: environment_t):
+        self.environment = environment
+        self.type = None
+        self.params = {ILS_RTOL: DEFAULT_RTOL, ILS_ATOL: DEFAULT_ATOL,
+                       ILS_MAX_NUM_ITERATIONS: DEFAULT_MAX_NUM_ITERATIONS,
+                       ILS_STOPPING_CRITERIUM: DEFAULT_CG_STOPPING_CRITERIA, ILS_PRECISION: "FP64",
+                       ILS_INNER_RTOL: 0.0, ILS_MAX_OUTER: 0}
+        self.A = None
+        self.M = None
+        self.h = None
+        self._lib = N.load()
+        return self
+
+    def set_type_from_string(self, name: str):
+        if name != cg_name:
+            raise FemhipError(N.ERR_INVALID, f"iterative linear solver '{name}' is not available (only CG)")
+        self.type = name
+
+    def set_parameters_from_pl(self, parameter_list: dict):
+        if ILS_TYPE in parameter_list:
+            self.set_type_from_string(parameter_list[ILS_TYPE])
+        for k in self.params:
+            if k in parameter_list:
+                self.params[k] = parameter_list[k]
+        if self.params[ILS_STOPPING_CRITERIUM] not in N.STOPPING_CRITERIA:
+            raise FemhipError(N.ERR_INVALID, f"CG does not support {self.params[ILS_STOPPING_CRITERIUM]}")
+        self._destroy()
+
+    def set_operators(self, A: fe_affine_operator_t, M):
+        self.A, self.M = A, M
+        self._destroy()
+
+    def _ensure(self):
+        if self.type is None:
+            raise FemhipError(N.ERR_STATE, "set_type_from_string not called")
+        if self.A is None:
+            raise FemhipError(N.ERR_STATE, "set_operators not called")
+        if self.h is None:
+            p = N.CgParams()
+            p.rtol = float(self.params[ILS_RTOL])
+            p.atol = float(self.params[ILS_ATOL])
+            p.max_num_iterations = int(self.params[ILS_MAX_NUM_ITERATIONS])
+            p.stopping_criterion = N.STOPPING_CRITERIA.index(self.params[ILS_STOPPING_CRITERIUM])
+            p.preconditioner = getattr(self.M, "kind", N.PREC_IDENTITY)
+            p.precision = {"FP64": N.PRECISION_FP64, "FP32_REFINED": N.PRECISION_FP32_REFINED}[
+                self.params[ILS_PRECISION]]
+            p.inner_rtol = float(self.params[ILS_INNER_RTOL])
+            p.max_outer = int(self.params[ILS_MAX_OUTER])
+            h = C.c_void_p()
+            N.check(self._lib.femhip_solver_create(self.environment.h, self.A.h, C.byref(p), C.byref(h)))
+            self.h = h
+
+    def solve(self, b=None, x=None):
+        """Solve A x = b from the initial guess x (zero if None); b None uses the
+        operator's assembled RHS on the device.  Returns x (host)."""
+        self._ensure()
+        n = self.A.n
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        xx = np.zeros(n) if x is None else np.array(x, dtype=np.float64)
+        res = N.CgResult()
+        N.check(self._lib.femhip_solver_solve(self.h, N.ptr(bb), N.ptr(xx), C.byref(res)))
+        self.result = {f: getattr(res, f) for f, _ in N.CgResult._fields_}
+        return xx
+
+    def solve_resident(self):
+        """Solve with the operator's device RHS from x0 = 0, solution kept on the device."""
+        self._ensure()
+        res = N.CgResult()
+        N.check(self._lib.femhip_solver_solve(self.h, None, None, C.byref(res)))
+        self.result = {f: getattr(res, f) for f, _ in N.CgResult._fields_}
+        return self.result
+
+    def get_solution(self):
+        x = np.empty(self.A.n)
+        N.check(self._lib.femhip_solver_get_solution(self.h, N.ptr(x)))
+        return x
+
+    def profile_spmv(self, reps=50):
+        """Average duration (ms) and algorithmic bytes of one SpMV launch of this solver."""
+        self._ensure()
+        ms, by = C.c_double(), C.c_double()
+        N.check(self._lib.femhip_solver_profile_spmv(self.h, int(reps), C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def get_num_iterations(self):
+        return self.result["num_iterations"]
+
+    def did_converge(self):
+        return bool(self.result["did_converge"])
+
+    def _destroy(self):
+        if getattr(self, "h", None):
+            self._lib.femhip_solver_destroy(self.h)
+            self.h = None
+
+    def free(self):
+        self._destroy()
+
+    def __del__(self):
+        try:
+            self._destroy()
+        except Exception:
+            pass

@@ -1,0 +1,208 @@
+/*
+ * femhip.h -- C-ABI of the MI355X (gfx950) finite-element hot path.
+ *
+ * One GPU, one subdomain: structured hex mesh -> first-touch DOF numbering ->
+ * cell-wise Galerkin integration -> deterministic coloured scatter-add assembly
+ * into CSR -> (Jacobi-)preconditioned CG, in plain FP64 or FP32-refined form.
+ *
+ * Conventions follow FEMPAR's Fortran side so the ISO_C_BINDING shim passes its
+ * arrays unchanged: DOF ids are 1-based, strong-Dirichlet DOFs carry negative
+ * ids, CSR irp/ja are 1-based, element matrices are column-major.  Host pointers
+ * are borrowed for the duration of a call and never freed by the library;
+ * device state is owned by the handles and released by the *_destroy calls.
+ * Every entry point returns FEMHIP_OK or an error code; femhip_last_error()
+ * gives the message of the last failure on the calling thread.
+ *
+ * Reference interfaces each entry point replaces are cited as file:line
+ * relative to FEMPAR's Sources/Lib.
+ */
+#ifndef FEMHIP_H
+#define FEMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference aborts through check/mcheck -> runend) ---- */
+#define FEMHIP_OK                 0
+#define FEMHIP_ERR_INVALID        1   /* bad argument / unsupported configuration   */
+#define FEMHIP_ERR_HIP            2   /* HIP runtime failure                         */
+#define FEMHIP_ERR_OOM            3   /* device or host allocation failed            */
+#define FEMHIP_ERR_NOT_CONVERGED  4   /* max iterations reached (cg.f90:214)          */
+#define FEMHIP_ERR_ZERO_DIAGONAL  5   /* Jacobi on a zero pivot (serial_scalar_array.f90:585) */
+#define FEMHIP_ERR_STATE          6   /* call out of order (e.g. solve before compute) */
+#define FEMHIP_ERR_ADMISSION      7   /* request exceeds the HBM budget (serving)     */
+#define FEMHIP_ERR_NO_DEVICE      8   /* no HIP device: the library has no CPU path   */
+
+/* ---- enums ---- */
+#define FEMHIP_BC_ALL_BOUNDARY    0   /* set_id = 1 on every boundary vef (test_poisson_driver.f90:193-204) */
+#define FEMHIP_BC_X0_FACE         1   /* set_id = 1 on vefs in the plane x = x_min (clamped face) */
+#define FEMHIP_BC_NONE            2
+
+#define FEMHIP_PHYS_POISSON       0   /* poisson_discrete_integration.f90:62-141 (+ coefficient k) */
+#define FEMHIP_PHYS_ELASTICITY    1   /* sbm_irreducible_discrete_integration.i90:46-134 */
+
+#define FEMHIP_K_CONST            0   /* k = k0 */
+#define FEMHIP_K_SIN3             1   /* k = 1 + 0.5 sin(2 pi x) sin(2 pi y) sin(2 pi z) */
+#define FEMHIP_K_CELL             2   /* k = k_cell[cell], piecewise constant */
+
+#define FEMHIP_F_CONST            0   /* f = f0 */
+#define FEMHIP_F_MANUFACTURED     1   /* f = -div(k grad u), u = prod_d sin(pi x_d) */
+#define FEMHIP_F_CELL_SIN         2   /* f = f0 k_cell d pi^2 prod_d sin(pi x_d) */
+
+#define FEMHIP_G_ZERO             0   /* homogeneous Dirichlet */
+#define FEMHIP_G_LINEAR           1   /* g = x + y (+ z), poisson_analytical_functions.f90:77-104 */
+
+/* stopping criteria, same meaning and order as iterative_solver_parameters.f90:37-46 */
+#define FEMHIP_RES_NRMGIVEN_RHS_NRMGIVEN 0
+#define FEMHIP_RES_NRMGIVEN_RES_NRMGIVEN 1
+#define FEMHIP_DELTA_RHS                 2
+#define FEMHIP_DELTA_DELTA               3
+#define FEMHIP_RES_RES                   4
+#define FEMHIP_RES_RHS                   5
+#define FEMHIP_DELTA_RHS_AND_RES_RES     6
+#define FEMHIP_DELTA_RHS_AND_RES_RHS     7
+#define FEMHIP_DELTA_DELTA_AND_RES_RES   8
+#define FEMHIP_DELTA_DELTA_AND_RES_RHS   9
+
+#define FEMHIP_PREC_IDENTITY      0   /* .identity. operator (operator.f90:39-180) */
+#define FEMHIP_PREC_JACOBI        1   /* jacobi_preconditioner.f90:136-450 */
+
+#define FEMHIP_PRECISION_FP64        0   /* plain drop-in path */
+#define FEMHIP_PRECISION_FP32_REFINED 1  /* FP32 inner PCG inside FP64 iterative refinement */
+
+/* ---- opaque handles ---- */
+typedef struct femhip_ctx femhip_ctx_t;
+typedef struct femhip_space femhip_space_t;
+typedef struct femhip_operator femhip_operator_t;
+typedef struct femhip_solver femhip_solver_t;
+
+/* ---- plain-data descriptors ---- */
+typedef struct {
+  int32_t dim;              /* 2 or 3 (STRUCT_HEX_MESH_GENERATOR_NUM_DIMS) */
+  int32_t n[3];             /* cells per direction (..._NUM_CELLS_X_DIM) */
+  double limits[6];         /* x0 x1 y0 y1 z0 z1 (..._DOMAIN_LIMITS) */
+  int32_t order;            /* Lagrangian order 1 or 2 (FES_REF_FE_ORDERS) */
+  int32_t num_components;   /* 1 scalar, dim for vector (elasticity) */
+  int32_t dirichlet;        /* FEMHIP_BC_* */
+} femhip_mesh_desc_t;
+
+typedef struct {
+  int32_t kind;             /* FEMHIP_PHYS_* */
+  int32_t k_kind;           /* FEMHIP_K_* */
+  double k0;
+  const double *k_cell;     /* host, num_cells values, read during set_physics */
+  int32_t f_kind;           /* FEMHIP_F_* */
+  double f0;
+  int32_t g_kind;           /* FEMHIP_G_* (used by femhip_space_interpolate_dirichlet) */
+  double E, nu;             /* elasticity: lambda = nu E/((1+nu)(1-2nu)), mu = E/(2(1+nu)) */
+  double body[3];           /* elasticity body force */
+} femhip_physics_t;
+
+typedef struct {
+  double rtol;              /* ILS_RTOL (default 1e-6, iterative_solver_parameters.f90:167) */
+  double atol;              /* ILS_ATOL (default 0) */
+  int32_t max_num_iterations; /* ILS_MAX_NUM_ITERATIONS (default 1000) */
+  int32_t stopping_criterion; /* FEMHIP_RES_RES etc. (CG default RES_RES) */
+  int32_t preconditioner;   /* FEMHIP_PREC_* */
+  int32_t precision;        /* FEMHIP_PRECISION_* */
+  double inner_rtol;        /* FP32_REFINED: relative reduction per inner solve (0 = default) */
+  int32_t max_outer;        /* FP32_REFINED: max refinement steps (0 = default) */
+} femhip_cg_params_t;
+
+typedef struct {
+  int32_t num_iterations;   /* CG iterations (inner iterations summed for FP32_REFINED) */
+  int32_t did_converge;
+  int32_t outer_iterations; /* refinement steps (0 for FP64) */
+  int32_t status;           /* FEMHIP_OK or FEMHIP_ERR_NOT_CONVERGED */
+  double error_estimate;    /* last error_estimate_convergence_test */
+  double rhs_convergence_test;
+  double true_residual;     /* ||b - A x||_2 in FP64 after the solve */
+  double rhs_norm;          /* ||b||_2 */
+  double solve_ms;          /* device time of the solve (hipEvents) */
+} femhip_cg_result_t;
+
+/* ---- context ---- */
+/* Opens HIP device `device`, creates the context's stream. */
+int femhip_ctx_create(int32_t device, femhip_ctx_t **ctx);
+int femhip_ctx_destroy(femhip_ctx_t *ctx);
+/* Message of the last failing call on this thread ("" if none). */
+const char *femhip_last_error(void);
+
+/* ---- FE space: serial_fe_space_t on a struct-hex triangulation ----
+ * Replaces serial_triangulation_t%create with the structured generator
+ * (sbm_serial_triangulation.i90:61-118, uniform_hex_mesh_generator.f90:145-647)
+ * + serial_fe_space_t%create: strong Dirichlet numbering
+ * (sbm_serial_fe_space.i90:525-715) and first-touch free numbering
+ * (sbm_serial_fe_space.i90:2623-2922), and uploads the cell DOF lists. */
+int femhip_space_create_structured(femhip_ctx_t *ctx, const femhip_mesh_desc_t *mesh,
+                                   femhip_space_t **space);
+/* Drop-in form: the caller (FEMPAR's fe_space) already numbered the DOFs and
+ * passes lst_dofs_gids (num_cells x nd, cell-major, reference local order,
+ * fe_space.f90:636-637). */
+int femhip_space_create_from_dofs(femhip_ctx_t *ctx, const femhip_mesh_desc_t *mesh,
+                                  const int32_t *lst_dofs_gids, int32_t n_free, int32_t n_fixed,
+                                  femhip_space_t **space);
+int femhip_space_info(const femhip_space_t *space, int64_t *num_cells, int32_t *nd,
+                      int32_t *nq, int32_t *n_free, int32_t *n_fixed);
+int femhip_space_get_dofs(const femhip_space_t *space, int32_t *lst_dofs_gids);
+/* serial_fe_space_t%interpolate_dirichlet_values (sbm_serial_fe_space.i90:997-1176):
+ * nodal interpolation of g (phys->g_kind) into fixed_values (n_fixed). */
+int femhip_space_interpolate_dirichlet(const femhip_space_t *space, const femhip_physics_t *phys,
+                                       double *fixed_values);
+int femhip_space_destroy(femhip_space_t *space);
+
+/* ---- operator: fe_affine_operator_t + sparse_assembler_t + CSR sparse_matrix_t ----
+ * femhip_operator_create replaces fe_affine_operator_t%create with
+ * sparse_matrix_storage 'CSR' (sbm_fe_operator.i90:28-71, 162-308): it builds
+ * the CSR pattern that sort_and_compress would produce
+ * (base_sparse_matrix.f90:4121-4829, csr_sparse_matrix.f90:416-479).
+ * symmetric_storage = diagonal_blocks_symmetric_storage (upper triangle kept). */
+int femhip_operator_create(femhip_ctx_t *ctx, femhip_space_t *space, int32_t symmetric_storage,
+                           femhip_operator_t **op);
+/* n = num_free_dofs; nnz = stored entries (upper if symmetric_storage); nnz_full = solver copy */
+int femhip_operator_info(const femhip_operator_t *op, int32_t *n, int64_t *nnz, int64_t *nnz_full);
+/* csr_sparse_matrix_t%get_irp/get_ja (csr_sparse_matrix.f90:192-225), 1-based */
+int femhip_operator_get_pattern(const femhip_operator_t *op, int32_t *irp, int32_t *ja);
+/* The discrete_integration_t plugin (discrete_integration.f90:44-73): which
+ * integrate_galerkin body runs and with which coefficients. */
+int femhip_operator_set_physics(femhip_operator_t *op, const femhip_physics_t *phys);
+/* fe_affine_operator_t%compute (sbm_fe_affine_operator.i90:15-34): integrate
+ * all cells, eliminate strong Dirichlet columns into the RHS with the given
+ * fixed values (n_fixed, may be NULL for homogeneous), scatter-add into the
+ * device CSR values and RHS.  Device-resident; elapsed device ms in *ms (may be NULL). */
+int femhip_operator_compute(femhip_operator_t *op, const double *fixed_values, double *ms);
+/* get_matrix()%val and get_translation() (sbm_fe_operator.i90:499-513) copied to host */
+int femhip_operator_get_values(const femhip_operator_t *op, double *val, double *rhs);
+/* sparse_matrix_t%apply (sparse_matrix.f90:1264-1276): y = A x, host vectors */
+int femhip_operator_apply(femhip_operator_t *op, const double *x, double *y);
+int femhip_operator_destroy(femhip_operator_t *op);
+
+/* ---- iterative_linear_solver_t of type 'CG' ----
+ * Replaces iterative_linear_solver_t%create/set_type_from_string('CG')/
+ * set_parameters_from_pl/set_operators (iterative_linear_solver.f90:62-241,
+ * base_iterative_linear_solver.f90:263-296, 714-774). */
+int femhip_solver_create(femhip_ctx_t *ctx, femhip_operator_t *op, const femhip_cg_params_t *params,
+                         femhip_solver_t **solver);
+/* iterative_linear_solver_t%solve(b, x) -> cg_solve_body (cg.f90:101-238).
+ * b == NULL uses the operator's assembled RHS (device-resident); x is the host
+ * initial guess on entry and the solution on exit (x == NULL: start from 0 and
+ * keep the solution on the device). */
+int femhip_solver_solve(femhip_solver_t *solver, const double *b, double *x, femhip_cg_result_t *res);
+/* Measurement aid (the reference's timer_t, Sources/Lib/timer.f90:42-207, has
+ * no kernel-level counterpart): launches the solver's SpMV kernel `reps` times
+ * on the context stream and reports the average launch duration from HIP
+ * events and the kernel's algorithmic HBM bytes per launch (matrix stream +
+ * row pointers + one read of p and one write of q). */
+int femhip_solver_profile_spmv(femhip_solver_t *solver, int32_t reps, double *ms_per_launch,
+                               double *bytes_per_launch);
+/* copy the last solution to the host (n values) */
+int femhip_solver_get_solution(const femhip_solver_t *solver, double *x);
+int femhip_solver_destroy(femhip_solver_t *solver);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

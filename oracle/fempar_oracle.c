@@ -1,0 +1,899 @@
+/*
+ * fempar_oracle.c -- CPU restatement of FEMPAR's one-subdomain FE hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see fempar_oracle.h): the checker for the HIP
+ * product path, never the product.  Plain C99, single thread, reference loop
+ * order throughout.  File:line citations are relative to the reference's
+ * Sources/Lib unless they start with Sources/.
+ */
+#include "fempar_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAXNF 27          /* n-faces of a hexahedron incl. the cell itself */
+#define MAXND 81          /* Q2 scalar (27) or vector Q1 3D (24); Q2 vector 3D = 81 */
+#define MAXNQ 27
+
+static void *xmalloc(size_t n) {
+  void *p = malloc(n ? n : 1);
+  if (!p) { fprintf(stderr, "oracle: out of memory (%zu bytes)\n", n); abort(); }
+  return p;
+}
+static void *xcalloc(size_t n, size_t sz) {
+  void *p = calloc(n ? n : 1, sz ? sz : 1);
+  if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+  return p;
+}
+void orc_free(void *p) { free(p); }
+
+static int popcount(int v) { int c = 0; while (v) { c += v & 1; v >>= 1; } return c; }
+
+struct orc_space {
+  orc_problem_t p;
+  int d;
+  /* polytope: n-faces sorted by (dimension, bit value) -- sbm_polytope.i90:175-227 */
+  int nnf;                 /* number of n-faces incl. the cell */
+  int nf_type[MAXNF];      /* free-direction mask */
+  int nf_anchor[MAXNF];    /* anchor-vertex bits */
+  int nf_vtype[MAXNF];     /* mesh vef type index (uniform_hex_mesh_generator.f90:386-391) */
+  int ntypes;
+  int64_t type_off[8];
+  int64_t type_cnt[8][3];
+  int64_t num_vefs;
+  /* reference FE (lagrangian_reference_fe_fill_scalar, sbm_lagrangian_reference_fe.i90:1549-1760) */
+  int order, np1, nnode, ncomp, nd;
+  int own_ptr[MAXNF + 1];
+  int own_lst[MAXND];
+  /* quadrature + tables */
+  int nq1, nq;
+  double qp[3][MAXNQ];     /* quadrature coordinates (dim x nq) */
+  double qw[MAXNQ];
+  double shp[MAXND][MAXNQ];          /* scalar shape values N_a(q) */
+  double dshp[MAXND][3][MAXNQ];      /* reference gradients dN_a/dxi_j (q) */
+  double gshp[8][MAXNQ];             /* geometric Q1 shape values */
+  double gdshp[8][3][MAXNQ];         /* geometric Q1 gradients */
+  /* affine map of the first cell, reused by translation (sbm_fe_cell_iterator.i90:970-987) */
+  double detj[MAXNQ];
+  double invj[MAXNQ][3][3];
+  double grad[MAXND][3][MAXNQ];      /* physical scalar gradients (sbm_lagrangian_reference_fe.i90:1334-1341) */
+  /* space */
+  int64_t ncells;
+  int32_t *dofs;           /* ncells x nd, signed 1-based */
+  int nfree, nfixed;
+};
+
+/* ------------------------------------------------------------------ mesh --- */
+
+static void cell_ijk(const orc_space_t *s, int64_t c, int ijk[3]) {
+  /* spatial_to_ijk_numbering, x fastest (uniform_hex_mesh_generator.f90:801-815) */
+  int64_t j = c;
+  ijk[0] = ijk[1] = ijk[2] = 0;
+  for (int d = 0; d < s->d; ++d) { ijk[d] = (int)(j % s->p.n[d]); j /= s->p.n[d]; }
+}
+
+static double vertex_coord(const orc_space_t *s, int d, int i) {
+  /* coordinates = i/n then mapped to the domain limits (uniform_hex_mesh_generator.f90:608-612, 633-635) */
+  double c = (double)i / (double)s->p.n[d];
+  return (s->p.lim[2 * d + 1] - s->p.lim[2 * d]) * c + s->p.lim[2 * d];
+}
+
+static int64_t vef_gid(const orc_space_t *s, const int ijk[3], int f) {
+  /* lst_vefs_lids (uniform_hex_mesh_generator.f90:441-456), 0-based */
+  int t = s->nf_vtype[f];
+  int64_t idx = 0, mul = 1;
+  for (int d = 0; d < s->d; ++d) {
+    int v = ijk[d] + ((s->nf_anchor[f] >> d) & 1);
+    idx += mul * v;
+    mul *= s->type_cnt[t][d];
+  }
+  return s->type_off[t] + idx;
+}
+
+static int vef_set_id(const orc_space_t *s, const int ijk[3], int f) {
+  /* boundary detection (sbm_fine_triangulation.i90:453-527): a vef is at the boundary
+     when it lies on a facet with one cell around, i.e. a fixed direction sits at 0 or n.
+     test_poisson_driver.f90:193-204 then sets set_id = 1 on boundary vefs. */
+  if (s->p.bc_kind == ORC_BC_NONE) return 0;
+  for (int d = 0; d < s->d; ++d) {
+    if ((s->nf_type[f] >> d) & 1) continue;           /* free direction */
+    int v = ijk[d] + ((s->nf_anchor[f] >> d) & 1);
+    if (s->p.bc_kind == ORC_BC_ALL_BOUNDARY) {
+      if (v == 0 || v == s->p.n[d]) return 1;
+    } else if (s->p.bc_kind == ORC_BC_X0_FACE) {
+      if (d == 0 && v == 0) return 1;
+    }
+  }
+  return 0;
+}
+
+static void build_polytope(orc_space_t *s) {
+  int d = s->d, nn = 0;
+  int val[MAXNF], typ[MAXNF], anc[MAXNF];
+  for (int t = 0; t < (1 << d); ++t)
+    for (int a = 0; a < (1 << d); ++a)
+      if ((a & t) == 0) { typ[nn] = t; anc[nn] = a; val[nn] = (t << d) | a; ++nn; }
+  /* sort_n_faces: by n-face dimension, then by bit-based index (sbm_polytope.i90:206-217) */
+  for (int i = 1; i < nn; ++i) {
+    int k = i;
+    while (k > 0) {
+      int dk = popcount(typ[k]), dp = popcount(typ[k - 1]);
+      if (dk < dp || (dk == dp && val[k] < val[k - 1])) {
+        int tv = val[k]; val[k] = val[k - 1]; val[k - 1] = tv;
+        int tt = typ[k]; typ[k] = typ[k - 1]; typ[k - 1] = tt;
+        int ta = anc[k]; anc[k] = anc[k - 1]; anc[k - 1] = ta;
+        --k;
+      } else break;
+    }
+  }
+  s->nnf = nn;
+  for (int i = 0; i < nn; ++i) { s->nf_type[i] = typ[i]; s->nf_anchor[i] = anc[i]; }
+  /* vef types: n-faces of dim < d with anchor 0, in polytope order
+     (uniform_hex_mesh_generator.f90:386-421) */
+  int it = -1;
+  int64_t off = 0;
+  for (int i = 0; i < nn; ++i) {
+    if (popcount(typ[i]) >= d) { s->nf_vtype[i] = -1; continue; }
+    if (anc[i] == 0) {
+      ++it;
+      int64_t cnt = 1;
+      for (int k = 0; k < d; ++k) {
+        s->type_cnt[it][k] = s->p.n[k] + 1 - ((typ[i] >> k) & 1);
+        cnt *= s->type_cnt[it][k];
+      }
+      s->type_off[it] = off;
+      off += cnt;
+    }
+    s->nf_vtype[i] = it;
+  }
+  s->ntypes = it + 1;
+  s->num_vefs = off;
+}
+
+/* ---------------------------------------------------- reference FE tables --- */
+
+void orc_gauss_legendre(int n, double *x, double *w) {
+  /* Gauss-Legendre abscissae by Newton iteration on the three-term recurrence,
+     ascending; weights 2/((1-x)(1+x)P_n'(x)^2) renormalised to sum 2 exactly as
+     legendre_compute_glr does (sbm_quadrature.i90:857-862).  The reference reaches
+     the same roots with the Glaser-Liu-Rokhlin ODE method; the two agree to
+     rounding (pinned against test_quadratures.f90 tables). */
+  const double pi = 3.14159265358979323846;
+  double *dp = (double *)xmalloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    double z = -cos(pi * (i + 0.75) / (n + 0.5));
+    double pp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      if (n == 0) p1 = 1.0;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1; p1 = p2;
+      }
+      double pn = (n == 1) ? z : p1, pnm1 = (n == 1) ? 1.0 : p0;
+      pp = n * (z * pn - pnm1) / (z * z - 1.0);
+      double dz = pn / pp;
+      z -= dz;
+      if (fabs(dz) < 1e-17) break;
+    }
+    {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1; p1 = p2;
+      }
+      double pn = (n == 1) ? z : p1, pnm1 = (n == 1) ? 1.0 : p0;
+      pp = n * (z * pn - pnm1) / (z * z - 1.0);
+    }
+    if (n % 2 == 1 && i == n / 2) z = 0.0;
+    x[i] = z;
+    dp[i] = pp;
+  }
+  /* symmetrise: exact antisymmetry of the rule */
+  for (int i = 0; i < n / 2; ++i) {
+    double a = 0.5 * (x[n - 1 - i] - x[i]);
+    x[i] = -a; x[n - 1 - i] = a;
+    double b = 0.5 * (fabs(dp[i]) + fabs(dp[n - 1 - i]));
+    dp[i] = b; dp[n - 1 - i] = b;
+  }
+  double sum = 0.0;
+  for (int i = 0; i < n; ++i) {
+    w[i] = 2.0 / (1.0 - x[i]) / (1.0 + x[i]) / dp[i] / dp[i];
+    sum += w[i];
+  }
+  for (int i = 0; i < n; ++i) w[i] = 2.0 * w[i] / sum;
+  free(dp);
+}
+
+static void lagrange_1d(int order, int i, double x, double *v, double *dv) {
+  /* lagrange_polynomial_generate_basis + get_values (sbm_polynomial.i90:61-111):
+     l_i(x) = c_i * prod_{k != i} (x - t_k), t equispaced on [-1,1], with the
+     value/derivative recurrence of get_values. */
+  double t[8];
+  for (int k = 0; k <= order; ++k) t[k] = (double)k;
+  if (order > 0) for (int k = 0; k <= order; ++k) t[k] = (2.0 / order) * t[k] - 1.0;
+  double coef[8];
+  int m = 0;
+  for (int k = 0; k <= order; ++k) if (k != i) coef[m++] = t[k];
+  double c = 1.0;
+  for (int k = 0; k <= order; ++k) if (k != i) c = c * (t[i] - t[k]);
+  c = 1.0 / c;
+  double px[3] = {1.0, 0.0, 0.0};
+  for (int k = 0; k < order; ++k) {
+    for (int j = 2; j >= 1; --j) px[j] = px[j] * (x - coef[k]) + px[j - 1];
+    px[0] = px[0] * (x - coef[k]);
+  }
+  *v = px[0] * c;
+  *dv = px[1] * c;
+}
+
+static void build_reffe(orc_space_t *s) {
+  int d = s->d;
+  s->np1 = s->order + 1;
+  s->nnode = 1;
+  for (int k = 0; k < d; ++k) s->nnode *= s->np1;
+  s->nd = s->nnode * s->ncomp;
+  /* own DOFs per n-face: the n-face of node m is (free dirs = interior coords,
+     anchor = coords at the upper end); own nodes in node order; vector fields
+     extend component-major (lagrangian_reference_fe_extend_list_components,
+     sbm_lagrangian_reference_fe.i90:1250-1286). */
+  int node_nf[MAXND];
+  for (int a = 0; a < s->nnode; ++a) {
+    int rem = a, typ = 0, anc = 0;
+    for (int k = 0; k < d; ++k) {
+      int mk = rem % s->np1; rem /= s->np1;
+      if (mk > 0 && mk < s->order) typ |= 1 << k;
+      else if (mk == s->order) anc |= 1 << k;
+    }
+    node_nf[a] = -1;
+    for (int f = 0; f < s->nnf; ++f)
+      if (s->nf_type[f] == typ && s->nf_anchor[f] == anc) node_nf[a] = f;
+  }
+  int cnt = 0;
+  for (int f = 0; f < s->nnf; ++f) {
+    s->own_ptr[f] = cnt;
+    for (int c = 0; c < s->ncomp; ++c)
+      for (int a = 0; a < s->nnode; ++a)
+        if (node_nf[a] == f) s->own_lst[cnt++] = c * s->nnode + a;
+  }
+  s->own_ptr[s->nnf] = cnt;
+
+  /* quadrature: degree 2*order -> order+1 points per direction
+     (sbm_lagrangian_reference_fe.i90:1414-1424, sbm_hex_lagrangian_reference_fe.i90:320-328),
+     first index fastest (sbm_quadrature.i90:797-815) */
+  s->nq1 = s->order + 1;
+  double x1[8], w1[8];
+  orc_gauss_legendre(s->nq1, x1, w1);
+  s->nq = 1;
+  for (int k = 0; k < d; ++k) s->nq *= s->nq1;
+  for (int q = 0; q < s->nq; ++q) {
+    int rem = q;
+    double w = 1.0;
+    for (int k = 0; k < 3; ++k) s->qp[k][q] = 0.0;
+    int id[3] = {0, 0, 0};
+    for (int k = 0; k < d; ++k) { id[k] = rem % s->nq1; rem /= s->nq1; }
+    for (int k = 0; k < d; ++k) { s->qp[k][q] = x1[id[k]]; w = (k == 0) ? w1[id[0]] : w * w1[id[k]]; }
+    s->qw[q] = w;
+  }
+  /* tensor-product values/gradients (sbm_tensor_product_polynomial.i90:63-127) */
+  for (int q = 0; q < s->nq; ++q) {
+    for (int a = 0; a < s->nnode; ++a) {
+      int rem = a, id[3] = {0, 0, 0};
+      for (int k = 0; k < d; ++k) { id[k] = rem % s->np1; rem /= s->np1; }
+      double v[3], dv[3];
+      for (int k = 0; k < d; ++k) lagrange_1d(s->order, id[k], s->qp[k][q], &v[k], &dv[k]);
+      double val = 1.0;
+      for (int k = 0; k < d; ++k) val = val * v[k];
+      s->shp[a][q] = val;
+      for (int i = 0; i < 3; ++i) s->dshp[a][i][q] = 0.0;
+      for (int i = 0; i < d; ++i) {
+        double g = dv[i];
+        for (int j = 0; j < d; ++j) if (j != i) g = g * v[j];
+        s->dshp[a][i][q] = g;
+      }
+    }
+    for (int a = 0; a < (1 << d); ++a) {
+      double v[3], dv[3];
+      for (int k = 0; k < d; ++k) lagrange_1d(1, (a >> k) & 1, s->qp[k][q], &v[k], &dv[k]);
+      double val = 1.0;
+      for (int k = 0; k < d; ++k) val = val * v[k];
+      s->gshp[a][q] = val;
+      for (int i = 0; i < 3; ++i) s->gdshp[a][i][q] = 0.0;
+      for (int i = 0; i < d; ++i) {
+        double g = dv[i];
+        for (int j = 0; j < d; ++j) if (j != i) g = g * v[j];
+        s->gdshp[a][i][q] = g;
+      }
+    }
+  }
+}
+
+static void cell_vertices(const orc_space_t *s, int64_t c, double xv[8][3]) {
+  int ijk[3];
+  cell_ijk(s, c, ijk);
+  for (int a = 0; a < (1 << s->d); ++a) {
+    for (int k = 0; k < 3; ++k) xv[a][k] = 0.0;
+    for (int k = 0; k < s->d; ++k) xv[a][k] = vertex_coord(s, k, ijk[k] + ((a >> k) & 1));
+  }
+}
+
+static void build_map(orc_space_t *s) {
+  /* cell_map_update on the first cell (sbm_cell_map.i90:245-300): Jacobian as the
+     linear combination of geometric gradients and node coordinates
+     (field.f90:763-787), inverse and determinant by invmtx
+     (sbm_cell_integrator.i90:448-560); afterwards every cell is "only translated". */
+  double xv[8][3];
+  cell_vertices(s, 0, xv);
+  int d = s->d;
+  for (int q = 0; q < s->nq; ++q) {
+    double J[3][3] = {{0}};
+    for (int j = 0; j < 3; ++j) {
+      for (int l = 0; l < 3; ++l) J[l][j] = 0.0;
+      for (int k = 0; k < (1 << d); ++k) {
+        double alpha = s->gdshp[k][j][q];
+        for (int l = 0; l < 3; ++l) J[l][j] = J[l][j] + alpha * xv[k][l];
+      }
+    }
+    double B[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double det;
+    if (d == 2) {
+      det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+      double den = 1.0 / det;
+      B[0][0] = J[1][1] * den; B[1][1] = J[0][0] * den;
+      B[1][0] = -J[1][0] * den; B[0][1] = -J[0][1] * den;
+    } else {
+      double t1 = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+      double t2 = -J[1][0] * J[2][2] + J[2][0] * J[1][2];
+      double t3 = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+      det = J[0][0] * t1 + J[0][1] * t2 + J[0][2] * t3;
+      double den = 1.0 / det;
+      B[0][0] = t1 * den; B[1][0] = t2 * den; B[2][0] = t3 * den;
+      B[1][1] = (J[0][0] * J[2][2] - J[2][0] * J[0][2]) * den;
+      B[2][1] = (-J[0][0] * J[2][1] + J[0][1] * J[2][0]) * den;
+      B[2][2] = (J[0][0] * J[1][1] - J[1][0] * J[0][1]) * den;
+      B[0][1] = (-J[0][1] * J[2][2] + J[2][1] * J[0][2]) * den;
+      B[0][2] = (J[0][1] * J[1][2] - J[1][1] * J[0][2]) * den;
+      B[1][2] = (-J[0][0] * J[1][2] + J[1][0] * J[0][2]) * den;
+    }
+    s->detj[q] = fabs(det);
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) s->invj[q][a][b] = B[a][b];
+    /* phy(i) += invJ(j,i) * ref(j)  (sbm_lagrangian_reference_fe.i90:1334-1341) */
+    for (int a = 0; a < s->nnode; ++a)
+      for (int i = 0; i < 3; ++i) {
+        double g = 0.0;
+        for (int j = 0; j < 3; ++j) g = g + B[j][i] * s->dshp[a][j][q];
+        s->grad[a][i][q] = g;
+      }
+  }
+}
+
+/* ------------------------------------------------------- DOF numbering --- */
+
+static void number_dofs(orc_space_t *s) {
+  int nd = s->nd, nvef = s->nnf - 1;
+  s->dofs = (int32_t *)xcalloc((size_t)(s->ncells * nd), sizeof(int32_t));
+  int64_t *vis_cell = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)s->num_vefs);
+  int8_t *vis_lid = (int8_t *)xmalloc((size_t)s->num_vefs);
+  for (int64_t g = 0; g < s->num_vefs; ++g) vis_cell[g] = -1;
+
+  /* strong Dirichlet sweep (sbm_serial_fe_space.i90:525-715): cells in order,
+     vefs in polytope order, negative ids on first visit, fetched afterwards. */
+  int ndir = 0;
+  int ijk[3];
+  for (int64_t c = 0; c < s->ncells; ++c) {
+    cell_ijk(s, c, ijk);
+    int32_t *cd = s->dofs + c * nd;
+    for (int f = 0; f < nvef; ++f) {
+      if (vef_set_id(s, ijk, f) == 0) continue;
+      int64_t g = vef_gid(s, ijk, f);
+      int prev = ndir;
+      if (vis_cell[g] == -1) {
+        /* generate_own_dofs_vef_component_wise: all components coded as fixed */
+        for (int k = s->own_ptr[f]; k < s->own_ptr[f + 1]; ++k) {
+          int l = s->own_lst[k];
+          if (cd[l] >= 0) cd[l] = -(++ndir);
+        }
+      } else {
+        /* fetch_own_dofs_vef_from_source_fe: one node per n-face for Q1/Q2 */
+        const int32_t *src = s->dofs + vis_cell[g] * nd;
+        int fs = vis_lid[g];
+        for (int k = 0; k < s->own_ptr[f + 1] - s->own_ptr[f]; ++k)
+          cd[s->own_lst[s->own_ptr[f] + k]] = src[s->own_lst[s->own_ptr[fs] + k]];
+      }
+      if (vis_cell[g] == -1 && prev < ndir) { vis_cell[g] = c; vis_lid[g] = (int8_t)f; }
+    }
+  }
+  s->nfixed = ndir;
+
+  /* free DOFs (serial_fe_space_list_dofs, sbm_serial_fe_space.i90:2623-2922): cell
+     interior first, then vefs; first visit numbers non-fixed own DOFs. */
+  for (int64_t g = 0; g < s->num_vefs; ++g) vis_cell[g] = -1;
+  int cur = 0;
+  for (int64_t c = 0; c < s->ncells; ++c) {
+    cell_ijk(s, c, ijk);
+    int32_t *cd = s->dofs + c * nd;
+    int fc = s->nnf - 1;
+    for (int k = s->own_ptr[fc]; k < s->own_ptr[fc + 1]; ++k) cd[s->own_lst[k]] = ++cur;
+    for (int f = 0; f < nvef; ++f) {
+      int64_t g = vef_gid(s, ijk, f);
+      if (vis_cell[g] == -1) {
+        int prev = cur;
+        for (int k = s->own_ptr[f]; k < s->own_ptr[f + 1]; ++k) {
+          int l = s->own_lst[k];
+          if (cd[l] >= 0) cd[l] = ++cur;
+        }
+        if (prev < cur) { vis_cell[g] = c; vis_lid[g] = (int8_t)f; }
+      } else {
+        const int32_t *src = s->dofs + vis_cell[g] * nd;
+        int fs = vis_lid[g];
+        for (int k = 0; k < s->own_ptr[f + 1] - s->own_ptr[f]; ++k)
+          cd[s->own_lst[s->own_ptr[f] + k]] = src[s->own_lst[s->own_ptr[fs] + k]];
+      }
+    }
+  }
+  s->nfree = cur;
+  free(vis_cell);
+  free(vis_lid);
+}
+
+orc_space_t *orc_space_create(const orc_problem_t *prob) {
+  if (prob->dim != 2 && prob->dim != 3) return NULL;
+  if (prob->order < 1 || prob->order > 2) return NULL;
+  if (prob->ncomp != 1 && prob->ncomp != prob->dim) return NULL;
+  orc_space_t *s = (orc_space_t *)xcalloc(1, sizeof(orc_space_t));
+  s->p = *prob;
+  s->d = prob->dim;
+  if (s->d == 2) { s->p.n[2] = 1; s->p.lim[4] = 0.0; s->p.lim[5] = 0.0; }
+  s->order = prob->order;
+  s->ncomp = prob->ncomp;
+  s->ncells = 1;
+  for (int k = 0; k < s->d; ++k) s->ncells *= prob->n[k];
+  build_polytope(s);
+  build_reffe(s);
+  build_map(s);
+  number_dofs(s);
+  return s;
+}
+
+void orc_space_free(orc_space_t *s) {
+  if (!s) return;
+  free(s->dofs);
+  free(s);
+}
+
+void orc_space_sizes(const orc_space_t *s, int64_t *num_cells, int *nd, int *nq, int *n_free,
+                     int *n_fixed) {
+  *num_cells = s->ncells; *nd = s->nd; *nq = s->nq; *n_free = s->nfree; *n_fixed = s->nfixed;
+}
+
+void orc_space_get_dofs(const orc_space_t *s, int32_t *out) {
+  memcpy(out, s->dofs, sizeof(int32_t) * (size_t)(s->ncells * s->nd));
+}
+
+void orc_reference_tables(const orc_space_t *s, double *qpts, double *qw, double *shape,
+                          double *dshape) {
+  for (int q = 0; q < s->nq; ++q) {
+    for (int k = 0; k < s->d; ++k) qpts[q * s->d + k] = s->qp[k][q];
+    qw[q] = s->qw[q];
+    for (int a = 0; a < s->nnode; ++a) {
+      shape[q * s->nnode + a] = s->shp[a][q];
+      for (int k = 0; k < s->d; ++k) dshape[(q * s->nnode + a) * s->d + k] = s->dshp[a][k][q];
+    }
+  }
+}
+
+/* --------------------------------------------------------- physics ------ */
+
+static const double PI = 3.14159265358979323846;
+
+static double k_value(const orc_physics_t *ph, int d, const double x[3], int64_t cell) {
+  switch (ph->k_kind) {
+    case ORC_K_SIN3: {
+      double v = 1.0 + 0.5 * sin(2 * PI * x[0]) * sin(2 * PI * x[1]) * (d == 3 ? sin(2 * PI * x[2]) : 1.0);
+      return v;
+    }
+    case ORC_K_CELL: return ph->k_cell[cell];
+    default: return ph->k0;
+  }
+}
+
+static double f_value(const orc_physics_t *ph, int d, const double x[3], int64_t cell) {
+  switch (ph->f_kind) {
+    case ORC_F_MANUFACTURED: {
+      /* f = -div(k grad u) = k * d pi^2 u - grad k . grad u, u = prod sin(pi x_k) */
+      double su[3], cu[3];
+      for (int k = 0; k < 3; ++k) { su[k] = (k < d) ? sin(PI * x[k]) : 1.0; cu[k] = (k < d) ? cos(PI * x[k]) : 0.0; }
+      double u = su[0] * su[1] * su[2];
+      double kk = k_value(ph, d, x, cell);
+      double f = kk * d * PI * PI * u;
+      if (ph->k_kind == ORC_K_SIN3) {
+        double s2[3], c2[3];
+        for (int k = 0; k < 3; ++k) { s2[k] = (k < d) ? sin(2 * PI * x[k]) : 1.0; c2[k] = (k < d) ? cos(2 * PI * x[k]) : 0.0; }
+        double gk[3], gu[3];
+        gk[0] = PI * c2[0] * s2[1] * s2[2];
+        gk[1] = PI * s2[0] * c2[1] * s2[2];
+        gk[2] = PI * s2[0] * s2[1] * c2[2];
+        gu[0] = PI * cu[0] * su[1] * su[2];
+        gu[1] = PI * su[0] * cu[1] * su[2];
+        gu[2] = PI * su[0] * su[1] * cu[2];
+        double dot = 0.0;
+        for (int k = 0; k < d; ++k) dot += gk[k] * gu[k];
+        f -= dot;
+      }
+      return f;
+    }
+    case ORC_F_CELL_SIN: {
+      double u = 1.0;
+      for (int k = 0; k < d; ++k) u *= sin(PI * x[k]);
+      return ph->f0 * k_value(ph, d, x, cell) * d * PI * PI * u;
+    }
+    default: return ph->f0;
+  }
+}
+
+static double g_value(const orc_physics_t *ph, int d, const double x[3], int comp) {
+  (void)comp;
+  if (ph->g_kind == ORC_G_LINEAR) {
+    double v = 0.0;
+    for (int k = 0; k < d; ++k) v += x[k];
+    return v;
+  }
+  return 0.0;
+}
+
+static void qpoint_coords(const orc_space_t *s, int64_t c, double xq[MAXNQ][3]) {
+  /* cell_map_compute_quadrature_points_coordinates (sbm_cell_map.i90:673-687) */
+  double xv[8][3];
+  cell_vertices(s, c, xv);
+  for (int q = 0; q < s->nq; ++q) {
+    for (int k = 0; k < 3; ++k) xq[q][k] = 0.0;
+    for (int a = 0; a < (1 << s->d); ++a)
+      for (int k = 0; k < 3; ++k) xq[q][k] = xq[q][k] + xv[a][k] * s->gshp[a][q];
+  }
+}
+
+void orc_element(const orc_space_t *s, const orc_physics_t *ph, int64_t c, double *elmat,
+                 double *elvec) {
+  int nd = s->nd, nn = s->nnode, d = s->d;
+  double xq[MAXNQ][3];
+  qpoint_coords(s, c, xq);
+  for (int i = 0; i < nd * nd; ++i) elmat[i] = 0.0;
+  for (int i = 0; i < nd; ++i) elvec[i] = 0.0;
+  if (ph->kind == ORC_PHYS_POISSON) {
+    /* poisson_discrete_integration.f90:116-130 (q outer, i, j), with the
+       coefficient at the quadrature point as in the heterogeneous-coefficient
+       plugin of par_test_pb_bddc_poisson */
+    for (int q = 0; q < s->nq; ++q) {
+      double factor = s->detj[q] * s->qw[q];
+      double kq = k_value(ph, d, xq[q], c);
+      for (int i = 0; i < nd; ++i)
+        for (int j = 0; j < nd; ++j) {
+          double dot = 0.0;
+          for (int k = 0; k < 3; ++k) dot += s->grad[j][k][q] * s->grad[i][k][q];
+          elmat[i + j * nd] = elmat[i + j * nd] + factor * kq * dot;
+        }
+      double fq = f_value(ph, d, xq[q], c);
+      for (int i = 0; i < nd; ++i) elvec[i] = elvec[i] + factor * fq * s->shp[i][q];
+    }
+  } else {
+    /* sbm_irreducible_discrete_integration.i90:64-130: sigma = lambda tr(eps) I + 2 mu eps,
+       elmat(i,j) += dV * grad(phi_i) : sigma(phi_j); elvec(i) += dV * f . phi_i */
+    double E = ph->E, nu = ph->nu;
+    double lambda = (nu * E) / ((1 + nu) * (1 - 2 * nu));
+    double mu = E / (2 * (1 + nu));
+    for (int q = 0; q < s->nq; ++q) {
+      double dV = s->detj[q] * s->qw[q];
+      for (int i = 0; i < nd; ++i) {
+        int ci = i / nn, ai = i % nn;
+        for (int j = 0; j < nd; ++j) {
+          int cj = j / nn, aj = j % nn;
+          /* grad(phi_j)(r,s) = e_cj(r) * dN_aj/dx_s ; eps = sym part */
+          double G[3][3] = {{0}}, eps[3][3], sig[3][3];
+          for (int k = 0; k < 3; ++k) G[cj][k] = s->grad[aj][k][q];
+          for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) eps[a][b] = 0.5 * (G[a][b] + G[b][a]);
+          double tr = eps[0][0] + eps[1][1] + eps[2][2];
+          for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) sig[a][b] = lambda * tr * (a == b ? 1.0 : 0.0) + 2 * mu * eps[a][b];
+          double dc = 0.0;
+          for (int k = 0; k < 3; ++k) dc += s->grad[ai][k][q] * sig[ci][k];
+          elmat[i + j * nd] = elmat[i + j * nd] + dV * dc;
+        }
+        elvec[i] = elvec[i] + dV * ph->body[ci] * s->shp[ai][q];
+      }
+    }
+  }
+}
+
+void orc_space_fixed_values(const orc_space_t *s, const orc_physics_t *ph, double *out) {
+  /* nodal interpolation of g at the physical node positions (cell map of the
+     reference Lagrange nodes), sbm_serial_fe_space.i90:997-1176 */
+  int nd = s->nd, nn = s->nnode, d = s->d;
+  for (int i = 0; i < s->nfixed; ++i) out[i] = 0.0;
+  for (int64_t c = 0; c < s->ncells; ++c) {
+    const int32_t *cd = s->dofs + c * nd;
+    int any = 0;
+    for (int l = 0; l < nd; ++l) if (cd[l] < 0) { any = 1; break; }
+    if (!any) continue;
+    double xv[8][3];
+    cell_vertices(s, c, xv);
+    for (int l = 0; l < nd; ++l) {
+      if (cd[l] >= 0) continue;
+      int a = l % nn, comp = l / nn, rem = a;
+      double xi[3] = {0, 0, 0};
+      for (int k = 0; k < d; ++k) { int m = rem % s->np1; rem /= s->np1; xi[k] = (s->order > 0) ? (2.0 / s->order) * m - 1.0 : 0.0; }
+      double x[3] = {0, 0, 0};
+      for (int v = 0; v < (1 << d); ++v) {
+        double N = 1.0;
+        for (int k = 0; k < d; ++k) N *= ((v >> k) & 1) ? 0.5 * (1.0 + xi[k]) : 0.5 * (1.0 - xi[k]);
+        for (int k = 0; k < 3; ++k) x[k] += xv[v][k] * N;
+      }
+      out[-cd[l] - 1] = g_value(ph, d, x, comp);
+    }
+  }
+}
+
+/* ------------------------------------------------- assembly + compress --- */
+
+typedef struct { int32_t *ia, *ja; double *val; int64_t nnz, cap; } coo_t;
+
+static void coo_append(coo_t *m, int32_t i, int32_t j, double v) {
+  if (m->nnz == m->cap) {
+    /* storage grows x1.5 (base_sparse_matrix.f90:3491-3496) */
+    int64_t nc = m->cap < 16 ? 64 : (int64_t)(1.5 * (double)m->cap);
+    m->ia = (int32_t *)realloc(m->ia, sizeof(int32_t) * (size_t)nc);
+    m->ja = (int32_t *)realloc(m->ja, sizeof(int32_t) * (size_t)nc);
+    m->val = (double *)realloc(m->val, sizeof(double) * (size_t)nc);
+    if (!m->ia || !m->ja || !m->val) { fprintf(stderr, "oracle: COO out of memory\n"); abort(); }
+    m->cap = nc;
+  }
+  m->ia[m->nnz] = i; m->ja[m->nnz] = j; m->val[m->nnz] = v; ++m->nnz;
+}
+
+int64_t orc_assemble(const orc_space_t *s, const orc_physics_t *ph, int sym,
+                     const double *fixed, int32_t **irp_out, int32_t **ja_out, double **val_out,
+                     double **rhs_out) {
+  return orc_assemble_cells(s, ph, sym, fixed, 0, s->ncells, irp_out, ja_out, val_out, rhs_out);
+}
+
+int64_t orc_assemble_cells(const orc_space_t *s, const orc_physics_t *ph, int sym,
+                           const double *fixed, int64_t cell_begin, int64_t cell_end, int32_t **irp_out,
+                           int32_t **ja_out, double **val_out, double **rhs_out) {
+  int nd = s->nd, n = s->nfree;
+  double *elmat = (double *)xmalloc(sizeof(double) * (size_t)(nd * nd));
+  double *elvec = (double *)xmalloc(sizeof(double) * (size_t)nd);
+  double *rhs = (double *)xcalloc((size_t)n, sizeof(double));
+  coo_t m = {0};
+  int64_t per = 0;
+  for (int i = 0; i < nd; ++i) for (int j = 0; j < nd; ++j) if (!sym || i <= j) ++per;
+  if (cell_begin < 0) cell_begin = 0;
+  if (cell_end > s->ncells) cell_end = s->ncells;
+  m.cap = (cell_end - cell_begin) * (sym ? per : (int64_t)nd * nd);
+  if (m.cap < 64) m.cap = 64;
+  m.ia = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)m.cap);
+  m.ja = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)m.cap);
+  m.val = (double *)xmalloc(sizeof(double) * (size_t)m.cap);
+
+  for (int64_t c = cell_begin; c < cell_end; ++c) {
+    orc_element(s, ph, c, elmat, elvec);
+    const int32_t *g = s->dofs + c * nd;
+    /* impose_strong_dirichlet_bcs_conforming (sbm_fe_cell_iterator.i90:1610-1637) */
+    if (fixed) {
+      for (int j = 0; j < nd; ++j)
+        if (g[j] < 0) {
+          double gv = fixed[-g[j] - 1];
+          for (int i = 0; i < nd; ++i) elvec[i] = elvec[i] - elmat[i + j * nd] * gv;
+        }
+    }
+    /* coo_sparse_matrix_append_dense_values_body (base_sparse_matrix.f90:3469-3513) */
+    for (int j = 0; j < nd; ++j) {
+      int32_t jas = g[j];
+      if (jas < 1 || jas > n) continue;
+      for (int i = 0; i < nd; ++i) {
+        int32_t ias = g[i];
+        if (ias < 1 || ias > n || (sym && ias > jas)) continue;
+        coo_append(&m, ias, jas, elmat[i + j * nd]);
+      }
+    }
+    /* serial_scalar_array_add_multiple_entries (serial_scalar_array.f90:247-262) */
+    for (int i = 0; i < nd; ++i)
+      if (g[i] > 0) rhs[g[i] - 1] = rhs[g[i] - 1] + elvec[i];
+  }
+  free(elmat);
+  free(elvec);
+
+  /* sort_and_compress_numeric by rows (base_sparse_matrix.f90:4482-4827):
+     stable counting sort on rows, stable column sort inside each row
+     (mergesort_link_list, sparse_matrix_utils.f90:136-250), duplicates summed in
+     insertion order; then move_from_coo builds irp (csr_sparse_matrix.f90:416-479). */
+  int64_t nnz = m.nnz;
+  int64_t *cnt = (int64_t *)xcalloc((size_t)n + 2, sizeof(int64_t));
+  for (int64_t k = 0; k < nnz; ++k) cnt[m.ia[k]]++;
+  int64_t acc = 0;
+  for (int r = 1; r <= n; ++r) { int64_t t = cnt[r]; cnt[r] = acc; acc += t; }
+  int32_t *ja2 = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+  double *v2 = (double *)xmalloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+  int64_t *rs = (int64_t *)xmalloc(sizeof(int64_t) * ((size_t)n + 2));
+  for (int r = 1; r <= n; ++r) rs[r] = cnt[r];
+  rs[n + 1] = acc;
+  for (int64_t k = 0; k < nnz; ++k) {
+    int64_t p = cnt[m.ia[k]]++;
+    ja2[p] = m.ja[k]; v2[p] = m.val[k];
+  }
+  free(m.ia); free(m.ja); free(m.val); free(cnt);
+
+  int32_t *irp = (int32_t *)xmalloc(sizeof(int32_t) * ((size_t)n + 1));
+  int32_t *jo = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+  double *vo = (double *)xmalloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+  /* per-row stable merge sort buffers */
+  int32_t *tj = NULL; double *tv = NULL; int64_t tcap = 0;
+  int64_t k = 0;
+  irp[0] = 1;
+  for (int r = 1; r <= n; ++r) {
+    int64_t b = rs[r], e = (r < n) ? rs[r + 1] : acc, len = e - b;
+    if (len > tcap) {
+      tcap = len;
+      tj = (int32_t *)realloc(tj, sizeof(int32_t) * (size_t)tcap);
+      tv = (double *)realloc(tv, sizeof(double) * (size_t)tcap);
+    }
+    /* bottom-up stable merge sort of (ja2, v2)[b:e) by column */
+    int32_t *aj = ja2 + b; double *av = v2 + b;
+    for (int64_t w = 1; w < len; w *= 2) {
+      for (int64_t lo = 0; lo < len; lo += 2 * w) {
+        int64_t mid = lo + w < len ? lo + w : len, hi = lo + 2 * w < len ? lo + 2 * w : len;
+        int64_t i = lo, j = mid, o = lo;
+        while (i < mid && j < hi) {
+          if (aj[j] < aj[i]) { tj[o] = aj[j]; tv[o] = av[j]; ++j; }
+          else { tj[o] = aj[i]; tv[o] = av[i]; ++i; }
+          ++o;
+        }
+        while (i < mid) { tj[o] = aj[i]; tv[o] = av[i]; ++i; ++o; }
+        while (j < hi) { tj[o] = aj[j]; tv[o] = av[j]; ++j; ++o; }
+      }
+      memcpy(aj, tj, sizeof(int32_t) * (size_t)len);
+      memcpy(av, tv, sizeof(double) * (size_t)len);
+    }
+    for (int64_t i = 0; i < len; ++i) {
+      if (i > 0 && aj[i] == aj[i - 1]) vo[k - 1] = vo[k - 1] + av[i];
+      else { jo[k] = aj[i]; vo[k] = av[i]; ++k; }
+    }
+    irp[r] = (int32_t)(k + 1);
+  }
+  free(tj); free(tv); free(ja2); free(v2); free(rs);
+  *irp_out = irp; *ja_out = jo; *val_out = vo; *rhs_out = rhs;
+  return k;
+}
+
+/* --------------------------------------------------------- solver --------- */
+
+void orc_spmv(int n, const int32_t *irp, const int32_t *ja, const double *val, int sym,
+              const double *x, double *y) {
+  if (!sym) {
+    for (int r = 0; r < n; ++r) {
+      double acc = 0.0;   /* beta = 0 */
+      for (int32_t k = irp[r] - 1; k < irp[r + 1] - 1; ++k) acc = acc + 1.0 * x[ja[k] - 1] * val[k];
+      y[r] = acc;
+    }
+  } else {
+    for (int r = 0; r < n; ++r) y[r] = 0.0;
+    for (int r = 0; r < n; ++r) {
+      if (irp[r + 1] - irp[r] <= 0) continue;
+      y[r] = y[r] + x[ja[irp[r] - 1] - 1] * val[irp[r] - 1];
+      for (int32_t k = irp[r]; k < irp[r + 1] - 1; ++k) {
+        int c = ja[k] - 1;
+        y[r] = y[r] + 1.0 * x[c] * val[k];
+        y[c] = y[c] + 1.0 * x[r] * val[k];
+      }
+    }
+  }
+}
+
+void orc_extract_diagonal(int n, const int32_t *irp, const int32_t *ja, const double *val, double *d) {
+  for (int r = 0; r < n; ++r) {
+    /* binary_search (sparse_matrix_utils.f90:105-134) */
+    int lo = irp[r] - 1, hi = irp[r + 1] - 2;
+    d[r] = 0.0;
+    while (lo <= hi) {
+      int mid = (lo + hi) / 2;
+      if (ja[mid] == r + 1) { d[r] = val[mid]; break; }
+      if (ja[mid] < r + 1) lo = mid + 1; else hi = mid - 1;
+    }
+  }
+}
+
+static double dot(int n, const double *a, const double *b) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+static double nrm2(int n, const double *a) { return sqrt(dot(n, a, a)); }
+
+void orc_cg(int n, const int32_t *irp, const int32_t *ja, const double *val, int sym, int precond,
+            int crit, double rtol, double atol, int maxit, const double *b, double *x,
+            orc_cg_result_t *res, double *history) {
+  double *r = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *z = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *p = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *Ap = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *inv = NULL;
+  if (precond == 1) {
+    /* jacobi numerical_setup: extract_diagonal + entrywise_invert
+       (jacobi_preconditioner.f90:356-389, serial_scalar_array.f90:581-588) */
+    inv = (double *)xmalloc(sizeof(double) * (size_t)n);
+    orc_extract_diagonal(n, irp, ja, val, inv);
+    for (int i = 0; i < n; ++i) inv[i] = 1.0 / inv[i];
+  }
+#define APPLY_M(src, dst) do { if (inv) { for (int i_ = 0; i_ < n; ++i_) (dst)[i_] = inv[i_] * (src)[i_]; } \
+                               else memcpy((dst), (src), sizeof(double) * (size_t)n); } while (0)
+  /* cg_solve_body (cg.f90:101-238) */
+  double b_nrm_M = 0.0;
+  if (crit == ORC_RES_NRMGIVEN_RHS_NRMGIVEN) {
+    APPLY_M(b, r);
+    b_nrm_M = sqrt(dot(n, b, r));
+  }
+  orc_spmv(n, irp, ja, val, sym, x, r);
+  for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];          /* r%axpby(1.0, b, -1.0) */
+  APPLY_M(r, z);
+  double r_z = dot(n, r, z);
+  double r_nrm_M = sqrt(r_z);
+  memcpy(p, z, sizeof(double) * (size_t)n);
+  /* init_convergence_data (cg.f90:279-321) */
+  double rhs_test = 0.0, rhs_extra = 0.0;
+  switch (crit) {
+    case ORC_DELTA_RHS: case ORC_RES_RHS: case ORC_DELTA_RHS_AND_RES_RHS: case ORC_DELTA_DELTA_AND_RES_RHS:
+      rhs_test = rtol * nrm2(n, b) + atol;
+      if (crit == ORC_DELTA_RHS_AND_RES_RHS || crit == ORC_DELTA_DELTA_AND_RES_RHS) rhs_extra = rhs_test;
+      break;
+    case ORC_RES_RES: case ORC_DELTA_DELTA_AND_RES_RES:
+      rhs_test = rtol * nrm2(n, r) + atol;
+      if (crit == ORC_DELTA_DELTA_AND_RES_RES) rhs_extra = rhs_test;
+      break;
+    case ORC_DELTA_RHS_AND_RES_RES:
+      rhs_test = rtol * nrm2(n, b) + atol;
+      rhs_extra = rtol * nrm2(n, r) + atol;
+      break;
+    case ORC_RES_NRMGIVEN_RHS_NRMGIVEN: rhs_test = rtol * b_nrm_M + atol; break;
+    case ORC_RES_NRMGIVEN_RES_NRMGIVEN: rhs_test = rtol * r_nrm_M + atol; break;
+    default: break;
+  }
+  int it = 0, conv = 0;
+  double est = 0.0;
+  int is_delta = (crit == ORC_DELTA_RHS || crit == ORC_DELTA_DELTA || crit == ORC_DELTA_RHS_AND_RES_RES ||
+                  crit == ORC_DELTA_RHS_AND_RES_RHS || crit == ORC_DELTA_DELTA_AND_RES_RES ||
+                  crit == ORC_DELTA_DELTA_AND_RES_RHS);
+  int is_dd = (crit == ORC_DELTA_DELTA || crit == ORC_DELTA_DELTA_AND_RES_RES || crit == ORC_DELTA_DELTA_AND_RES_RHS);
+  int has_extra = (crit >= ORC_DELTA_RHS_AND_RES_RES);
+  for (;;) {
+    ++it;
+    orc_spmv(n, irp, ja, val, sym, p, Ap);
+    double Ap_p = dot(n, Ap, p);
+    double alpha = (Ap_p != 0.0) ? r_z / Ap_p : 0.0;
+    for (int i = 0; i < n; ++i) x[i] = x[i] + alpha * p[i];
+    for (int i = 0; i < n; ++i) r[i] = r[i] + (-alpha) * Ap[i];
+    /* update_convergence_data_and_evaluate_stopping_criteria (cg.f90:323-407) */
+    if (it == 1 && is_delta) {
+      est = alpha * nrm2(n, p);
+      if (is_dd) rhs_test = rtol * est + atol;
+    }
+    if (crit == ORC_RES_RES || crit == ORC_RES_RHS) est = nrm2(n, r);
+    else if (is_delta) { if (it != 1) est = alpha * nrm2(n, p); }
+    else est = r_nrm_M;
+    if (history) history[it - 1] = est;
+    conv = (est <= rhs_test);
+    if (has_extra && conv) conv = (nrm2(n, r) <= rhs_extra);
+    if (conv || it >= maxit) break;
+    APPLY_M(r, z);
+    double beta = 1.0 / r_z;
+    r_z = dot(n, r, z);
+    beta = beta * r_z;
+    r_nrm_M = sqrt(r_z);
+    /* p%axpby(1.0, z, beta): scal then daxpy (serial_scalar_array.f90:386-394) */
+    for (int i = 0; i < n; ++i) { p[i] = beta * p[i]; p[i] = p[i] + 1.0 * z[i]; }
+  }
+#undef APPLY_M
+  res->iterations = it;
+  res->converged = conv;
+  res->error_estimate = est;
+  res->rhs_test = rhs_test;
+  free(r); free(z); free(p); free(Ap); free(inv);
+}

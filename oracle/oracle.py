@@ -1,0 +1,214 @@
+"""ctypes wrapper around the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of bench.py, always as the checker.  Nothing in the
+fempar_amd package may import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+BC_ALL, BC_X0, BC_NONE = 0, 1, 2
+PHYS_POISSON, PHYS_ELASTICITY = 0, 1
+K_CONST, K_SIN3, K_CELL = 0, 1, 2
+F_CONST, F_MANUFACTURED, F_CELL_SIN = 0, 1, 2
+G_ZERO, G_LINEAR = 0, 1
+
+CRITERIA = ["RES_NRMGIVEN_RHS_NRMGIVEN", "RES_NRMGIVEN_RES_NRMGIVEN", "DELTA_RHS", "DELTA_DELTA",
+            "RES_RES", "RES_RHS", "DELTA_RHS_AND_RES_RES", "DELTA_RHS_AND_RES_RHS",
+            "DELTA_DELTA_AND_RES_RES", "DELTA_DELTA_AND_RES_RHS"]
+
+
+class Problem(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n", C.c_int * 3), ("lim", C.c_double * 6), ("order", C.c_int),
+                ("ncomp", C.c_int), ("bc_kind", C.c_int)]
+
+
+class Physics(C.Structure):
+    _fields_ = [("kind", C.c_int), ("k_kind", C.c_int), ("k0", C.c_double),
+                ("k_cell", C.POINTER(C.c_double)), ("f_kind", C.c_int), ("f0", C.c_double),
+                ("g_kind", C.c_int), ("E", C.c_double), ("nu", C.c_double), ("body", C.c_double * 3)]
+
+
+class CgResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("error_estimate", C.c_double),
+                ("rhs_test", C.c_double)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        L.orc_space_create.restype = P
+        L.orc_space_create.argtypes = [C.POINTER(Problem)]
+        L.orc_space_free.argtypes = [P]
+        L.orc_space_sizes.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_space_get_dofs.argtypes = [P, P]
+        L.orc_space_fixed_values.argtypes = [P, C.POINTER(Physics), P]
+        L.orc_reference_tables.argtypes = [P, P, P, P, P]
+        L.orc_gauss_legendre.argtypes = [C.c_int, P, P]
+        L.orc_element.argtypes = [P, C.POINTER(Physics), C.c_int64, P, P]
+        L.orc_assemble.restype = C.c_int64
+        L.orc_assemble.argtypes = [P, C.POINTER(Physics), C.c_int, P, C.POINTER(P), C.POINTER(P),
+                                   C.POINTER(P), C.POINTER(P)]
+        L.orc_assemble_cells.restype = C.c_int64
+        L.orc_assemble_cells.argtypes = [P, C.POINTER(Physics), C.c_int, P, C.c_int64, C.c_int64, C.POINTER(P),
+                                         C.POINTER(P), C.POINTER(P), C.POINTER(P)]
+        L.orc_free.argtypes = [P]
+        L.orc_spmv.argtypes = [C.c_int, P, P, P, C.c_int, P, P]
+        L.orc_extract_diagonal.argtypes = [C.c_int, P, P, P, P]
+        L.orc_cg.argtypes = [C.c_int, P, P, P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                             C.c_int, P, P, C.POINTER(CgResult), P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def make_problem(dim, n, order=1, ncomp=1, bc=BC_ALL, lim=None):
+    p = Problem()
+    p.dim = dim
+    nn = list(n) + [1] * (3 - len(n))
+    for i in range(3):
+        p.n[i] = nn[i]
+    lim = lim or [0.0, 1.0, 0.0, 1.0, 0.0, 1.0 if dim == 3 else 0.0]
+    for i in range(6):
+        p.lim[i] = lim[i]
+    p.order, p.ncomp, p.bc_kind = order, ncomp, bc
+    return p
+
+
+def make_physics(kind=PHYS_POISSON, k_kind=K_CONST, k0=1.0, k_cell=None, f_kind=F_CONST, f0=0.0,
+                 g_kind=G_ZERO, E=1.0, nu=0.3, body=(0.0, 0.0, -1.0)):
+    ph = Physics()
+    ph.kind, ph.k_kind, ph.k0 = kind, k_kind, k0
+    ph._k_cell_ref = None
+    if k_cell is not None:
+        arr = np.ascontiguousarray(k_cell, dtype=np.float64)
+        ph._k_cell_ref = arr
+        ph.k_cell = arr.ctypes.data_as(C.POINTER(C.c_double))
+    ph.f_kind, ph.f0, ph.g_kind, ph.E, ph.nu = f_kind, f0, g_kind, E, nu
+    for i in range(3):
+        ph.body[i] = body[i]
+    return ph
+
+
+class Space:
+    """Structured mesh + reference FE + DOF numbering (oracle)."""
+
+    def __init__(self, problem: Problem):
+        self.problem = problem
+        self.h = lib().orc_space_create(C.byref(problem))
+        if not self.h:
+            raise ValueError("oracle: unsupported problem")
+        nc, nd, nq, nf, nx = C.c_int64(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        lib().orc_space_sizes(self.h, C.byref(nc), C.byref(nd), C.byref(nq), C.byref(nf), C.byref(nx))
+        self.num_cells, self.nd, self.nq = nc.value, nd.value, nq.value
+        self.n_free, self.n_fixed = nf.value, nx.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_space_free(self.h)
+        except Exception:
+            pass
+
+    def dofs(self):
+        out = np.empty((self.num_cells, self.nd), dtype=np.int32)
+        lib().orc_space_get_dofs(self.h, _ptr(out))
+        return out
+
+    def fixed_values(self, phys):
+        out = np.zeros(max(self.n_fixed, 1), dtype=np.float64)
+        lib().orc_space_fixed_values(self.h, C.byref(phys), _ptr(out))
+        return out[: self.n_fixed]
+
+    def tables(self):
+        d = self.problem.dim
+        nnode = self.nd // self.problem.ncomp
+        qp = np.empty((self.nq, d)); qw = np.empty(self.nq)
+        shp = np.empty((self.nq, nnode)); dshp = np.empty((self.nq, nnode, d))
+        lib().orc_reference_tables(self.h, _ptr(qp), _ptr(qw), _ptr(shp), _ptr(dshp))
+        return qp, qw, shp, dshp
+
+    def element(self, phys, cell):
+        em = np.empty((self.nd, self.nd), dtype=np.float64, order="F")
+        ev = np.empty(self.nd)
+        lib().orc_element(self.h, C.byref(phys), cell, em.ctypes.data_as(C.c_void_p), _ptr(ev))
+        return em, ev
+
+    def assemble(self, phys, symmetric_storage=True, fixed_values=None, cells=None):
+        irp, ja, val, rhs = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        fv = None if fixed_values is None else np.ascontiguousarray(fixed_values, dtype=np.float64)
+        cb, ce = (0, self.num_cells) if cells is None else cells
+        nnz = lib().orc_assemble_cells(self.h, C.byref(phys), int(bool(symmetric_storage)), _ptr(fv), cb, ce,
+                                       C.byref(irp), C.byref(ja), C.byref(val), C.byref(rhs))
+        n = self.n_free
+
+        def take(ptr, count, ctype, dtype):
+            if count == 0:
+                lib().orc_free(ptr)
+                return np.zeros(0, dtype=dtype)
+            arr = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,)).copy()
+            lib().orc_free(ptr)
+            return arr.astype(dtype, copy=False)
+
+        return (take(irp, n + 1, C.c_int32, np.int32), take(ja, nnz, C.c_int32, np.int32),
+                take(val, nnz, C.c_double, np.float64), take(rhs, n, C.c_double, np.float64))
+
+
+def gauss_legendre(n):
+    x = np.empty(n); w = np.empty(n)
+    lib().orc_gauss_legendre(n, _ptr(x), _ptr(w))
+    return x, w
+
+
+def spmv(irp, ja, val, x, symmetric_storage):
+    n = len(irp) - 1
+    y = np.empty(n)
+    lib().orc_spmv(n, _ptr(irp), _ptr(ja), _ptr(val), int(bool(symmetric_storage)),
+                   _ptr(np.ascontiguousarray(x, dtype=np.float64)), _ptr(y))
+    return y
+
+
+def extract_diagonal(irp, ja, val):
+    n = len(irp) - 1
+    d = np.empty(n)
+    lib().orc_extract_diagonal(n, _ptr(irp), _ptr(ja), _ptr(val), _ptr(d))
+    return d
+
+
+def cg(irp, ja, val, b, symmetric_storage=True, precond="jacobi", criterion="RES_RES", rtol=1e-10,
+       atol=0.0, maxit=5000, x0=None, history=False):
+    n = len(irp) - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    res = CgResult()
+    hist = np.zeros(maxit) if history else None
+    lib().orc_cg(n, _ptr(irp), _ptr(ja), _ptr(val), int(bool(symmetric_storage)),
+                 1 if precond == "jacobi" else 0, CRITERIA.index(criterion), rtol, atol, maxit,
+                 _ptr(np.ascontiguousarray(b, dtype=np.float64)), _ptr(x), C.byref(res), _ptr(hist))
+    out = {"iterations": res.iterations, "converged": bool(res.converged),
+           "error_estimate": res.error_estimate, "rhs_test": res.rhs_test}
+    if history:
+        out["history"] = hist[: res.iterations].copy()
+    return x, out

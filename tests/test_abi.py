@@ -1,0 +1,50 @@
+"""The C-ABI library builds, loads and exports every entry point that
+include/femhip.h declares.  CPU only: no compute calls."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "femhip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char|void)\s*\**\s*(femhip_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "femhip_solver_solve" in syms and "femhip_operator_compute" in syms
+    assert len(syms) >= 21
+
+
+def test_library_exports_every_declared_symbol():
+    from fempar_amd import _native as N
+    assert os.path.exists(N.LIB_PATH), "run fempar_amd.build.build() first"
+    lib = ctypes.CDLL(N.LIB_PATH)
+    for s in declared_symbols():
+        assert hasattr(lib, s), s
+    assert sorted(N.EXPORTS) == sorted(s for s in declared_symbols() if s in N.EXPORTS)
+    missing = [s for s in declared_symbols() if s not in N.EXPORTS]
+    assert not missing, f"bindings lack {missing}"
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from fempar_amd import _native as N
+    lib = N.load()
+    h = ctypes.c_void_p()
+    rc = lib.femhip_ctx_create(0, ctypes.byref(h))
+    assert rc == N.ERR_NO_DEVICE
+    assert b"no HIP device" in lib.femhip_last_error()
+
+
+def test_invalid_arguments_are_rejected_without_device():
+    from fempar_amd import _native as N
+    lib = N.load()
+    assert lib.femhip_space_info(None, None, None, None, None, None) == N.ERR_INVALID
+    assert b"null" in lib.femhip_last_error()
