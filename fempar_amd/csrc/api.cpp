@@ -43,7 +43,7 @@ void check_mesh(const femhip_mesh_desc_t *m) {
 }
 
 void init_space(femhip_space &s, femhip_ctx *ctx, const femhip_mesh_desc_t *mesh) {
-  s.ctx = ctx;
+  s.ctx = ctx_retain(ctx);
   s.mesh = *mesh;
   if (s.mesh.dim == 2) {
     s.mesh.n[2] = 1;
@@ -96,13 +96,7 @@ int femhip_ctx_create(int32_t device, femhip_ctx_t **ctx) {
 }
 
 int femhip_ctx_destroy(femhip_ctx_t *ctx) {
-  if (!ctx) return FEMHIP_OK;
-  (void)hipSetDevice(ctx->device);
-  if (ctx->stream) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-  }
-  delete ctx;
+  ctx_release(ctx);   // freed with the last dependent handle
   return FEMHIP_OK;
 }
 
@@ -117,7 +111,7 @@ int femhip_space_create_structured(femhip_ctx_t *ctx, const femhip_mesh_desc_t *
     number_structured(*s);
     upload_dofs(*s);
   } catch (...) {
-    delete s;
+    femhip_space_destroy(s);
     throw;
   }
   *space = s;
@@ -148,7 +142,7 @@ int femhip_space_create_from_dofs(femhip_ctx_t *ctx, const femhip_mesh_desc_t *m
     s->n_fixed = n_fixed;
     upload_dofs(*s);
   } catch (...) {
-    delete s;
+    femhip_space_destroy(s);
     throw;
   }
   *space = s;
@@ -216,9 +210,11 @@ int femhip_space_interpolate_dirichlet(const femhip_space_t *s, const femhip_phy
 }
 
 int femhip_space_destroy(femhip_space_t *s) {
-  if (s) {
-    (void)hipSetDevice(s->ctx->device);
+  if (s && s->refs.fetch_sub(1) == 1) {
+    femhip_ctx *c = s->ctx;
+    if (c) (void)hipSetDevice(c->device);
     delete s;
+    ctx_release(c);
   }
   return FEMHIP_OK;
 }
@@ -229,9 +225,10 @@ int femhip_operator_create(femhip_ctx_t *ctx, femhip_space_t *space, int32_t sym
   FH_CHECK(ctx && space && op, FEMHIP_ERR_INVALID, "null argument");
   FH_HIP(hipSetDevice(ctx->device));
   auto *o = new femhip_operator();
+  o->ctx = ctx_retain(ctx);
+  space->refs.fetch_add(1);
+  o->space = space;
   try {
-    o->ctx = ctx;
-    o->space = space;
     o->symmetric_storage = symmetric_storage ? 1 : 0;
     build_pattern(*o);
     o->rowptr.alloc((size_t)o->n + 1);
@@ -247,6 +244,8 @@ int femhip_operator_create(femhip_ctx_t *ctx, femhip_space_t *space, int32_t sym
     }
   } catch (...) {
     delete o;
+    femhip_space_destroy(space);
+    ctx_release(ctx);
     throw;
   }
   *op = o;
@@ -349,10 +348,14 @@ int femhip_operator_apply(femhip_operator_t *op, const double *x, double *y) {
 }
 
 int femhip_operator_destroy(femhip_operator_t *op) {
-  if (op) {
-    (void)hipSetDevice(op->ctx->device);
-    (void)hipStreamSynchronize(op->ctx->stream);
+  if (op && op->refs.fetch_sub(1) == 1) {
+    femhip_ctx *c = op->ctx;
+    femhip_space *sp = op->space;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     delete op;
+    femhip_space_destroy(sp);
+    ctx_release(c);
   }
   return FEMHIP_OK;
 }

@@ -308,7 +308,7 @@ template <int LPR>
 __global__ __launch_bounds__(kThreads) void k_ir_residual(int32_t n, const int32_t *__restrict__ rowptr,
                                                           const int32_t *__restrict__ col,
                                                           const double *__restrict__ val, const double *__restrict__ b,
-                                                          const double *__restrict__ x, int first,
+                                                          const double *__restrict__ x, int first, int x_zero,
                                                           double *__restrict__ r, IrState *ir, CgState *in,
                                                           double *part, unsigned *cnt) {
   if (ir->done) return;
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kThreads) void k_ir_residual(int32_t n, const int32
   double v[2] = {0, 0};
   for (int64_t rr = grp; rr < n; rr += ngrp) {
     const int32_t row = (int32_t)rr;
-    double ax = first ? 0.0 : row_dot<LPR, double, double>(rowptr, col, val, x, row, lane);
+    double ax = x_zero ? 0.0 : row_dot<LPR, double, double>(rowptr, col, val, x, row, lane);
     if (lane == 0) {
       double bi = b[row];
       double ri = bi - ax;
@@ -607,8 +607,8 @@ void solve_refined(femhip_solver &s, bool x_zero, femhip_cg_result_t *res) {
   for (;;) {
     dispatch_lpr(s.lpr, [&](auto L) {
       hipLaunchKernelGGL((k_ir_residual<decltype(L)::value>), dim3(s.nblk), dim3(kThreads), 0, st, s.n,
-                         op.rowptr.p, op.col.p, op.val.p, s.b.p, s.x.p, first && x_zero ? 1 : 0, s.r.p, s.ir.p,
-                         s.st32.p, s.part.p, s.cnt.p + 2);
+                         op.rowptr.p, op.col.p, op.val.p, s.b.p, s.x.p, first, first && x_zero ? 1 : 0, s.r.p,
+                         s.ir.p, s.st32.p, s.part.p, s.cnt.p + 2);
     });
     FH_HIP(hipGetLastError());
     first = 0;
@@ -662,7 +662,8 @@ int femhip_solver_create(femhip_ctx_t *ctx, femhip_operator_t *op, const femhip_
              FEMHIP_ERR_INVALID, "FP32_REFINED supports the RES_RES and RES_RHS criteria");
     FH_HIP(hipSetDevice(ctx->device));
     auto *s = new femhip_solver();
-    s->ctx = ctx;
+    s->ctx = ctx_retain(ctx);
+    op->refs.fetch_add(1);
     s->op = op;
     s->prm = p;
     *solver = s;
@@ -782,9 +783,13 @@ int femhip_solver_get_solution(const femhip_solver_t *s, double *x) {
 
 int femhip_solver_destroy(femhip_solver_t *s) {
   if (s) {
-    (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
+    femhip_ctx *c = s->ctx;
+    femhip_operator *op = s->op;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     delete s;
+    femhip_operator_destroy(op);
+    ctx_release(c);
   }
   return FEMHIP_OK;
 }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -79,14 +80,36 @@ void gauss_legendre(int n, double *x, double *w);
 // --------------------------------------------------------------- context ---
 }  // namespace femhip
 
+// The context is reference counted: every space, operator and solver holds a
+// reference, so femhip_ctx_destroy may be called before its dependents are
+// destroyed (the stream is released with the last reference).
 struct femhip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
+  std::atomic<int> refs{1};
 };
+
+namespace femhip {
+inline femhip_ctx *ctx_retain(femhip_ctx *c) {
+  c->refs.fetch_add(1);
+  return c;
+}
+inline void ctx_release(femhip_ctx *c) {
+  if (c && c->refs.fetch_sub(1) == 1) {
+    (void)hipSetDevice(c->device);
+    if (c->stream) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+  }
+}
+}  // namespace femhip
 
 struct femhip_space {
   femhip_ctx *ctx = nullptr;
+  std::atomic<int> refs{1};           // the handle + every operator built on it
   femhip_mesh_desc_t mesh{};
   femhip::RefFE fe;
   int64_t ncells = 0;
@@ -97,6 +120,7 @@ struct femhip_space {
 
 struct femhip_operator {
   femhip_ctx *ctx = nullptr;
+  std::atomic<int> refs{1};           // the handle + every solver built on it
   femhip_space *space = nullptr;
   int symmetric_storage = 0;
   int32_t n = 0;

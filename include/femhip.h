@@ -11,7 +11,10 @@
  * are borrowed for the duration of a call and never freed by the library;
  * device state is owned by the handles and released by the *_destroy calls.
  * Every entry point returns FEMHIP_OK or an error code; femhip_last_error()
- * gives the message of the last failure on the calling thread.
+ * gives the message of the last failure on the calling thread.  Handles are
+ * reference counted along ctx <- space <- operator <- solver: destroying a
+ * parent first only drops its reference, and it is released with the last
+ * dependent.  One thread drives a context at a time.
  *
  * Reference interfaces each entry point replaces are cited as file:line
  * relative to FEMPAR's Sources/Lib.
