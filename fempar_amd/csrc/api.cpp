@@ -308,6 +308,7 @@ int femhip_operator_compute(femhip_operator_t *op, const double *fixed_values, d
   float t = 0.f;
   assemble(*op, fixed_values, &t);
   op->computed = true;
+  ++op->version;
   if (ms) *ms = t;
   return FEMHIP_OK;
   FH_CATCH

@@ -1,24 +1,38 @@
 // cg.hip -- (Jacobi-)preconditioned conjugate gradients on gfx950.
 //
 // Plain path: the reference's cg_solve_body (LinearAlgebra/IterativeLinearSolver/
-// cg.f90:101-238) with its stopping criteria (cg.f90:279-407) in FP64.  Three
-// kernels per iteration, each a single HBM pass:
-//   spmv : q = A p,  <p,q> (and ||p||^2 for the DELTA criteria)  -> alpha
-//   upd  : x += alpha p, r -= alpha q, z = invD r, <r,z>, ||r||^2 -> test, beta
-//   dir  : p = z + beta p   (z recomputed from r and invD, never stored)
-// Scalars live in a device state block.  Each kernel reduces its block partials
-// in a fixed order in the last block to finish (deterministic results), and
-// every kernel returns at entry once the state says `done`, so the host can
-// queue iterations ahead and poll the state only every few iterations.
+// cg.f90:101-238) with all its stopping criteria (cg.f90:279-407), in FP64.
+// Three kernels per iteration, each one HBM pass:
+//   spmv : q = A p ; block partials of <p,q>, ||p||^2
+//   upd  : alpha = rz/<p,q> ; x += alpha p ; r -= alpha q ; partials <r,z>, ||r||^2
+//   dir  : stopping test, beta ; p = z + beta p        (z = invD r, never stored)
+// Scalars never leave the device.  A kernel reduces the block partials of its
+// producer at entry, every block in the same fixed order, so all blocks agree
+// bit for bit and no grid-wide fence or atomic is needed (visibility comes from
+// the kernel boundary).  The scalar state is double-buffered by iteration
+// parity: iteration i reads S[i&1] and its last kernel writes S[(i+1)&1], so no
+// block ever reads a field another block of the same launch writes.  Once the
+// state says `done`, every kernel returns at entry; the host launches
+// hipGraph-captured chunks of iterations and polls a small result block.
 //
-// FP32-refined path: outer FP64 iterative refinement (residual r = b - A x,
-// solution update and convergence test in FP64 on the device) around an inner
-// FP32 Jacobi-PCG on an FP32 copy of the matrix with FP32 arithmetic.
+// The SpMV runs on the sliced diagonal/ELL copy of the CSR matrix (one
+// wavefront per 64-row slice, column offsets of diagonal slices are
+// wave-uniform, so the matrix stream carries no indices), with each XCD given a
+// contiguous range of slices so that the +-plane neighbours of p stay in its L2.
+//
+// FP32-refined path: CG iterated in FP32 (FP32 matrix copy, FP32 arithmetic and
+// accumulation) with reliable updates (residual replacement, van der Vorst & Ye,
+// SIAM J. Sci. Comput. 22, 2000): when the FP32 recurrence residual has fallen
+// by a factor delta since the last update, the FP32 solution increment is
+// added into the FP64 solution, the residual r = b - A x is recomputed in FP64
+// on the FP64 matrix and replaces the FP32 residual, and the stopping test is
+// made on that FP64 residual.  The search direction is kept, so the Krylov
+// convergence rate of plain CG is preserved.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "femhip_internal.h"
@@ -27,18 +41,26 @@ namespace femhip {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
 
 struct CgState {
   double rz, rnrm_m, alpha, beta;
   double rhs_test, rhs_extra, est, pap, pp;
   double bnorm, r0norm, rnorm;
   double rtol, atol;
-  int iter, done, converged, maxit, crit, zero_diag;
+  double rmax, delta;          // reliable updates
+  int iter, done, converged, maxit, crit, ru, updates, pad;
 };
 
-struct IrState {
-  double rnorm, bnorm, r0norm, rhs_test, rtol, atol, inner_rtol;
-  int outer, max_outer, done, converged, total_inner, crit;
+struct CgResult {
+  double est, rhs_test, rnorm, bnorm;
+  int iter, done, converged, updates;
+};
+
+struct SlicedDev {
+  int32_t ns, n;
+  const int64_t *base;
+  const int32_t *meta, *aux, *offs, *ecol;
 };
 
 __device__ __forceinline__ bool is_delta(int c) {
@@ -48,10 +70,11 @@ __device__ __forceinline__ bool is_delta_delta(int c) {
   return c == FEMHIP_DELTA_DELTA || c == FEMHIP_DELTA_DELTA_AND_RES_RES || c == FEMHIP_DELTA_DELTA_AND_RES_RHS;
 }
 
-// sum NV values over the block; result valid in thread 0
+// Sum NV values over the block; every thread gets the result.
 template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV]) {
-  __shared__ double sm[NV][kThreads / 64];
+__device__ __forceinline__ void block_sum_all(double (&v)[NV]) {
+  __shared__ double sm[NV][kWaves];
+  __shared__ double out[NV];
   for (int off = 32; off > 0; off >>= 1)
     for (int k = 0; k < NV; ++k) v[k] += __shfl_down(v[k], off, 64);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -61,220 +84,455 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
   if (threadIdx.x == 0)
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
-      for (int i = 0; i < kThreads / 64; ++i) s += sm[k][i];
-      v[k] = s;
+      for (int i = 0; i < kWaves; ++i) s += sm[k][i];
+      out[k] = s;
     }
+  __syncthreads();
+  for (int k = 0; k < NV; ++k) v[k] = out[k];
+  __syncthreads();
 }
 
-// write this block's partials; true in every thread of the last block to finish
+// Block partials of this launch: part[k * gridDim.x + blockIdx.x].
 template <int NV>
-__device__ __forceinline__ bool publish_partials(double (&v)[NV], double *part, unsigned *cnt) {
-  __shared__ bool last;
-  block_sum<NV>(v);
-  if (threadIdx.x == 0) {
+__device__ __forceinline__ void write_partials(double (&v)[NV], double *part) {
+  block_sum_all<NV>(v);
+  if (threadIdx.x == 0)
     for (int k = 0; k < NV; ++k) part[k * gridDim.x + blockIdx.x] = v[k];
-    __threadfence();
-    last = (atomicAdd(cnt, 1u) == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
 }
 
-// last block: fixed-order sum of all partials; result valid in thread 0
+// Fixed-order reduction of a producer's g block partials (same in every block).
 template <int NV>
-__device__ __forceinline__ void gather_partials(double (&v)[NV], const double *part, unsigned *cnt) {
+__device__ __forceinline__ void reduce_partials(const double *part, int g, double (&v)[NV]) {
   for (int k = 0; k < NV; ++k) v[k] = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads)
-    for (int k = 0; k < NV; ++k) v[k] += __hip_atomic_load(part + k * gridDim.x + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  block_sum<NV>(v);
-  if (threadIdx.x == 0) *cnt = 0u;
+  for (int b = threadIdx.x; b < g; b += kThreads)
+    for (int k = 0; k < NV; ++k) v[k] += part[k * g + b];
+  block_sum_all<NV>(v);
 }
 
-// CSR row dot-product by a group of LPR lanes (all lanes of the group call it)
-template <int LPR, typename T, typename A>
-__device__ __forceinline__ A row_dot(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                                     const T *__restrict__ val, const T *__restrict__ x, int32_t r, int lane) {
-  A s = 0;
-  const int32_t b = rowptr[r], e = rowptr[r + 1];
-  for (int32_t k = b + lane; k < e; k += LPR) s += (A)val[k] * (A)x[col[k]];
-#pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-  return s;
+// logical block index giving each XCD (blocks dispatched round-robin over 8)
+// a contiguous range
+__device__ __forceinline__ int xcd_block() {
+  const int g = gridDim.x, b = blockIdx.x;
+  return (g % 8 == 0) ? (b % 8) * (g / 8) + b / 8 : b;
 }
 
-// ------------------------------------------------------------ FP64 / generic CG ---
+// one row (= lane) of a 64-row slice
+template <typename T, typename A>
+__device__ __forceinline__ A slice_row(const SlicedDev &m, const T *__restrict__ v, const T *__restrict__ x, int s,
+                                       int lane) {
+  const int64_t b = m.base[s];
+  const int meta = m.meta[s];
+  const int K = meta & 0x3fffffff;
+  const int row = s * kSlice + lane;
+  A acc = 0;
+  const T *vv = v + b + lane;
+  if (!(meta >> 30)) {
+    const int32_t *off = m.offs + m.aux[s];
+    const int hi = m.n - 1;
+#pragma unroll 9
+    for (int k = 0; k < K; ++k) {
+      int c = row + off[k];
+      c = min(max(c, 0), hi);
+      acc += (A)vv[k * kSlice] * (A)x[c];
+    }
+  } else {
+    const int32_t *ec = m.ecol + m.aux[s] + lane;
+#pragma unroll 4
+    for (int k = 0; k < K; ++k) acc += (A)vv[k * kSlice] * (A)x[ec[k * kSlice]];
+  }
+  return acc;
+}
 
-// r = b - A x ; z = M r ; p = z ; partials <r,z>, <r,r>, <b,b>, <b,M b>
-template <int LPR, typename T>
-__global__ __launch_bounds__(kThreads) void k_cg_init(int32_t n, const int32_t *__restrict__ rowptr,
-                                                      const int32_t *__restrict__ col, const T *__restrict__ val,
-                                                      const T *__restrict__ b, const T *__restrict__ x, int x_zero,
-                                                      T *__restrict__ r, T *__restrict__ p, const T *__restrict__ invd,
-                                                      CgState *st, double *part, unsigned *cnt) {
-  const int lane = threadIdx.x % LPR;
-  const int64_t grp = ((int64_t)blockIdx.x * kThreads + threadIdx.x) / LPR;
-  const int64_t ngrp = (int64_t)gridDim.x * kThreads / LPR;
-  double v[4] = {0, 0, 0, 0};
-  for (int64_t rr = grp; rr < n; rr += ngrp) {
-    const int32_t row = (int32_t)rr;
-    T ax = x_zero ? (T)0 : row_dot<LPR, T, T>(rowptr, col, val, x, row, lane);
-    if (lane == 0) {
-      T bi = b[row];
-      T ri = bi - ax;
+// ------------------------------------------------------------------ kernels ---
+
+// r = b - A x (x_zero: r = b) ; z = M r ; p = z ; partials <r,z>, ||r||^2, ||b||^2, <b,Mb>
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_init(SlicedDev m, const T *__restrict__ v, const T *__restrict__ b,
+                                                   const T *__restrict__ x, int x_zero, T *__restrict__ r,
+                                                   T *__restrict__ p, const T *__restrict__ invd, double *part) {
+  const int lane = threadIdx.x & 63;
+  const int wave = xcd_block() * kWaves + (threadIdx.x >> 6), nw = gridDim.x * kWaves;
+  const int s0 = (int)((int64_t)m.ns * wave / nw), s1 = (int)((int64_t)m.ns * (wave + 1) / nw);
+  double acc[4] = {0, 0, 0, 0};
+  for (int s = s0; s < s1; ++s) {
+    T ax = x_zero ? (T)0 : slice_row<T, T>(m, v, x, s, lane);
+    const int row = s * kSlice + lane;
+    if (row < m.n) {
+      T bi = b[row], ri = bi - ax;
       T zi = invd ? invd[row] * ri : ri;
       r[row] = ri;
       p[row] = zi;
-      T mbi = invd ? invd[row] * bi : bi;
-      v[0] += (double)(ri * zi);
-      v[1] += (double)(ri * ri);
-      v[2] += (double)(bi * bi);
-      v[3] += (double)(bi * mbi);
+      T mb = invd ? invd[row] * bi : bi;
+      acc[0] += (double)(ri * zi);
+      acc[1] += (double)(ri * ri);
+      acc[2] += (double)(bi * bi);
+      acc[3] += (double)(bi * mb);
     }
   }
-  if (!publish_partials<4>(v, part, cnt)) return;
-  gather_partials<4>(v, part, cnt);
-  if (threadIdx.x == 0) {
-    // init_convergence_data (cg.f90:279-321)
-    const double rz = v[0], rnorm = sqrt(v[1]), bnorm = sqrt(v[2]), bnrm_m = sqrt(v[3]);
-    const double rtol = st->rtol, atol = st->atol;
-    const int c = st->crit;
-    st->rz = rz;
-    st->rnrm_m = sqrt(rz);
-    st->bnorm = bnorm;
-    st->r0norm = rnorm;
-    st->rnorm = rnorm;
-    double t = 0.0, extra = 0.0;
-    switch (c) {
-      case FEMHIP_DELTA_RHS:
-      case FEMHIP_RES_RHS:
-      case FEMHIP_DELTA_RHS_AND_RES_RHS:
-      case FEMHIP_DELTA_DELTA_AND_RES_RHS:
-        t = rtol * bnorm + atol;
-        if (c == FEMHIP_DELTA_RHS_AND_RES_RHS || c == FEMHIP_DELTA_DELTA_AND_RES_RHS) extra = t;
-        break;
-      case FEMHIP_RES_RES:
-      case FEMHIP_DELTA_DELTA_AND_RES_RES:
-        t = rtol * rnorm + atol;
-        if (c == FEMHIP_DELTA_DELTA_AND_RES_RES) extra = t;
-        break;
-      case FEMHIP_DELTA_RHS_AND_RES_RES:
-        t = rtol * bnorm + atol;
-        extra = rtol * rnorm + atol;
-        break;
-      case FEMHIP_RES_NRMGIVEN_RHS_NRMGIVEN: t = rtol * bnrm_m + atol; break;
-      case FEMHIP_RES_NRMGIVEN_RES_NRMGIVEN: t = rtol * sqrt(rz) + atol; break;
-      default: break;
-    }
-    st->rhs_test = t;
-    st->rhs_extra = extra;
-    st->iter = 0;
-    st->done = 0;
-    st->converged = 0;
-  }
+  write_partials<4>(acc, part);
 }
 
-// q = A p ; <p,q> (, ||p||^2) ; alpha
-template <int LPR, typename T, typename A>
-__global__ __launch_bounds__(kThreads) void k_cg_spmv(int32_t n, const int32_t *__restrict__ rowptr,
-                                                      const int32_t *__restrict__ col, const T *__restrict__ val,
-                                                      const T *__restrict__ p, T *__restrict__ q, CgState *st,
-                                                      double *part, unsigned *cnt) {
-  if (st->done) return;
-  const int lane = threadIdx.x % LPR;
-  const int64_t grp = ((int64_t)blockIdx.x * kThreads + threadIdx.x) / LPR;
-  const int64_t ngrp = (int64_t)gridDim.x * kThreads / LPR;
+// one block: init_convergence_data (cg.f90:279-321)
+__global__ __launch_bounds__(kThreads) void k_init_fin(const double *part, int g, CgState *S, CgResult *R) {
+  double v[4];
+  reduce_partials<4>(part, g, v);
+  if (threadIdx.x != 0) return;
+  CgState &s = S[0];
+  const double rz = v[0], rnorm = sqrt(v[1]), bnorm = sqrt(v[2]), bnrm_m = sqrt(v[3]);
+  const double rtol = s.rtol, atol = s.atol;
+  const int c = s.crit;
+  s.rz = rz;
+  s.rnrm_m = sqrt(rz);
+  s.bnorm = bnorm;
+  s.r0norm = rnorm;
+  s.rnorm = rnorm;
+  double t = 0.0, extra = 0.0;
+  switch (c) {
+    case FEMHIP_DELTA_RHS:
+    case FEMHIP_RES_RHS:
+    case FEMHIP_DELTA_RHS_AND_RES_RHS:
+    case FEMHIP_DELTA_DELTA_AND_RES_RHS:
+      t = rtol * bnorm + atol;
+      if (c == FEMHIP_DELTA_RHS_AND_RES_RHS || c == FEMHIP_DELTA_DELTA_AND_RES_RHS) extra = t;
+      break;
+    case FEMHIP_RES_RES:
+    case FEMHIP_DELTA_DELTA_AND_RES_RES:
+      t = rtol * rnorm + atol;
+      if (c == FEMHIP_DELTA_DELTA_AND_RES_RES) extra = t;
+      break;
+    case FEMHIP_DELTA_RHS_AND_RES_RES:
+      t = rtol * bnorm + atol;
+      extra = rtol * rnorm + atol;
+      break;
+    case FEMHIP_RES_NRMGIVEN_RHS_NRMGIVEN: t = rtol * bnrm_m + atol; break;
+    case FEMHIP_RES_NRMGIVEN_RES_NRMGIVEN: t = rtol * sqrt(rz) + atol; break;
+    default: break;
+  }
+  s.rhs_test = t;
+  s.rhs_extra = extra;
+  s.iter = 0;
+  s.done = 0;
+  s.converged = 0;
+  s.est = 0.0;
+  s.ru = 0;
+  s.updates = 0;
+  s.rmax = rnorm;
+  S[1] = s;
+  R->est = 0.0;
+  R->rhs_test = t;
+  R->rnorm = rnorm;
+  R->bnorm = bnorm;
+  R->iter = 0;
+  R->done = 0;
+  R->converged = 0;
+  R->updates = 0;
+}
+
+// q = A p ; partials <p,q>, ||p||^2
+template <typename T, typename A>
+__global__ __launch_bounds__(kThreads) void k_spmv(SlicedDev m, const T *__restrict__ v, const T *__restrict__ p,
+                                                   T *__restrict__ q, const CgState *S, int par, double *part) {
+  if (S[par].done) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = xcd_block() * kWaves + (threadIdx.x >> 6), nw = gridDim.x * kWaves;
+  const int s0 = (int)((int64_t)m.ns * wave / nw), s1 = (int)((int64_t)m.ns * (wave + 1) / nw);
   A pq = 0, pp = 0;
-  for (int64_t rr = grp; rr < n; rr += ngrp) {
-    const int32_t row = (int32_t)rr;
-    A s = row_dot<LPR, T, A>(rowptr, col, val, p, row, lane);
-    if (lane == 0) {
-      q[row] = (T)s;
-      A pi = (A)p[row];
-      pq += pi * (A)(T)s;
+  for (int s = s0; s < s1; ++s) {
+    const A acc = slice_row<T, A>(m, v, p, s, lane);
+    const int row = s * kSlice + lane;
+    if (row < m.n) {
+      q[row] = (T)acc;
+      const A pi = (A)p[row];
+      pq += pi * (A)(T)acc;
       pp += pi * pi;
     }
   }
-  double v[2] = {(double)pq, (double)pp};
-  if (!publish_partials<2>(v, part, cnt)) return;
-  gather_partials<2>(v, part, cnt);
-  if (threadIdx.x == 0) {
-    st->pap = v[0];
-    st->pp = v[1];
-    st->alpha = (v[0] != 0.0) ? st->rz / v[0] : 0.0;   // cg.f90:195-199
-  }
+  double red[2] = {(double)pq, (double)pp};
+  write_partials<2>(red, part);
 }
 
-// x += alpha p ; r -= alpha q ; <r,z>, ||r||^2 ; stopping test ; beta
+// alpha ; x += alpha p ; r -= alpha q ; partials <r,z>, ||r||^2
 template <typename T, typename A>
-__global__ __launch_bounds__(kThreads) void k_cg_update(int32_t n, T *__restrict__ x, T *__restrict__ r,
-                                                        const T *__restrict__ p, const T *__restrict__ q,
-                                                        const T *__restrict__ invd, CgState *st, double *part,
-                                                        unsigned *cnt, double *hist) {
-  if (st->done) return;
-  const T alpha = (T)st->alpha;
+__global__ __launch_bounds__(kThreads) void k_update(int32_t n, T *__restrict__ x, T *__restrict__ r,
+                                                     const T *__restrict__ p, const T *__restrict__ q,
+                                                     const T *__restrict__ invd, CgState *S, int par,
+                                                     const double *part_s, int gs, double *part) {
+  if (S[par].done) return;
+  double red[2];
+  reduce_partials<2>(part_s, gs, red);
+  const double alpha = (red[0] != 0.0) ? S[par].rz / red[0] : 0.0;   // cg.f90:195-199
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S[par].alpha = alpha;
+    S[par].pap = red[0];
+    S[par].pp = red[1];
+  }
+  const T a = (T)alpha;
   A rz = 0, rr = 0;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    x[i] = x[i] + alpha * p[i];
-    T ri = r[i] + (-alpha) * q[i];
+    x[i] = x[i] + a * p[i];
+    const T ri = r[i] + (-a) * q[i];
     r[i] = ri;
-    T zi = invd ? invd[i] * ri : ri;
+    const T zi = invd ? invd[i] * ri : ri;
     rz += (A)ri * (A)zi;
     rr += (A)ri * (A)ri;
   }
-  double v[2] = {(double)rz, (double)rr};
-  if (!publish_partials<2>(v, part, cnt)) return;
-  gather_partials<2>(v, part, cnt);
-  if (threadIdx.x == 0) {
-    // update_convergence_data_and_evaluate_stopping_criteria (cg.f90:323-407)
-    const int it = st->iter + 1;
-    const int c = st->crit;
-    const double rnorm = sqrt(v[1]);
-    double est = st->est;
-    if (it == 1 && is_delta(c)) {
-      est = st->alpha * sqrt(st->pp);
-      if (is_delta_delta(c)) st->rhs_test = st->rtol * est + st->atol;
-    }
-    if (c == FEMHIP_RES_RES || c == FEMHIP_RES_RHS) est = rnorm;
-    else if (is_delta(c)) { if (it != 1) est = st->alpha * sqrt(st->pp); }
-    else est = st->rnrm_m;
-    int conv = est <= st->rhs_test;
-    if (c >= FEMHIP_DELTA_RHS_AND_RES_RES && conv) conv = rnorm <= st->rhs_extra;
-    st->iter = it;
-    st->est = est;
-    st->rnorm = rnorm;
-    st->converged = conv;
-    if (hist) hist[it - 1] = est;
-    if (conv || it >= st->maxit) {
-      st->done = 1;
-    } else {
-      double beta = 1.0 / st->rz;
-      beta = beta * v[0];
-      st->beta = beta;
-      st->rz = v[0];
-      st->rnrm_m = sqrt(v[0]);
-    }
-  }
+  double out[2] = {(double)rz, (double)rr};
+  write_partials<2>(out, part);
 }
 
-// p = z + beta p
+// stopping test (cg.f90:323-407) ; beta ; p = z + beta p
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_cg_dir(int32_t n, T *__restrict__ p, const T *__restrict__ r,
-                                                     const T *__restrict__ invd, const CgState *st) {
-  if (st->done) return;
-  const T beta = (T)st->beta;
+__global__ __launch_bounds__(kThreads) void k_dir(int32_t n, T *__restrict__ p, const T *__restrict__ r,
+                                                  const T *__restrict__ invd, CgState *S, int par,
+                                                  const double *part_u, int gu, double *hist, CgResult *R) {
+  const CgState &cur = S[par];
+  if (cur.done) {   // keep `done` sticky in both parity slots
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[par ^ 1].done = 1;
+    return;
+  }
+  double red[2];
+  reduce_partials<2>(part_u, gu, red);
+  const int it = cur.iter + 1;
+  const int c = cur.crit;
+  const double rnorm = sqrt(red[1]);
+  double est = cur.est, rhs_test = cur.rhs_test;
+  if (it == 1 && is_delta(c)) {
+    est = cur.alpha * sqrt(cur.pp);
+    if (is_delta_delta(c)) rhs_test = cur.rtol * est + cur.atol;
+  }
+  if (c == FEMHIP_RES_RES || c == FEMHIP_RES_RHS) est = rnorm;
+  else if (is_delta(c)) {
+    if (it != 1) est = cur.alpha * sqrt(cur.pp);
+  } else est = cur.rnrm_m;
+  int conv = est <= rhs_test;
+  if (c >= FEMHIP_DELTA_RHS_AND_RES_RES && conv) conv = rnorm <= cur.rhs_extra;
+  const bool done = conv || it >= cur.maxit;
+  double beta = 1.0 / cur.rz;
+  beta = beta * red[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CgState nx = cur;
+    nx.iter = it;
+    nx.est = est;
+    nx.rhs_test = rhs_test;
+    nx.rnorm = rnorm;
+    nx.converged = conv;
+    nx.done = done;
+    nx.beta = beta;
+    nx.rz = red[0];
+    nx.rnrm_m = sqrt(red[0]);
+    S[par ^ 1] = nx;
+    if (hist) hist[it - 1] = est;
+    R->iter = it;
+    R->est = est;
+    R->rhs_test = rhs_test;
+    R->rnorm = rnorm;
+    R->converged = conv;
+    R->done = done;
+  }
+  if (done) return;
+  const T bt = (T)beta;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    T zi = invd ? invd[i] * r[i] : r[i];
-    T t = beta * p[i];
+    const T zi = invd ? invd[i] * r[i] : r[i];
+    const T t = bt * p[i];
     p[i] = t + zi;
   }
 }
 
+// ------------------------------------------------- FP32 with reliable updates ---
+
+// r64 = b - A64 x64 ; r32 = r64 ; z = M r32 ; p = z ; d = 0 ; partials rz32, ||r64||^2, ||b||^2
+__global__ __launch_bounds__(kThreads) void k_ru_init(SlicedDev m, const double *__restrict__ v64,
+                                                      const double *__restrict__ b, const double *__restrict__ x,
+                                                      int x_zero, double *__restrict__ r64, float *__restrict__ r32,
+                                                      float *__restrict__ p32, float *__restrict__ d32,
+                                                      const float *__restrict__ invd32, double *part) {
+  const int lane = threadIdx.x & 63;
+  const int wave = xcd_block() * kWaves + (threadIdx.x >> 6), nw = gridDim.x * kWaves;
+  const int s0 = (int)((int64_t)m.ns * wave / nw), s1 = (int)((int64_t)m.ns * (wave + 1) / nw);
+  double acc[4] = {0, 0, 0, 0};
+  for (int s = s0; s < s1; ++s) {
+    const double ax = x_zero ? 0.0 : slice_row<double, double>(m, v64, x, s, lane);
+    const int row = s * kSlice + lane;
+    if (row < m.n) {
+      const double bi = b[row], ri = bi - ax;
+      r64[row] = ri;
+      const float rf = (float)ri;
+      const float zf = invd32 ? invd32[row] * rf : rf;
+      r32[row] = rf;
+      p32[row] = zf;
+      d32[row] = 0.0f;
+      acc[0] += (double)(rf * zf);
+      acc[1] += ri * ri;
+      acc[2] += bi * bi;
+      acc[3] += (double)(rf * rf);
+    }
+  }
+  write_partials<4>(acc, part);
+}
+
+__global__ __launch_bounds__(kThreads) void k_ru_init_fin(const double *part, int g, CgState *S, CgResult *R) {
+  double v[4];
+  reduce_partials<4>(part, g, v);
+  if (threadIdx.x != 0) return;
+  CgState &s = S[0];
+  const double rnorm = sqrt(v[1]), bnorm = sqrt(v[2]);
+  s.rz = v[0];
+  s.bnorm = bnorm;
+  s.r0norm = rnorm;
+  s.rnorm = rnorm;
+  s.rhs_test = (s.crit == FEMHIP_RES_RHS ? bnorm : rnorm) * s.rtol + s.atol;
+  s.rmax = sqrt(v[3]);
+  s.iter = 0;
+  s.converged = rnorm <= s.rhs_test;
+  s.done = s.converged;
+  s.ru = 0;
+  s.updates = 0;
+  S[1] = s;
+  R->est = rnorm;
+  R->rhs_test = s.rhs_test;
+  R->rnorm = rnorm;
+  R->bnorm = bnorm;
+  R->iter = 0;
+  R->done = s.done;
+  R->converged = s.converged;
+  R->updates = 0;
+}
+
+// FP32 stopping decision: continue (p update) or request a reliable update
+__global__ __launch_bounds__(kThreads) void k_dir_ru(int32_t n, float *__restrict__ p, const float *__restrict__ r,
+                                                     const float *__restrict__ invd, CgState *S, int par,
+                                                     const double *part_u, int gu) {
+  const CgState &cur = S[par];
+  if (cur.done) return;
+  double red[2];
+  reduce_partials<2>(part_u, gu, red);
+  const int it = cur.iter + 1;
+  const double rnorm = sqrt(red[1]);
+  // residual fell by delta since the last update, looks converged, or budget spent
+  const bool ru = rnorm < cur.delta * cur.rmax || rnorm <= cur.rhs_test || it >= cur.maxit;
+  double beta = 1.0 / cur.rz;
+  beta = beta * red[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CgState nx = cur;
+    nx.iter = it;
+    nx.ru = ru;
+    nx.rnorm = rnorm;
+    if (!ru) {
+      nx.beta = beta;
+      nx.rz = red[0];
+      nx.rmax = fmax(cur.rmax, rnorm);
+    }
+    S[par ^ 1] = nx;
+  }
+  if (ru) return;
+  const float bt = (float)beta;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const float zi = invd ? invd[i] * r[i] : r[i];
+    const float t = bt * p[i];
+    p[i] = t + zi;
+  }
+}
+
+// reliable update, part 1: r64 = b - A64 (x + d) in FP64 ; r32 = r64 ; partials ||r64||^2, <r32, M r32>
+__global__ __launch_bounds__(kThreads) void k_ru_res(SlicedDev m, const double *__restrict__ v64,
+                                                     const double *__restrict__ b, const double *__restrict__ x,
+                                                     const float *__restrict__ d32, double *__restrict__ r64,
+                                                     float *__restrict__ r32, const float *__restrict__ invd32,
+                                                     const CgState *S, int par, double *part) {
+  const CgState &nx = S[par ^ 1];
+  if (S[par].done || !nx.ru) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = xcd_block() * kWaves + (threadIdx.x >> 6), nw = gridDim.x * kWaves;
+  const int s0 = (int)((int64_t)m.ns * wave / nw), s1 = (int)((int64_t)m.ns * (wave + 1) / nw);
+  double acc[3] = {0, 0, 0};
+  for (int s = s0; s < s1; ++s) {
+    // A (x + d) as two products over the same slice
+    const double ax = slice_row<double, double>(m, v64, x, s, lane);
+    const int64_t bb = m.base[s];
+    const int meta = m.meta[s];
+    const int K = meta & 0x3fffffff;
+    const int row = s * kSlice + lane;
+    double ad = 0.0;
+    const double *vv = v64 + bb + lane;
+    if (!(meta >> 30)) {
+      const int32_t *off = m.offs + m.aux[s];
+      for (int k = 0; k < K; ++k) {
+        int c = min(max(row + off[k], 0), m.n - 1);
+        ad += vv[k * kSlice] * (double)d32[c];
+      }
+    } else {
+      const int32_t *ec = m.ecol + m.aux[s] + lane;
+      for (int k = 0; k < K; ++k) ad += vv[k * kSlice] * (double)d32[ec[k * kSlice]];
+    }
+    if (row < m.n) {
+      const double ri = b[row] - ax - ad;
+      r64[row] = ri;
+      const float rf = (float)ri;
+      r32[row] = rf;
+      const float zf = invd32 ? invd32[row] * rf : rf;
+      acc[0] += ri * ri;
+      acc[1] += (double)(rf * zf);
+      acc[2] += (double)(rf * rf);
+    }
+  }
+  write_partials<3>(acc, part);
+}
+
+// reliable update, part 2: x += d ; d = 0 ; FP64 stopping test ; beta ; p = z + beta p
+__global__ __launch_bounds__(kThreads) void k_ru_fin(int32_t n, double *__restrict__ x, float *__restrict__ d32,
+                                                     float *__restrict__ p, const float *__restrict__ r,
+                                                     const float *__restrict__ invd, CgState *S, int par,
+                                                     const double *part_r, int gr, CgResult *R) {
+  // Fields of S[par^1] this kernel writes (rz, beta, rmax, est, converged,
+  // updates, done) are never read by its other blocks, which only read ru,
+  // iter, maxit and rhs_test there.
+  const CgState &nx = S[par ^ 1];
+  if (S[par].done) {   // keep `done` sticky in both parity slots
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[par ^ 1].done = 1;
+    return;
+  }
+  if (!nx.ru) return;
+  double red[3];
+  reduce_partials<3>(part_r, gr, red);
+  const double rnorm = sqrt(red[0]);
+  const int conv = rnorm <= nx.rhs_test;
+  const bool done = conv || nx.iter >= nx.maxit;
+  double beta = 1.0 / S[par].rz;
+  beta = beta * red[1];
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    x[i] = x[i] + (double)d32[i];
+    d32[i] = 0.0f;
+    if (!done) {
+      const float zi = invd ? invd[i] * r[i] : r[i];
+      const float t = (float)beta * p[i];
+      p[i] = t + zi;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CgState &w = S[par ^ 1];
+    w.rz = red[1];
+    w.beta = beta;
+    w.rmax = sqrt(red[2]);
+    w.est = rnorm;
+    w.converged = conv;
+    const int upd = w.updates + 1;
+    w.updates = upd;
+    if (done) w.done = 1;
+    R->iter = w.iter;
+    R->est = rnorm;
+    R->rnorm = rnorm;
+    R->converged = conv;
+    R->done = done;
+    R->updates = upd;
+  }
+}
+
+// ------------------------------------------------------------------ helpers ---
+
 // Jacobi setup: invD = 1 / diag(A) (csr_sparse_matrix.f90:4231-4256, serial_scalar_array.f90:581-588)
 __global__ void k_jacobi(int32_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                          const double *__restrict__ val, double *__restrict__ invd, float *__restrict__ invd32,
-                         CgState *st) {
+                         int *zero_diag) {
   int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int32_t lo = rowptr[i], hi = rowptr[i + 1] - 1;
@@ -282,18 +540,28 @@ __global__ void k_jacobi(int32_t n, const int32_t *__restrict__ rowptr, const in
   while (lo <= hi) {
     int32_t mid = (lo + hi) >> 1;
     int32_t c = col[mid];
-    if (c == i) { d = val[mid]; break; }
-    if (c < i) lo = mid + 1; else hi = mid - 1;
+    if (c == i) {
+      d = val[mid];
+      break;
+    }
+    if (c < i) lo = mid + 1;
+    else hi = mid - 1;
   }
-  if (d == 0.0) st->zero_diag = 1;
+  if (d == 0.0) *zero_diag = 1;
   double inv = 1.0 / d;
   invd[i] = inv;
   if (invd32) invd32[i] = (float)inv;
 }
 
-__global__ void k_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = (float)a[i];
+// gather CSR values into the sliced layout (FP64 and/or FP32)
+__global__ void k_fill_sliced(int64_t E, const int32_t *__restrict__ map, const double *__restrict__ val,
+                              double *__restrict__ v64, float *__restrict__ v32) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t k = map[e];
+    const double a = k >= 0 ? val[k] : 0.0;
+    if (v64) v64[e] = a;
+    if (v32) v32[e] = (float)a;
+  }
 }
 
 __global__ void k_fill0(int64_t n, double *__restrict__ a) {
@@ -301,118 +569,28 @@ __global__ void k_fill0(int64_t n, double *__restrict__ a) {
     a[i] = 0.0;
 }
 
-// ---------------------------------------------------------------- refinement ---
+__global__ void k_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
 
-// r64 = b - A64 x64 ; ||r||, ||b|| ; outer test ; prepare the inner FP32 rhs
-template <int LPR>
-__global__ __launch_bounds__(kThreads) void k_ir_residual(int32_t n, const int32_t *__restrict__ rowptr,
-                                                          const int32_t *__restrict__ col,
-                                                          const double *__restrict__ val, const double *__restrict__ b,
-                                                          const double *__restrict__ x, int first, int x_zero,
-                                                          double *__restrict__ r, IrState *ir, CgState *in,
-                                                          double *part, unsigned *cnt) {
-  if (ir->done) return;
-  const int lane = threadIdx.x % LPR;
-  const int64_t grp = ((int64_t)blockIdx.x * kThreads + threadIdx.x) / LPR;
-  const int64_t ngrp = (int64_t)gridDim.x * kThreads / LPR;
-  double v[2] = {0, 0};
-  for (int64_t rr = grp; rr < n; rr += ngrp) {
-    const int32_t row = (int32_t)rr;
-    double ax = x_zero ? 0.0 : row_dot<LPR, double, double>(rowptr, col, val, x, row, lane);
-    if (lane == 0) {
-      double bi = b[row];
-      double ri = bi - ax;
-      r[row] = ri;
-      v[0] += ri * ri;
-      v[1] += bi * bi;
+// partials of ||b - A x||^2 (reporting only)
+__global__ __launch_bounds__(kThreads) void k_true_res(SlicedDev m, const double *__restrict__ v64,
+                                                       const double *__restrict__ b, const double *__restrict__ x,
+                                                       double *part) {
+  const int lane = threadIdx.x & 63;
+  const int wave = xcd_block() * kWaves + (threadIdx.x >> 6), nw = gridDim.x * kWaves;
+  const int s0 = (int)((int64_t)m.ns * wave / nw), s1 = (int)((int64_t)m.ns * (wave + 1) / nw);
+  double acc[1] = {0};
+  for (int s = s0; s < s1; ++s) {
+    const double ax = slice_row<double, double>(m, v64, x, s, lane);
+    const int row = s * kSlice + lane;
+    if (row < m.n) {
+      const double ri = b[row] - ax;
+      acc[0] += ri * ri;
     }
   }
-  if (!publish_partials<2>(v, part, cnt)) return;
-  gather_partials<2>(v, part, cnt);
-  if (threadIdx.x == 0) {
-    const double rnorm = sqrt(v[0]);
-    if (first) {
-      ir->bnorm = sqrt(v[1]);
-      ir->r0norm = rnorm;
-      ir->rhs_test = (ir->crit == FEMHIP_RES_RHS ? ir->bnorm : rnorm) * ir->rtol + ir->atol;
-      ir->outer = 0;
-      ir->total_inner = 0;
-    }
-    ir->rnorm = rnorm;
-    int conv = rnorm <= ir->rhs_test;
-    ir->converged = conv;
-    if (conv || ir->outer >= ir->max_outer) ir->done = 1;
-    else in->done = 0;
-  }
-}
-
-// inner start: r32 = r64 / ||r64|| ; z = M r ; p = z ; d = 0 ; inner state init
-__global__ __launch_bounds__(kThreads) void k_ir_inner_init(int32_t n, const double *__restrict__ r64,
-                                                            float *__restrict__ r32, float *__restrict__ p32,
-                                                            float *__restrict__ d32, const float *__restrict__ invd32,
-                                                            const IrState *ir, CgState *in, double *part,
-                                                            unsigned *cnt) {
-  if (ir->done) return;
-  const double s = 1.0 / ir->rnorm;
-  double v[2] = {0, 0};
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    float ri = (float)(r64[i] * s);
-    float zi = invd32 ? invd32[i] * ri : ri;
-    r32[i] = ri;
-    p32[i] = zi;
-    d32[i] = 0.0f;
-    v[0] += (double)(ri * zi);
-    v[1] += (double)(ri * ri);
-  }
-  if (!publish_partials<2>(v, part, cnt)) return;
-  gather_partials<2>(v, part, cnt);
-  if (threadIdx.x == 0) {
-    in->rz = v[0];
-    in->rnrm_m = sqrt(v[0]);
-    in->r0norm = sqrt(v[1]);
-    in->rhs_test = ir->inner_rtol * sqrt(v[1]);
-    in->iter = 0;
-    in->done = 0;
-    in->converged = 0;
-  }
-}
-
-// x64 += ||r64|| * d32 ; bookkeeping of the outer step
-__global__ __launch_bounds__(kThreads) void k_ir_update(int32_t n, double *__restrict__ x,
-                                                        const float *__restrict__ d32, IrState *ir, CgState *in,
-                                                        unsigned *cnt) {
-  if (ir->done) return;
-  const double s = ir->rnorm;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
-    x[i] = x[i] + s * (double)d32[i];
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    last = (atomicAdd(cnt, 1u) == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    *cnt = 0u;
-    ir->outer += 1;
-    ir->total_inner += in->iter;
-  }
-}
-
-int pick_lpr(double avg_row) {
-  if (avg_row <= 6) return 4;
-  if (avg_row <= 14) return 8;
-  if (avg_row <= 40) return 16;
-  return 32;
-}
-
-template <typename F>
-void dispatch_lpr(int lpr, F &&f) {
-  switch (lpr) {
-    case 4: f(std::integral_constant<int, 4>()); break;
-    case 8: f(std::integral_constant<int, 8>()); break;
-    case 16: f(std::integral_constant<int, 16>()); break;
-    default: f(std::integral_constant<int, 32>()); break;
-  }
+  write_partials<1>(acc, part);
 }
 
 }  // namespace
@@ -425,26 +603,40 @@ struct femhip_solver {
   femhip_operator *op = nullptr;
   femhip_cg_params_t prm{};
   int32_t n = 0;
-  int nblk = 0, lpr = 8;
-  DevBuf<double> x, b, r, p, q, invd, part, hist;
+  int g_mat = 8, g_vec = 8;      // grid sizes of the matrix and vector kernels
+  DevBuf<double> x, b, r, p, q, invd, part_i, part_s, part_u, part_r, hist;
   DevBuf<float> r32, p32, q32, d32, invd32;
-  DevBuf<CgState> st, st32;
-  DevBuf<IrState> ir;
-  DevBuf<unsigned> cnt;
-  CgState *h_st = nullptr;   // pinned
-  CgState *h_st2 = nullptr;  // pinned, second polling slot
-  IrState *h_ir = nullptr;   // pinned
+  DevBuf<CgState> st;            // S[2]
+  DevBuf<CgResult> res;
+  DevBuf<int> flag;
+  CgResult *h_res[2] = {nullptr, nullptr};   // pinned polling slots
+  hipGraphExec_t graph = nullptr;
+  int graph_kind = -1, graph_chunk = 0;
   bool prepared = false;
   bool have_solution = false;
   ~femhip_solver() {
-    if (h_st) (void)hipHostFree(h_st);
-    if (h_st2) (void)hipHostFree(h_st2);
-    if (h_ir) (void)hipHostFree(h_ir);
+    for (auto *h : h_res)
+      if (h) (void)hipHostFree(h);
+    if (graph) (void)hipGraphExecDestroy(graph);
   }
 };
 
 namespace femhip {
 namespace {
+
+SlicedDev sliced_dev(femhip_operator &op) {
+  SlicedDev m;
+  m.ns = op.sl.ns;
+  m.n = op.n;
+  m.base = op.sl.base.p;
+  m.meta = op.sl.meta.p;
+  m.aux = op.sl.aux.p;
+  m.offs = op.sl.offs.p;
+  m.ecol = op.sl.ecol.p;
+  return m;
+}
+
+bool mixed(const femhip_solver &s) { return s.prm.precision == FEMHIP_PRECISION_FP32_REFINED; }
 
 void prepare(femhip_solver &s) {
   femhip_operator &op = *s.op;
@@ -452,191 +644,187 @@ void prepare(femhip_solver &s) {
   hipStream_t st = s.ctx->stream;
   const int32_t n = op.n;
   const bool jac = s.prm.preconditioner == FEMHIP_PREC_JACOBI;
-  const bool mixed = s.prm.precision == FEMHIP_PRECISION_FP32_REFINED;
+  const bool mx = mixed(s);
+  if (!op.sl.built) build_sliced(op);
   if (!s.prepared) {
     s.n = n;
+    const int ns = op.sl.ns;
+    // about 4 slices per wavefront for the matrix kernels, multiple of 8 blocks
+    s.g_mat = std::max(8, std::min(2048, (ns + 4 * kWaves - 1) / (4 * kWaves)));
+    s.g_mat = (s.g_mat + 7) / 8 * 8;
+    s.g_vec = std::max(8, std::min(1024, (n + 4 * kThreads - 1) / (4 * kThreads)));
     s.x.alloc(n); s.b.alloc(n); s.r.alloc(n); s.p.alloc(n); s.q.alloc(n);
-    s.nblk = std::max(1, std::min(2048, (n + kThreads - 1) / kThreads));
-    s.part.alloc((size_t)4 * s.nblk);
+    s.part_i.alloc((size_t)4 * s.g_mat);
+    s.part_s.alloc((size_t)2 * s.g_mat);
+    s.part_u.alloc((size_t)2 * s.g_vec);
+    s.part_r.alloc((size_t)3 * s.g_mat);
     s.hist.alloc((size_t)std::max(1, s.prm.max_num_iterations));
-    s.st.alloc(1); s.st32.alloc(1); s.ir.alloc(1); s.cnt.alloc(4);
+    s.st.alloc(2);
+    s.res.alloc(1);
+    s.flag.alloc(1);
     if (jac) s.invd.alloc(n);
-    if (mixed) {
+    if (mx) {
       s.r32.alloc(n); s.p32.alloc(n); s.q32.alloc(n); s.d32.alloc(n);
       if (jac) s.invd32.alloc(n);
     }
-    FH_HIP(hipHostMalloc((void **)&s.h_st, sizeof(CgState), hipHostMallocDefault));
-    FH_HIP(hipHostMalloc((void **)&s.h_st2, sizeof(CgState), hipHostMallocDefault));
-    FH_HIP(hipHostMalloc((void **)&s.h_ir, sizeof(IrState), hipHostMallocDefault));
-    FH_HIP(hipMemsetAsync(s.cnt.p, 0, 4 * sizeof(unsigned), st));
-    s.lpr = pick_lpr(n ? (double)op.nnz_full / n : 1.0);
+    for (auto &h : s.h_res) FH_HIP(hipHostMalloc((void **)&h, sizeof(CgResult), hipHostMallocDefault));
     s.prepared = true;
   }
-  // matrix-dependent setup (every solve: the operator may have been recomputed)
+  // matrix-dependent work, redone whenever the operator was recomputed
+  if (op.sl.filled_version != op.version || (mx && op.sl.filled_version32 != op.version)) {
+    if (!op.sl.v64.p) op.sl.v64.alloc((size_t)std::max<int64_t>(1, op.sl.entries));
+    if (mx && !op.sl.v32.p) op.sl.v32.alloc((size_t)std::max<int64_t>(1, op.sl.entries));
+    hipLaunchKernelGGL(k_fill_sliced, dim3(2048), dim3(256), 0, st, op.sl.entries, op.sl.map.p, op.val.p,
+                       op.sl.v64.p, mx ? op.sl.v32.p : nullptr);
+    FH_HIP(hipGetLastError());
+    op.sl.filled_version = op.version;
+    if (mx) op.sl.filled_version32 = op.version;
+  }
+  if (jac && n > 0) {
+    int zero = 0;
+    FH_HIP(hipMemcpyAsync(s.flag.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_jacobi, dim3((n + 255) / 256), dim3(256), 0, st, n, op.rowptr.p, op.col.p, op.val.p,
+                       s.invd.p, mx ? s.invd32.p : nullptr, s.flag.p);
+    FH_HIP(hipGetLastError());
+    FH_HIP(hipMemcpyAsync(&zero, s.flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    FH_CHECK(!zero, FEMHIP_ERR_ZERO_DIAGONAL, "op1 cannot be entrywise inverted; there exists a null diagonal entry");
+  }
+}
+
+// Enqueue `chunk` iterations (chunk even: iteration parity restarts at 0).
+void enqueue_iterations(femhip_solver &s, int chunk) {
+  femhip_operator &op = *s.op;
+  hipStream_t st = s.ctx->stream;
+  const SlicedDev m = sliced_dev(op);
+  const int32_t n = s.n;
+  const bool jac = s.prm.preconditioner == FEMHIP_PREC_JACOBI;
+  const dim3 B(kThreads), GM(s.g_mat), GV(s.g_vec);
+  for (int i = 0; i < chunk; ++i) {
+    const int par = i & 1;
+    if (!mixed(s)) {
+      const double *invd = jac ? s.invd.p : nullptr;
+      hipLaunchKernelGGL((k_spmv<double, double>), GM, B, 0, st, m, op.sl.v64.p, s.p.p, s.q.p, s.st.p, par,
+                         s.part_s.p);
+      hipLaunchKernelGGL((k_update<double, double>), GV, B, 0, st, n, s.x.p, s.r.p, s.p.p, s.q.p, invd, s.st.p, par,
+                         s.part_s.p, s.g_mat, s.part_u.p);
+      hipLaunchKernelGGL((k_dir<double>), GV, B, 0, st, n, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec,
+                         s.hist.p, s.res.p);
+    } else {
+      const float *invd = jac ? s.invd32.p : nullptr;
+      hipLaunchKernelGGL((k_spmv<float, float>), GM, B, 0, st, m, op.sl.v32.p, s.p32.p, s.q32.p, s.st.p, par,
+                         s.part_s.p);
+      hipLaunchKernelGGL((k_update<float, float>), GV, B, 0, st, n, s.d32.p, s.r32.p, s.p32.p, s.q32.p, invd, s.st.p,
+                         par, s.part_s.p, s.g_mat, s.part_u.p);
+      hipLaunchKernelGGL(k_dir_ru, GV, B, 0, st, n, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec);
+      hipLaunchKernelGGL(k_ru_res, GM, B, 0, st, m, op.sl.v64.p, s.b.p, s.x.p, s.d32.p, s.r.p, s.r32.p, invd, s.st.p,
+                         par, s.part_r.p);
+      hipLaunchKernelGGL(k_ru_fin, GV, B, 0, st, n, s.x.p, s.d32.p, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_r.p,
+                         s.g_mat, s.res.p);
+    }
+  }
+}
+
+constexpr int kChunk = 16;
+
+void launch_chunk(femhip_solver &s) {
+  hipStream_t st = s.ctx->stream;
+  const int kind = mixed(s) ? 1 : 0;
+  static const bool no_graph = getenv("FEMHIP_NO_GRAPH") != nullptr;
+  if (no_graph) {
+    enqueue_iterations(s, kChunk);
+    FH_HIP(hipGetLastError());
+    return;
+  }
+  if (!s.graph || s.graph_kind != kind) {
+    if (s.graph) FH_HIP(hipGraphExecDestroy(s.graph));
+    s.graph = nullptr;
+    hipGraph_t g;
+    FH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue_iterations(s, kChunk);
+    FH_HIP(hipStreamEndCapture(st, &g));
+    FH_HIP(hipGraphInstantiate(&s.graph, g, nullptr, nullptr, 0));
+    FH_HIP(hipGraphDestroy(g));
+    s.graph_kind = kind;
+  }
+  FH_HIP(hipGraphLaunch(s.graph, st));
+}
+
+// launch chunks until the device state says done; one chunk of look-ahead
+void iterate(femhip_solver &s, int maxit) {
+  hipStream_t st = s.ctx->stream;
+  hipEvent_t ev[2];
+  FH_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  FH_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int queued = 0, cur = 0;
+  auto push = [&](int slot) {
+    launch_chunk(s);
+    queued += kChunk;
+    FH_HIP(hipMemcpyAsync(s.h_res[slot], s.res.p, sizeof(CgResult), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipEventRecord(ev[slot], st));
+  };
+  push(0);
+  for (;;) {
+    const bool more = queued < maxit;
+    if (more) push(cur ^ 1);
+    FH_HIP(hipEventSynchronize(ev[cur]));
+    if (s.h_res[cur]->done || !more) break;
+    cur ^= 1;
+  }
+  FH_HIP(hipStreamSynchronize(st));
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+}
+
+void upload_state(femhip_solver &s) {
   CgState h{};
   h.rtol = s.prm.rtol;
   h.atol = s.prm.atol;
   h.maxit = s.prm.max_num_iterations;
   h.crit = s.prm.stopping_criterion;
-  FH_HIP(hipMemcpyAsync(s.st.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
-  if (jac && n > 0) {
-    hipLaunchKernelGGL(k_jacobi, dim3((n + 255) / 256), dim3(256), 0, st, n, op.rowptr.p, op.col.p, op.val.p,
-                       s.invd.p, mixed ? s.invd32.p : nullptr, s.st.p);
-    FH_HIP(hipGetLastError());
-    FH_HIP(hipMemcpyAsync(s.h_st, s.st.p, sizeof(CgState), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipStreamSynchronize(st));
-    FH_CHECK(!s.h_st->zero_diag, FEMHIP_ERR_ZERO_DIAGONAL,
-             "op1 cannot be entrywise inverted; there exists a null diagonal entry");
-  }
-  if (mixed) {
-    op.val32.alloc((size_t)op.nnz_full);
-    hipLaunchKernelGGL(k_to_f32, dim3(2048), dim3(256), 0, st, op.nnz_full, op.val.p, op.val32.p);
-    FH_HIP(hipGetLastError());
-  }
+  h.delta = s.prm.inner_rtol > 0 ? s.prm.inner_rtol : 0.1;
+  FH_HIP(hipMemcpyAsync(s.st.p, &h, sizeof(h), hipMemcpyHostToDevice, s.ctx->stream));
 }
 
-void poll(femhip_solver &s, const void *dsrc, void *hdst, size_t bytes, hipEvent_t ev) {
-  FH_HIP(hipMemcpyAsync(hdst, dsrc, bytes, hipMemcpyDeviceToHost, s.ctx->stream));
-  FH_HIP(hipEventRecord(ev, s.ctx->stream));
-}
-
-// run CG iterations on (rowptr, col, val) until the state says done
-template <typename T, typename A>
-void run_iterations(femhip_solver &s, const T *val, T *x, T *r, T *p, T *q, const T *invd, CgState *dst,
-                    double *hist, int maxit) {
+void solve(femhip_solver &s, bool x_zero, femhip_cg_result_t *out) {
   hipStream_t st = s.ctx->stream;
   femhip_operator &op = *s.op;
-  const int32_t n = s.n;
-  const int nb = s.nblk;
-  auto enqueue = [&](int k) {
-    for (int i = 0; i < k; ++i) {
-      dispatch_lpr(s.lpr, [&](auto L) {
-        hipLaunchKernelGGL((k_cg_spmv<decltype(L)::value, T, A>), dim3(nb), dim3(kThreads), 0, st, n, op.rowptr.p,
-                           op.col.p, val, p, q, dst, s.part.p, s.cnt.p + 0);
-      });
-      hipLaunchKernelGGL((k_cg_update<T, A>), dim3(nb), dim3(kThreads), 0, st, n, x, r, p, q, invd, dst, s.part.p,
-                         s.cnt.p + 1, hist);
-      hipLaunchKernelGGL((k_cg_dir<T>), dim3(nb), dim3(kThreads), 0, st, n, p, r, invd, dst);
-    }
-  };
-  hipEvent_t ev[2];
-  FH_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  FH_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  CgState *slot[2] = {s.h_st, s.h_st2};
-  const int chunk = 8;
-  int queued = 0, cur = 0;
-  enqueue(chunk);
-  queued += chunk;
-  poll(s, dst, slot[cur], sizeof(CgState), ev[cur]);
-  for (;;) {
-    bool more = queued < maxit;
-    if (more) {
-      enqueue(chunk);
-      queued += chunk;
-      poll(s, dst, slot[cur ^ 1], sizeof(CgState), ev[cur ^ 1]);
-    }
-    FH_HIP(hipEventSynchronize(ev[cur]));
-    if (slot[cur]->done || !more) break;
-    cur ^= 1;
+  const SlicedDev m = sliced_dev(op);
+  const bool jac = s.prm.preconditioner == FEMHIP_PREC_JACOBI;
+  upload_state(s);
+  if (!mixed(s)) {
+    hipLaunchKernelGGL((k_init<double>), dim3(s.g_mat), dim3(kThreads), 0, st, m, op.sl.v64.p, s.b.p, s.x.p,
+                       x_zero ? 1 : 0, s.r.p, s.p.p, jac ? s.invd.p : nullptr, s.part_i.p);
+  } else {
+    hipLaunchKernelGGL(k_ru_init, dim3(s.g_mat), dim3(kThreads), 0, st, m, op.sl.v64.p, s.b.p, s.x.p, x_zero ? 1 : 0,
+                       s.r.p, s.r32.p, s.p32.p, s.d32.p, jac ? s.invd32.p : nullptr, s.part_i.p);
   }
-  FH_HIP(hipStreamSynchronize(st));
-  FH_HIP(hipMemcpy(s.h_st, dst, sizeof(CgState), hipMemcpyDeviceToHost));
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
+  hipLaunchKernelGGL(mixed(s) ? k_ru_init_fin : k_init_fin, dim3(1), dim3(kThreads), 0, st, s.part_i.p, s.g_mat,
+                     s.st.p, s.res.p);
+  FH_HIP(hipGetLastError());
+  iterate(s, s.prm.max_num_iterations);
+  CgResult r;
+  FH_HIP(hipMemcpy(&r, s.res.p, sizeof(r), hipMemcpyDeviceToHost));
+  out->num_iterations = r.iter;
+  out->did_converge = r.converged;
+  out->outer_iterations = mixed(s) ? r.updates : 0;
+  out->error_estimate = r.est;
+  out->rhs_convergence_test = r.rhs_test;
+  out->rhs_norm = r.bnorm;
 }
 
 double true_residual(femhip_solver &s) {
-  // ||b - A x||_2 in FP64 (reporting only)
   hipStream_t st = s.ctx->stream;
   femhip_operator &op = *s.op;
-  CgState h{};
-  h.crit = FEMHIP_RES_RES;
-  FH_HIP(hipMemcpyAsync(s.st32.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
-  dispatch_lpr(s.lpr, [&](auto L) {
-    hipLaunchKernelGGL((k_cg_init<decltype(L)::value, double>), dim3(s.nblk), dim3(kThreads), 0, st, s.n,
-                       op.rowptr.p, op.col.p, op.val.p, s.b.p, s.x.p, 0, s.q.p, s.q.p, (const double *)nullptr,
-                       s.st32.p, s.part.p, s.cnt.p + 2);
-  });
+  hipLaunchKernelGGL(k_true_res, dim3(s.g_mat), dim3(kThreads), 0, st, sliced_dev(op), op.sl.v64.p, s.b.p, s.x.p,
+                     s.part_i.p);
   FH_HIP(hipGetLastError());
-  CgState out{};
-  FH_HIP(hipMemcpyAsync(&out, s.st32.p, sizeof(out), hipMemcpyDeviceToHost, st));
+  std::vector<double> h((size_t)s.g_mat);
+  FH_HIP(hipMemcpyAsync(h.data(), s.part_i.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
   FH_HIP(hipStreamSynchronize(st));
-  return out.r0norm;
-}
-
-void solve_fp64(femhip_solver &s, bool x_zero, femhip_cg_result_t *res) {
-  hipStream_t st = s.ctx->stream;
-  femhip_operator &op = *s.op;
-  const double *invd = s.prm.preconditioner == FEMHIP_PREC_JACOBI ? s.invd.p : nullptr;
-  dispatch_lpr(s.lpr, [&](auto L) {
-    hipLaunchKernelGGL((k_cg_init<decltype(L)::value, double>), dim3(s.nblk), dim3(kThreads), 0, st, s.n,
-                       op.rowptr.p, op.col.p, op.val.p, s.b.p, s.x.p, x_zero ? 1 : 0, s.r.p, s.p.p, invd, s.st.p,
-                       s.part.p, s.cnt.p + 2);
-  });
-  FH_HIP(hipGetLastError());
-  run_iterations<double, double>(s, op.val.p, s.x.p, s.r.p, s.p.p, s.q.p, invd, s.st.p, s.hist.p,
-                                 s.prm.max_num_iterations);
-  const CgState &h = *s.h_st;
-  res->num_iterations = h.iter;
-  res->did_converge = h.converged;
-  res->outer_iterations = 0;
-  res->error_estimate = h.est;
-  res->rhs_convergence_test = h.rhs_test;
-  res->rhs_norm = h.bnorm;
-}
-
-void solve_refined(femhip_solver &s, bool x_zero, femhip_cg_result_t *res) {
-  hipStream_t st = s.ctx->stream;
-  femhip_operator &op = *s.op;
-  const bool jac = s.prm.preconditioner == FEMHIP_PREC_JACOBI;
-  IrState hi{};
-  hi.rtol = s.prm.rtol;
-  hi.atol = s.prm.atol;
-  hi.inner_rtol = s.prm.inner_rtol > 0 ? s.prm.inner_rtol : 1e-4;
-  hi.max_outer = s.prm.max_outer > 0 ? s.prm.max_outer : 30;
-  hi.crit = s.prm.stopping_criterion == FEMHIP_RES_RHS ? FEMHIP_RES_RHS : FEMHIP_RES_RES;
-  FH_HIP(hipMemcpyAsync(s.ir.p, &hi, sizeof(hi), hipMemcpyHostToDevice, st));
-  CgState h32{};
-  h32.maxit = s.prm.max_num_iterations;
-  h32.crit = FEMHIP_RES_RES;
-  h32.done = 1;
-  FH_HIP(hipMemcpyAsync(s.st32.p, &h32, sizeof(h32), hipMemcpyHostToDevice, st));
-  if (x_zero) hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s.n, s.x.p);
-  int first = 1;
-  int inner_budget = s.prm.max_num_iterations;
-  for (;;) {
-    dispatch_lpr(s.lpr, [&](auto L) {
-      hipLaunchKernelGGL((k_ir_residual<decltype(L)::value>), dim3(s.nblk), dim3(kThreads), 0, st, s.n,
-                         op.rowptr.p, op.col.p, op.val.p, s.b.p, s.x.p, first, first && x_zero ? 1 : 0, s.r.p,
-                         s.ir.p, s.st32.p, s.part.p, s.cnt.p + 2);
-    });
-    FH_HIP(hipGetLastError());
-    first = 0;
-    FH_HIP(hipMemcpyAsync(s.h_ir, s.ir.p, sizeof(IrState), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipStreamSynchronize(st));
-    if (s.h_ir->done || inner_budget <= 0) break;
-    hipLaunchKernelGGL(k_ir_inner_init, dim3(s.nblk), dim3(kThreads), 0, st, s.n, s.r.p, s.r32.p, s.p32.p, s.d32.p,
-                       jac ? s.invd32.p : nullptr, s.ir.p, s.st32.p, s.part.p, s.cnt.p + 3);
-    // inner FP32 PCG with its own iteration budget (the remaining total)
-    CgState lim{};
-    FH_HIP(hipMemcpyAsync(&lim, s.st32.p, sizeof(lim), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipStreamSynchronize(st));
-    lim.maxit = inner_budget;
-    FH_HIP(hipMemcpyAsync(s.st32.p, &lim, sizeof(lim), hipMemcpyHostToDevice, st));
-    run_iterations<float, float>(s, op.val32.p, s.d32.p, s.r32.p, s.p32.p, s.q32.p, jac ? s.invd32.p : nullptr,
-                                 s.st32.p, nullptr, inner_budget);
-    inner_budget -= s.h_st->iter;
-    hipLaunchKernelGGL(k_ir_update, dim3(s.nblk), dim3(kThreads), 0, st, s.n, s.x.p, s.d32.p, s.ir.p, s.st32.p,
-                       s.cnt.p + 3);
-    FH_HIP(hipGetLastError());
-  }
-  const IrState &h = *s.h_ir;
-  res->num_iterations = h.total_inner;
-  res->did_converge = h.converged;
-  res->outer_iterations = h.outer;
-  res->error_estimate = h.rnorm;
-  res->rhs_convergence_test = h.rhs_test;
-  res->rhs_norm = h.bnorm;
+  double sum = 0.0;
+  for (double v : h) sum += v;
+  return std::sqrt(sum);
 }
 
 }  // namespace
@@ -660,6 +848,7 @@ int femhip_solver_create(femhip_ctx_t *ctx, femhip_operator_t *op, const femhip_
     FH_CHECK(p.precision == FEMHIP_PRECISION_FP64 || p.stopping_criterion == FEMHIP_RES_RES ||
                  p.stopping_criterion == FEMHIP_RES_RHS,
              FEMHIP_ERR_INVALID, "FP32_REFINED supports the RES_RES and RES_RHS criteria");
+    FH_CHECK(p.inner_rtol >= 0 && p.inner_rtol < 1, FEMHIP_ERR_INVALID, "reliable-update factor must be in [0,1)");
     FH_HIP(hipSetDevice(ctx->device));
     auto *s = new femhip_solver();
     s->ctx = ctx_retain(ctx);
@@ -679,29 +868,23 @@ int femhip_solver_solve(femhip_solver_t *s, const double *b, double *x, femhip_c
     FH_CHECK(s && res, FEMHIP_ERR_INVALID, "null argument");
     FH_HIP(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
+    std::memset(res, 0, sizeof(*res));
     hipEvent_t e0, e1;
     FH_HIP(hipEventCreate(&e0));
     FH_HIP(hipEventCreate(&e1));
     FH_HIP(hipEventRecord(e0, st));
     prepare(*s);
     const size_t bytes = sizeof(double) * (size_t)s->n;
-    if (b) FH_HIP(hipMemcpyAsync(s->b.p, b, bytes, hipMemcpyHostToDevice, st));
-    else FH_HIP(hipMemcpyAsync(s->b.p, s->op->rhs.p, bytes, hipMemcpyDeviceToDevice, st));
-    bool x_zero = (x == nullptr);
-    if (x) FH_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, st));
-    else hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s->n, s->x.p);
-    std::memset(res, 0, sizeof(*res));
-    if (s->n == 0) {   // no free DOFs: nothing to solve
-      res->did_converge = 1;
-      FH_HIP(hipEventRecord(e1, st));
-      FH_HIP(hipEventSynchronize(e1));
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-      s->have_solution = true;
-      return FEMHIP_OK;
+    const bool x_zero = (x == nullptr);
+    if (s->n > 0) {
+      if (b) FH_HIP(hipMemcpyAsync(s->b.p, b, bytes, hipMemcpyHostToDevice, st));
+      else FH_HIP(hipMemcpyAsync(s->b.p, s->op->rhs.p, bytes, hipMemcpyDeviceToDevice, st));
+      if (x) FH_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, st));
+      else hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s->n, s->x.p);
+      solve(*s, x_zero, res);
+    } else {
+      res->did_converge = 1;   // no free DOFs: nothing to solve
     }
-    if (s->prm.precision == FEMHIP_PRECISION_FP64) solve_fp64(*s, x_zero, res);
-    else solve_refined(*s, x_zero, res);
     FH_HIP(hipEventRecord(e1, st));
     FH_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -709,10 +892,10 @@ int femhip_solver_solve(femhip_solver_t *s, const double *b, double *x, femhip_c
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     res->solve_ms = ms;
-    res->true_residual = true_residual(*s);
+    if (s->n > 0) res->true_residual = true_residual(*s);
     res->status = res->did_converge ? FEMHIP_OK : FEMHIP_ERR_NOT_CONVERGED;
     s->have_solution = true;
-    if (x) FH_HIP(hipMemcpy(x, s->x.p, bytes, hipMemcpyDeviceToHost));
+    if (x && s->n > 0) FH_HIP(hipMemcpy(x, s->x.p, bytes, hipMemcpyDeviceToHost));
     return FEMHIP_OK;
   } catch (const Error &e) {
     set_error(e.msg);
@@ -725,25 +908,21 @@ int femhip_solver_profile_spmv(femhip_solver_t *s, int32_t reps, double *ms_per_
     FH_CHECK(s && reps > 0, FEMHIP_ERR_INVALID, "bad argument");
     FH_HIP(hipSetDevice(s->ctx->device));
     prepare(*s);
+    FH_CHECK(s->n > 0, FEMHIP_ERR_STATE, "empty operator");
     femhip_operator &op = *s->op;
     hipStream_t st = s->ctx->stream;
-    const bool mixed = s->prm.precision == FEMHIP_PRECISION_FP32_REFINED;
-    CgState h{};
-    h.rz = 1.0;
-    FH_HIP(hipMemcpyAsync(s->st32.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
+    const bool mx = mixed(*s);
+    upload_state(*s);   // done = 0
     hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s->n, s->p.p);
-    if (mixed) hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, st, (int64_t)s->n, s->p.p, s->p32.p);
+    if (mx) hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, st, (int64_t)s->n, s->p.p, s->p32.p);
+    const SlicedDev m = sliced_dev(op);
     auto launch = [&] {
-      dispatch_lpr(s->lpr, [&](auto L) {
-        if (mixed)
-          hipLaunchKernelGGL((k_cg_spmv<decltype(L)::value, float, float>), dim3(s->nblk), dim3(kThreads), 0, st,
-                             s->n, op.rowptr.p, op.col.p, op.val32.p, s->p32.p, s->q32.p, s->st32.p, s->part.p,
-                             s->cnt.p + 0);
-        else
-          hipLaunchKernelGGL((k_cg_spmv<decltype(L)::value, double, double>), dim3(s->nblk), dim3(kThreads), 0, st,
-                             s->n, op.rowptr.p, op.col.p, op.val.p, s->p.p, s->q.p, s->st32.p, s->part.p,
-                             s->cnt.p + 0);
-      });
+      if (mx)
+        hipLaunchKernelGGL((k_spmv<float, float>), dim3(s->g_mat), dim3(kThreads), 0, st, m, op.sl.v32.p, s->p32.p,
+                           s->q32.p, s->st.p, 0, s->part_s.p);
+      else
+        hipLaunchKernelGGL((k_spmv<double, double>), dim3(s->g_mat), dim3(kThreads), 0, st, m, op.sl.v64.p, s->p.p,
+                           s->q.p, s->st.p, 0, s->part_s.p);
     };
     for (int i = 0; i < 3; ++i) launch();
     hipEvent_t e0, e1;
@@ -759,8 +938,13 @@ int femhip_solver_profile_spmv(femhip_solver_t *s, int32_t reps, double *ms_per_
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (ms_per_launch) *ms_per_launch = ms / reps;
-    const double vb = mixed ? 4.0 : 8.0;
-    if (bytes) *bytes = (double)op.nnz_full * (vb + 4.0) + 4.0 * (op.n + 1) + 2.0 * vb * op.n;
+    // algorithmic bytes of the sliced format: values + per-slice metadata +
+    // diagonal offsets + ELL indices + one read of p and one write of q
+    const double vb = mx ? 4.0 : 8.0;
+    const int64_t ell = (int64_t)(op.sl.ecol.n > 1 ? op.sl.ecol.n : 0);
+    if (bytes)
+      *bytes = (double)op.sl.entries * vb + 16.0 * op.sl.ns + 4.0 * (double)op.sl.offs.n + 4.0 * (double)ell +
+               2.0 * vb * op.n;
     return FEMHIP_OK;
   } catch (const Error &e) {
     set_error(e.msg);
@@ -773,7 +957,7 @@ int femhip_solver_get_solution(const femhip_solver_t *s, double *x) {
     FH_CHECK(s && x, FEMHIP_ERR_INVALID, "null argument");
     FH_CHECK(s->have_solution, FEMHIP_ERR_STATE, "no solution yet");
     FH_HIP(hipSetDevice(s->ctx->device));
-    FH_HIP(hipMemcpy(x, s->x.p, sizeof(double) * (size_t)s->n, hipMemcpyDeviceToHost));
+    if (s->n > 0) FH_HIP(hipMemcpy(x, s->x.p, sizeof(double) * (size_t)s->n, hipMemcpyDeviceToHost));
     return FEMHIP_OK;
   } catch (const Error &e) {
     set_error(e.msg);

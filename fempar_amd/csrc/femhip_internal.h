@@ -118,9 +118,36 @@ struct femhip_space {
   femhip::DevBuf<int32_t> d_dofs;     // device: free -> row (0-based), fixed -> -(fixed index)-1
 };
 
+namespace femhip {
+// Solver copy of the matrix in 64-row slices (one wavefront per slice).  A
+// slice whose rows share few distinct column offsets (col - row) is stored by
+// diagonals: K offsets per slice and K x 64 values, no column indices, and the
+// x gathers of one diagonal are contiguous.  Other slices are stored as ELL
+// with K = longest row and explicit column indices.  Values are laid out
+// [slice][k][lane]; padding entries hold 0 and a valid column.
+constexpr int kSlice = 64;
+struct Sliced {
+  int32_t ns = 0;                 // number of slices
+  int64_t entries = 0;            // padded entries
+  int64_t dia_slices = 0;
+  DevBuf<int64_t> base;           // ns+1 entry offsets
+  DevBuf<int32_t> meta;           // ns: K | (is_ell << 30)
+  DevBuf<int32_t> aux;            // ns: offset into offs (diagonal) or ecol (ELL)
+  DevBuf<int32_t> offs;           // diagonal offsets
+  DevBuf<int32_t> ecol;           // ELL column indices
+  DevBuf<int32_t> map;            // entries: CSR position, -1 for padding
+  DevBuf<double> v64;
+  DevBuf<float> v32;
+  int64_t filled_version = -1, filled_version32 = -1;
+  bool built = false;
+};
+}  // namespace femhip
+
 struct femhip_operator {
   femhip_ctx *ctx = nullptr;
   std::atomic<int> refs{1};           // the handle + every solver built on it
+  int64_t version = 0;                // bumped by every compute()
+  femhip::Sliced sl;                  // solver format (built on first solve)
   femhip_space *space = nullptr;
   int symmetric_storage = 0;
   int32_t n = 0;
@@ -148,6 +175,8 @@ namespace femhip {
 // host setup (space.cpp)
 void number_structured(femhip_space &s);
 void build_pattern(femhip_operator &op);
+// sliced solver format (sliced.cpp)
+void build_sliced(femhip_operator &op);
 
 // assembly (assemble.hip)
 void assemble(femhip_operator &op, const double *fixed_host, float *ms);

@@ -1,0 +1,126 @@
+// sliced.cpp -- host construction of the sliced diagonal/ELL solver format from
+// the CSR pattern (pattern only; values are gathered on the device through
+// `map` after every assembly).
+#include <algorithm>
+#include <thread>
+
+#include "femhip_internal.h"
+
+namespace femhip {
+
+void build_sliced(femhip_operator &op) {
+  Sliced &S = op.sl;
+  const int32_t n = op.n;
+  const int32_t ns = (n + kSlice - 1) / kSlice;
+  const std::vector<int32_t> &rp = op.h_rowptr;
+  const std::vector<int32_t> &cl = op.h_col;
+
+  // per slice: distinct offsets, longest row, storage choice
+  std::vector<std::vector<int32_t>> offsets((size_t)ns);
+  std::vector<int32_t> K((size_t)ns), is_ell((size_t)ns);
+  unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  auto work = [&](unsigned t) {
+    std::vector<int32_t> buf;
+    for (int32_t s = (int32_t)((int64_t)ns * t / nth); s < (int32_t)((int64_t)ns * (t + 1) / nth); ++s) {
+      buf.clear();
+      int32_t maxlen = 0;
+      for (int32_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r) {
+        maxlen = std::max(maxlen, rp[(size_t)r + 1] - rp[(size_t)r]);
+        for (int32_t k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) buf.push_back(cl[(size_t)k] - r);
+      }
+      std::sort(buf.begin(), buf.end());
+      buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+      // diagonals pay off when they cost at most ~25% more values than ELL
+      // (they save the 4-byte column index per entry)
+      const bool dia = (int64_t)buf.size() * 4 <= (int64_t)maxlen * 5 + 4;
+      is_ell[(size_t)s] = dia ? 0 : 1;
+      K[(size_t)s] = dia ? (int32_t)buf.size() : maxlen;
+      if (dia) offsets[(size_t)s] = buf;
+    }
+  };
+  {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nth; ++t) th.emplace_back(work, t);
+    for (auto &x : th) x.join();
+  }
+  std::vector<int64_t> base((size_t)ns + 1, 0);
+  std::vector<int32_t> meta((size_t)ns), aux((size_t)ns);
+  std::vector<int32_t> offs;
+  int64_t ell_entries = 0, dia = 0;
+  for (int32_t s = 0; s < ns; ++s) {
+    base[(size_t)s + 1] = base[(size_t)s] + (int64_t)K[(size_t)s] * kSlice;
+    meta[(size_t)s] = K[(size_t)s] | (is_ell[(size_t)s] << 30);
+    if (is_ell[(size_t)s]) {
+      aux[(size_t)s] = (int32_t)ell_entries;
+      ell_entries += (int64_t)K[(size_t)s] * kSlice;
+    } else {
+      aux[(size_t)s] = (int32_t)offs.size();
+      offs.insert(offs.end(), offsets[(size_t)s].begin(), offsets[(size_t)s].end());
+      ++dia;
+    }
+  }
+  const int64_t E = base[(size_t)ns];
+  FH_CHECK(E < (int64_t)INT32_MAX && ell_entries < (int64_t)INT32_MAX, FEMHIP_ERR_INVALID,
+           "sliced matrix exceeds 2^31 entries");
+  std::vector<int32_t> map((size_t)E, -1), ecol((size_t)std::max<int64_t>(1, ell_entries), 0);
+  auto fill = [&](unsigned t) {
+    for (int32_t s = (int32_t)((int64_t)ns * t / nth); s < (int32_t)((int64_t)ns * (t + 1) / nth); ++s) {
+      const int32_t k_s = K[(size_t)s];
+      for (int32_t lane = 0; lane < kSlice; ++lane) {
+        const int32_t r = s * kSlice + lane;
+        if (r >= n) {
+          if (is_ell[(size_t)s])
+            for (int32_t k = 0; k < k_s; ++k) ecol[(size_t)(aux[(size_t)s] + (int64_t)k * kSlice + lane)] = n - 1;
+          continue;
+        }
+        const int32_t b = rp[(size_t)r], e = rp[(size_t)r + 1];
+        if (is_ell[(size_t)s]) {
+          for (int32_t k = 0; k < k_s; ++k) {
+            const int64_t idx = base[(size_t)s] + (int64_t)k * kSlice + lane;
+            const int64_t ci = aux[(size_t)s] + (int64_t)k * kSlice + lane;
+            if (b + k < e) {
+              map[(size_t)idx] = b + k;
+              ecol[(size_t)ci] = cl[(size_t)(b + k)];
+            } else {
+              ecol[(size_t)ci] = r;   // padding: valid column, zero value
+            }
+          }
+        } else {
+          const std::vector<int32_t> &off = offsets[(size_t)s];
+          int32_t p = b;
+          for (int32_t k = 0; k < k_s; ++k) {   // both sorted: merge
+            const int32_t c = r + off[(size_t)k];
+            while (p < e && cl[(size_t)p] < c) ++p;
+            if (p < e && cl[(size_t)p] == c) map[(size_t)(base[(size_t)s] + (int64_t)k * kSlice + lane)] = p;
+          }
+        }
+      }
+    }
+  };
+  {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nth; ++t) th.emplace_back(fill, t);
+    for (auto &x : th) x.join();
+  }
+  S.ns = ns;
+  S.entries = E;
+  S.dia_slices = dia;
+  S.base.alloc((size_t)ns + 1);
+  S.meta.alloc((size_t)std::max(1, ns));
+  S.aux.alloc((size_t)std::max(1, ns));
+  S.offs.alloc(std::max<size_t>(1, offs.size()));
+  S.ecol.alloc(ecol.size());
+  S.map.alloc((size_t)std::max<int64_t>(1, E));
+  FH_HIP(hipMemcpy(S.base.p, base.data(), sizeof(int64_t) * base.size(), hipMemcpyHostToDevice));
+  if (ns) {
+    FH_HIP(hipMemcpy(S.meta.p, meta.data(), sizeof(int32_t) * meta.size(), hipMemcpyHostToDevice));
+    FH_HIP(hipMemcpy(S.aux.p, aux.data(), sizeof(int32_t) * aux.size(), hipMemcpyHostToDevice));
+  }
+  if (!offs.empty()) FH_HIP(hipMemcpy(S.offs.p, offs.data(), sizeof(int32_t) * offs.size(), hipMemcpyHostToDevice));
+  FH_HIP(hipMemcpy(S.ecol.p, ecol.data(), sizeof(int32_t) * ecol.size(), hipMemcpyHostToDevice));
+  if (E) FH_HIP(hipMemcpy(S.map.p, map.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+  S.built = true;
+  S.filled_version = S.filled_version32 = -1;
+}
+
+}  // namespace femhip
