@@ -40,7 +40,10 @@
 namespace femhip {
 namespace {
 
-constexpr int kThreads = 256;
+#ifndef FEMHIP_CG_THREADS
+#define FEMHIP_CG_THREADS 512
+#endif
+constexpr int kThreads = FEMHIP_CG_THREADS;
 constexpr int kWaves = kThreads / 64;
 
 struct CgState {
@@ -49,7 +52,8 @@ struct CgState {
   double bnorm, r0norm, rnorm;
   double rtol, atol;
   double rmax, delta;          // reliable updates
-  int iter, done, converged, maxit, crit, ru, updates, pad;
+  double ru_rz, ru_rmax, ru_rnorm, ru_beta;
+  int iter, done, converged, maxit, crit, ru, updates, ru_conv, ru_done, pad;
 };
 
 struct CgResult {
@@ -61,6 +65,10 @@ struct SlicedDev {
   int32_t ns, n;
   const int64_t *base;
   const int32_t *meta, *aux, *offs, *ecol;
+  int32_t sym, kplus;          // symmetric-diagonal mode
+  int64_t stride;
+  const int32_t *soffs;
+  int32_t order;               // traversal order of wave_slices
 };
 
 __device__ __forceinline__ bool is_delta(int c) {
@@ -116,31 +124,120 @@ __device__ __forceinline__ int xcd_block() {
   return (g % 8 == 0) ? (b % 8) * (g / 8) + b / 8 : b;
 }
 
-// one row (= lane) of a 64-row slice
-template <typename T, typename A>
-__device__ __forceinline__ A slice_row(const SlicedDev &m, const T *__restrict__ v, const T *__restrict__ x, int s,
-                                       int lane) {
+// Slice range of the calling wavefront, wave-uniform (scalar registers), so
+// slice metadata and diagonal offsets are scalar loads.
+// Two traversal orders (both give each XCD one contiguous eighth of the
+// slices): order 0, each wave owns a contiguous run of slices; order 1, the
+// XCD's waves sweep its eighth together (wave-strided), so the processing front
+// is narrow and rows a few planes apart are touched close in time.
+struct WaveRange {
+  int s0, s1, step;
+};
+__device__ __forceinline__ WaveRange wave_slices(int ns, int order) {
+  const int w = (int)(threadIdx.x >> 6);
+  if (order == 1 && gridDim.x % 8 == 0) {
+    const int b = blockIdx.x, x = b % 8, per = gridDim.x / 8;
+    const int wx = __builtin_amdgcn_readfirstlane((b / 8) * kWaves + w);
+    const int c0 = (int)((int64_t)ns * x / 8), c1 = (int)((int64_t)ns * (x + 1) / 8);
+    return {c0 + wx, c1, per * kWaves};
+  }
+  const int wave = __builtin_amdgcn_readfirstlane(xcd_block() * kWaves + w);
+  const int nw = gridDim.x * kWaves;
+  return {(int)((int64_t)ns * wave / nw), (int)((int64_t)ns * (wave + 1) / nw), 1};
+}
+
+// Row `lane` of slice s (s wave-uniform): returns A x for NX right-hand vectors
+// x[0..NX) sharing one pass over the slice's values.  Diagonal slices use a
+// scalar base pointer per diagonal and a per-lane 32-bit byte offset, so a
+// gather is one load with no vector address arithmetic; slices whose columns
+// may leave [0, n) (flag bit 29) clamp the column.
+// Symmetric-diagonal rows: y_r = sum_k V[k][r] x[r+o_k] + sum_{k>0} V[k][r-o_k] x[r-o_k]
+// (o_0 = 0).  Indices are clamped into [0, n) and out-of-range terms zeroed;
+// padding values are 0.
+template <int NX, typename T, typename A, typename X>
+__device__ __forceinline__ void sym_rows(const SlicedDev &m, const T *__restrict__ v, const X *const (&x)[NX], int s,
+                                         int lane, A (&acc)[NX]) {
+  const int row = s * kSlice + lane;
+  const int hi = m.n - 1;
+  for (int j = 0; j < NX; ++j) acc[j] = 0;
+  const T *vr = v + row;
+  for (int k = 0; k < m.kplus; ++k) {   // diagonal and upper part
+    const int o = m.soffs[k];
+    const A a = (A)vr[k * m.stride];
+    const int c = min(row + o, hi);
+    for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][c];
+  }
+  for (int k = 1; k < m.kplus; ++k) {   // lower part from the stored transpose
+    const int o = m.soffs[k];
+    const int rr = row - o;
+    const int rc = min(max(rr, 0), hi);
+    A a = (A)v[k * m.stride + rc];
+    if (rr < 0) a = 0;
+    for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][rc];
+  }
+}
+
+template <int NX, typename T, typename A, typename X>
+__device__ __forceinline__ void slice_rows(const SlicedDev &m, const T *__restrict__ v, const X *const (&x)[NX],
+                                           int s, int lane, A (&acc)[NX]) {
+  if (m.sym) {
+    sym_rows<NX, T, A, X>(m, v, x, s, lane, acc);
+    return;
+  }
   const int64_t b = m.base[s];
   const int meta = m.meta[s];
-  const int K = meta & 0x3fffffff;
+  const int K = meta & 0x1fffffff;
   const int row = s * kSlice + lane;
-  A acc = 0;
+  for (int j = 0; j < NX; ++j) acc[j] = 0;
   const T *vv = v + b + lane;
   if (!(meta >> 30)) {
     const int32_t *off = m.offs + m.aux[s];
-    const int hi = m.n - 1;
+    if (!((meta >> 29) & 1)) {
+      const uint32_t rb = (uint32_t)row * (uint32_t)sizeof(X);
+      if (K == 27) {   // 27-point stencil slices: all loads of the slice in flight at once
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+          const A a = (A)vv[k * kSlice];
+          const int o = off[k];
+          for (int j = 0; j < NX; ++j) acc[j] += a * (A)*(const X *)((const char *)(x[j] + o) + rb);
+        }
+        return;
+      }
 #pragma unroll 9
-    for (int k = 0; k < K; ++k) {
-      int c = row + off[k];
-      c = min(max(c, 0), hi);
-      acc += (A)vv[k * kSlice] * (A)x[c];
+      for (int k = 0; k < K; ++k) {
+        const A a = (A)vv[k * kSlice];
+        const int o = off[k];
+        for (int j = 0; j < NX; ++j) {
+          const char *xk = (const char *)(x[j] + o);
+          acc[j] += a * (A)*(const X *)(xk + rb);
+        }
+      }
+    } else {
+      const int hi = m.n - 1;
+      for (int k = 0; k < K; ++k) {
+        const A a = (A)vv[k * kSlice];
+        const int c = min(max(row + off[k], 0), hi);
+        for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][c];
+      }
     }
   } else {
     const int32_t *ec = m.ecol + m.aux[s] + lane;
 #pragma unroll 4
-    for (int k = 0; k < K; ++k) acc += (A)vv[k * kSlice] * (A)x[ec[k * kSlice]];
+    for (int k = 0; k < K; ++k) {
+      const A a = (A)vv[k * kSlice];
+      const int c = ec[k * kSlice];
+      for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][c];
+    }
   }
-  return acc;
+}
+
+template <typename T, typename A>
+__device__ __forceinline__ A slice_row(const SlicedDev &m, const T *__restrict__ v, const T *__restrict__ x, int s,
+                                       int lane) {
+  const T *const xs[1] = {x};
+  A acc[1];
+  slice_rows<1, T, A, T>(m, v, xs, s, lane, acc);
+  return acc[0];
 }
 
 // ------------------------------------------------------------------ kernels ---
@@ -151,10 +248,10 @@ __global__ __launch_bounds__(kThreads) void k_init(SlicedDev m, const T *__restr
                                                    const T *__restrict__ x, int x_zero, T *__restrict__ r,
                                                    T *__restrict__ p, const T *__restrict__ invd, double *part) {
   const int lane = threadIdx.x & 63;
-  const int wave = xcd_block() * kWaves + (threadIdx.x >> 6), nw = gridDim.x * kWaves;
-  const int s0 = (int)((int64_t)m.ns * wave / nw), s1 = (int)((int64_t)m.ns * (wave + 1) / nw);
+  const WaveRange wr = wave_slices(m.ns, m.order);
+  const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
   double acc[4] = {0, 0, 0, 0};
-  for (int s = s0; s < s1; ++s) {
+  for (int s = s0; s < s1; s += sstep) {
     T ax = x_zero ? (T)0 : slice_row<T, T>(m, v, x, s, lane);
     const int row = s * kSlice + lane;
     if (row < m.n) {
@@ -234,10 +331,10 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SlicedDev m, const T *__restr
                                                    T *__restrict__ q, const CgState *S, int par, double *part) {
   if (S[par].done) return;
   const int lane = threadIdx.x & 63;
-  const int wave = xcd_block() * kWaves + (threadIdx.x >> 6), nw = gridDim.x * kWaves;
-  const int s0 = (int)((int64_t)m.ns * wave / nw), s1 = (int)((int64_t)m.ns * (wave + 1) / nw);
+  const WaveRange wr = wave_slices(m.ns, m.order);
+  const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
   A pq = 0, pp = 0;
-  for (int s = s0; s < s1; ++s) {
+  for (int s = s0; s < s1; s += sstep) {
     const A acc = slice_row<T, A>(m, v, p, s, lane);
     const int row = s * kSlice + lane;
     if (row < m.n) {
@@ -347,10 +444,10 @@ __global__ __launch_bounds__(kThreads) void k_ru_init(SlicedDev m, const double 
                                                       float *__restrict__ p32, float *__restrict__ d32,
                                                       const float *__restrict__ invd32, double *part) {
   const int lane = threadIdx.x & 63;
-  const int wave = xcd_block() * kWaves + (threadIdx.x >> 6), nw = gridDim.x * kWaves;
-  const int s0 = (int)((int64_t)m.ns * wave / nw), s1 = (int)((int64_t)m.ns * (wave + 1) / nw);
+  const WaveRange wr = wave_slices(m.ns, m.order);
+  const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
   double acc[4] = {0, 0, 0, 0};
-  for (int s = s0; s < s1; ++s) {
+  for (int s = s0; s < s1; s += sstep) {
     const double ax = x_zero ? 0.0 : slice_row<double, double>(m, v64, x, s, lane);
     const int row = s * kSlice + lane;
     if (row < m.n) {
@@ -398,12 +495,152 @@ __global__ __launch_bounds__(kThreads) void k_ru_init_fin(const double *part, in
   R->updates = 0;
 }
 
-// FP32 stopping decision: continue (p update) or request a reliable update
-__global__ __launch_bounds__(kThreads) void k_dir_ru(int32_t n, float *__restrict__ p, const float *__restrict__ r,
-                                                     const float *__restrict__ invd, CgState *S, int par,
-                                                     const double *part_u, int gu) {
+// One iteration slot of the FP32 path is either a CG iteration or, when the
+// previous slot requested it (state.ru), a reliable update; the same three
+// kernels serve both, so an update costs no extra launches.
+
+// slot 1: q = A32 p (partials <p,q>, ||p||^2), or r = b - A64 (x + d) in FP64
+// rounded into r32 (partials ||r64||^2, <r32, M r32>, ||r32||^2)
+__global__ __launch_bounds__(kThreads) void k_ru_spmv(SlicedDev m, const float *__restrict__ v32,
+                                                      const double *__restrict__ v64, const float *__restrict__ p,
+                                                      float *__restrict__ q, const double *__restrict__ b,
+                                                      const double *__restrict__ x, const float *__restrict__ d,
+                                                      float *__restrict__ r32, const float *__restrict__ invd,
+                                                      const CgState *S, int par, double *part) {
   const CgState &cur = S[par];
   if (cur.done) return;
+  const int lane = threadIdx.x & 63;
+  const WaveRange wr = wave_slices(m.ns, m.order);
+  const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
+  double acc[3] = {0, 0, 0};
+  if (!cur.ru) {
+    float pq = 0.f, pp = 0.f;
+    for (int s = s0; s < s1; s += sstep) {
+      const float a = slice_row<float, float>(m, v32, p, s, lane);
+      const int row = s * kSlice + lane;
+      if (row < m.n) {
+        q[row] = a;
+        const float pi = p[row];
+        pq += pi * a;
+        pp += pi * pi;
+      }
+    }
+    acc[0] = pq;
+    acc[1] = pp;
+  } else {
+    for (int s = s0; s < s1; s += sstep) {
+      const double ax = slice_row<double, double>(m, v64, x, s, lane);
+      const float *const ds[1] = {d};
+      double ad[1];
+      slice_rows<1, double, double, float>(m, v64, ds, s, lane, ad);
+      const int row = s * kSlice + lane;
+      if (row < m.n) {
+        const double ri = b[row] - ax - ad[0];
+        const float rf = (float)ri;
+        r32[row] = rf;
+        const float zf = invd ? invd[row] * rf : rf;
+        acc[0] += ri * ri;
+        acc[1] += (double)(rf * zf);
+        acc[2] += (double)(rf * rf);
+      }
+    }
+  }
+  write_partials<3>(acc, part);
+}
+
+// slot 2: alpha ; d += alpha p ; r -= alpha q (partials <r,z>, ||r||^2), or
+// the update's second half: x += d ; d = 0 ; FP64 stopping test ; p = z + beta p
+__global__ __launch_bounds__(kThreads) void k_ru_update(int32_t n, double *__restrict__ x, float *__restrict__ d,
+                                                        float *__restrict__ r, float *__restrict__ p,
+                                                        const float *__restrict__ q, const float *__restrict__ invd,
+                                                        CgState *S, int par, const double *part_s, int gs,
+                                                        double *part) {
+  // Block 0 writes only fields the other blocks do not read (alpha, pap, pp,
+  // ru_*); they read done, ru, rz, iter, maxit and rhs_test.
+  const CgState &cur = S[par];
+  if (cur.done) return;
+  double red[3];
+  reduce_partials<3>(part_s, gs, red);
+  if (cur.ru) {
+    const double rnorm = sqrt(red[0]);
+    const int conv = rnorm <= cur.rhs_test;
+    const int done = conv || cur.iter >= cur.maxit;
+    double beta = 1.0 / cur.rz;
+    beta = beta * red[1];
+    const float bt = (float)beta;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+      x[i] = x[i] + (double)d[i];
+      d[i] = 0.0f;
+      if (!done) {
+        const float zi = invd ? invd[i] * r[i] : r[i];
+        const float t = bt * p[i];
+        p[i] = t + zi;
+      }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      CgState &w = S[par];
+      w.ru_rz = red[1];
+      w.ru_rmax = sqrt(red[2]);
+      w.ru_rnorm = rnorm;
+      w.ru_beta = beta;
+      w.ru_conv = conv;
+      w.ru_done = done;
+    }
+    return;
+  }
+  const double alpha = (red[0] != 0.0) ? cur.rz / red[0] : 0.0;   // cg.f90:195-199
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S[par].alpha = alpha;
+    S[par].pap = red[0];
+    S[par].pp = red[1];
+  }
+  const float a = (float)alpha;
+  float rz = 0.f, rr = 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    d[i] = d[i] + a * p[i];
+    const float ri = r[i] + (-a) * q[i];
+    r[i] = ri;
+    const float zi = invd ? invd[i] * ri : ri;
+    rz += ri * zi;
+    rr += ri * ri;
+  }
+  double out[2] = {(double)rz, (double)rr};
+  write_partials<2>(out, part);
+}
+
+// slot 3: request a reliable update or take the CG direction step; after an
+// update, publish its results to the next parity slot
+__global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restrict__ p, const float *__restrict__ r,
+                                                     const float *__restrict__ invd, CgState *S, int par,
+                                                     const double *part_u, int gu, CgResult *R) {
+  const CgState &cur = S[par];
+  if (cur.done) {   // keep `done` sticky in both parity slots
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[par ^ 1].done = 1;
+    return;
+  }
+  if (cur.ru) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      CgState nx = cur;
+      nx.ru = 0;
+      nx.rz = cur.ru_rz;
+      nx.rnrm_m = sqrt(cur.ru_rz);
+      nx.rmax = cur.ru_rmax;
+      nx.beta = cur.ru_beta;
+      nx.est = cur.ru_rnorm;
+      nx.rnorm = cur.ru_rnorm;
+      nx.converged = cur.ru_conv;
+      nx.done = cur.ru_done;
+      nx.updates = cur.updates + 1;
+      S[par ^ 1] = nx;
+      R->iter = cur.iter;
+      R->est = cur.ru_rnorm;
+      R->rnorm = cur.ru_rnorm;
+      R->converged = cur.ru_conv;
+      R->done = cur.ru_done;
+      R->updates = nx.updates;
+    }
+    return;
+  }
   double red[2];
   reduce_partials<2>(part_u, gu, red);
   const int it = cur.iter + 1;
@@ -430,100 +667,6 @@ __global__ __launch_bounds__(kThreads) void k_dir_ru(int32_t n, float *__restric
     const float zi = invd ? invd[i] * r[i] : r[i];
     const float t = bt * p[i];
     p[i] = t + zi;
-  }
-}
-
-// reliable update, part 1: r64 = b - A64 (x + d) in FP64 ; r32 = r64 ; partials ||r64||^2, <r32, M r32>
-__global__ __launch_bounds__(kThreads) void k_ru_res(SlicedDev m, const double *__restrict__ v64,
-                                                     const double *__restrict__ b, const double *__restrict__ x,
-                                                     const float *__restrict__ d32, double *__restrict__ r64,
-                                                     float *__restrict__ r32, const float *__restrict__ invd32,
-                                                     const CgState *S, int par, double *part) {
-  const CgState &nx = S[par ^ 1];
-  if (S[par].done || !nx.ru) return;
-  const int lane = threadIdx.x & 63;
-  const int wave = xcd_block() * kWaves + (threadIdx.x >> 6), nw = gridDim.x * kWaves;
-  const int s0 = (int)((int64_t)m.ns * wave / nw), s1 = (int)((int64_t)m.ns * (wave + 1) / nw);
-  double acc[3] = {0, 0, 0};
-  for (int s = s0; s < s1; ++s) {
-    // A (x + d) as two products over the same slice
-    const double ax = slice_row<double, double>(m, v64, x, s, lane);
-    const int64_t bb = m.base[s];
-    const int meta = m.meta[s];
-    const int K = meta & 0x3fffffff;
-    const int row = s * kSlice + lane;
-    double ad = 0.0;
-    const double *vv = v64 + bb + lane;
-    if (!(meta >> 30)) {
-      const int32_t *off = m.offs + m.aux[s];
-      for (int k = 0; k < K; ++k) {
-        int c = min(max(row + off[k], 0), m.n - 1);
-        ad += vv[k * kSlice] * (double)d32[c];
-      }
-    } else {
-      const int32_t *ec = m.ecol + m.aux[s] + lane;
-      for (int k = 0; k < K; ++k) ad += vv[k * kSlice] * (double)d32[ec[k * kSlice]];
-    }
-    if (row < m.n) {
-      const double ri = b[row] - ax - ad;
-      r64[row] = ri;
-      const float rf = (float)ri;
-      r32[row] = rf;
-      const float zf = invd32 ? invd32[row] * rf : rf;
-      acc[0] += ri * ri;
-      acc[1] += (double)(rf * zf);
-      acc[2] += (double)(rf * rf);
-    }
-  }
-  write_partials<3>(acc, part);
-}
-
-// reliable update, part 2: x += d ; d = 0 ; FP64 stopping test ; beta ; p = z + beta p
-__global__ __launch_bounds__(kThreads) void k_ru_fin(int32_t n, double *__restrict__ x, float *__restrict__ d32,
-                                                     float *__restrict__ p, const float *__restrict__ r,
-                                                     const float *__restrict__ invd, CgState *S, int par,
-                                                     const double *part_r, int gr, CgResult *R) {
-  // Fields of S[par^1] this kernel writes (rz, beta, rmax, est, converged,
-  // updates, done) are never read by its other blocks, which only read ru,
-  // iter, maxit and rhs_test there.
-  const CgState &nx = S[par ^ 1];
-  if (S[par].done) {   // keep `done` sticky in both parity slots
-    if (blockIdx.x == 0 && threadIdx.x == 0) S[par ^ 1].done = 1;
-    return;
-  }
-  if (!nx.ru) return;
-  double red[3];
-  reduce_partials<3>(part_r, gr, red);
-  const double rnorm = sqrt(red[0]);
-  const int conv = rnorm <= nx.rhs_test;
-  const bool done = conv || nx.iter >= nx.maxit;
-  double beta = 1.0 / S[par].rz;
-  beta = beta * red[1];
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    x[i] = x[i] + (double)d32[i];
-    d32[i] = 0.0f;
-    if (!done) {
-      const float zi = invd ? invd[i] * r[i] : r[i];
-      const float t = (float)beta * p[i];
-      p[i] = t + zi;
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    CgState &w = S[par ^ 1];
-    w.rz = red[1];
-    w.beta = beta;
-    w.rmax = sqrt(red[2]);
-    w.est = rnorm;
-    w.converged = conv;
-    const int upd = w.updates + 1;
-    w.updates = upd;
-    if (done) w.done = 1;
-    R->iter = w.iter;
-    R->est = rnorm;
-    R->rnorm = rnorm;
-    R->converged = conv;
-    R->done = done;
-    R->updates = upd;
   }
 }
 
@@ -579,10 +722,10 @@ __global__ __launch_bounds__(kThreads) void k_true_res(SlicedDev m, const double
                                                        const double *__restrict__ b, const double *__restrict__ x,
                                                        double *part) {
   const int lane = threadIdx.x & 63;
-  const int wave = xcd_block() * kWaves + (threadIdx.x >> 6), nw = gridDim.x * kWaves;
-  const int s0 = (int)((int64_t)m.ns * wave / nw), s1 = (int)((int64_t)m.ns * (wave + 1) / nw);
+  const WaveRange wr = wave_slices(m.ns, m.order);
+  const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
   double acc[1] = {0};
-  for (int s = s0; s < s1; ++s) {
+  for (int s = s0; s < s1; s += sstep) {
     const double ax = slice_row<double, double>(m, v64, x, s, lane);
     const int row = s * kSlice + lane;
     if (row < m.n) {
@@ -633,6 +776,12 @@ SlicedDev sliced_dev(femhip_operator &op) {
   m.aux = op.sl.aux.p;
   m.offs = op.sl.offs.p;
   m.ecol = op.sl.ecol.p;
+  m.sym = op.sl.sym ? 1 : 0;
+  m.kplus = op.sl.kplus;
+  m.stride = op.sl.stride;
+  m.soffs = op.sl.soffs.p;
+  static const char *ord = getenv("FEMHIP_SLICE_ORDER");
+  m.order = ord ? atoi(ord) : 1;
   return m;
 }
 
@@ -650,12 +799,16 @@ void prepare(femhip_solver &s) {
     s.n = n;
     const int ns = op.sl.ns;
     // about 4 slices per wavefront for the matrix kernels, multiple of 8 blocks
-    s.g_mat = std::max(8, std::min(2048, (ns + 4 * kWaves - 1) / (4 * kWaves)));
+    // one resident wave set: about 32 waves per CU, grids a multiple of the 8 XCDs
+    const char *bpc_env = getenv("FEMHIP_CG_BPC");
+    const int bpc = bpc_env ? std::max(1, atoi(bpc_env)) : std::max(1, 2048 / kThreads);
+    const int cap = std::max(8, bpc * std::max(1, s.ctx->num_cus));
+    s.g_mat = std::max(8, std::min(cap, (ns + 2 * kWaves - 1) / (2 * kWaves)));
     s.g_mat = (s.g_mat + 7) / 8 * 8;
-    s.g_vec = std::max(8, std::min(1024, (n + 4 * kThreads - 1) / (4 * kThreads)));
+    s.g_vec = std::max(8, std::min(cap, (n + 2 * kThreads - 1) / (2 * kThreads)));
     s.x.alloc(n); s.b.alloc(n); s.r.alloc(n); s.p.alloc(n); s.q.alloc(n);
     s.part_i.alloc((size_t)4 * s.g_mat);
-    s.part_s.alloc((size_t)2 * s.g_mat);
+    s.part_s.alloc((size_t)3 * s.g_mat);
     s.part_u.alloc((size_t)2 * s.g_vec);
     s.part_r.alloc((size_t)3 * s.g_mat);
     s.hist.alloc((size_t)std::max(1, s.prm.max_num_iterations));
@@ -712,15 +865,11 @@ void enqueue_iterations(femhip_solver &s, int chunk) {
                          s.hist.p, s.res.p);
     } else {
       const float *invd = jac ? s.invd32.p : nullptr;
-      hipLaunchKernelGGL((k_spmv<float, float>), GM, B, 0, st, m, op.sl.v32.p, s.p32.p, s.q32.p, s.st.p, par,
-                         s.part_s.p);
-      hipLaunchKernelGGL((k_update<float, float>), GV, B, 0, st, n, s.d32.p, s.r32.p, s.p32.p, s.q32.p, invd, s.st.p,
-                         par, s.part_s.p, s.g_mat, s.part_u.p);
-      hipLaunchKernelGGL(k_dir_ru, GV, B, 0, st, n, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec);
-      hipLaunchKernelGGL(k_ru_res, GM, B, 0, st, m, op.sl.v64.p, s.b.p, s.x.p, s.d32.p, s.r.p, s.r32.p, invd, s.st.p,
-                         par, s.part_r.p);
-      hipLaunchKernelGGL(k_ru_fin, GV, B, 0, st, n, s.x.p, s.d32.p, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_r.p,
-                         s.g_mat, s.res.p);
+      hipLaunchKernelGGL(k_ru_spmv, GM, B, 0, st, m, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.d32.p,
+                         s.r32.p, invd, s.st.p, par, s.part_s.p);
+      hipLaunchKernelGGL(k_ru_update, GV, B, 0, st, n, s.x.p, s.d32.p, s.r32.p, s.p32.p, s.q32.p, invd, s.st.p, par,
+                         s.part_s.p, s.g_mat, s.part_u.p);
+      hipLaunchKernelGGL(k_ru_dir, GV, B, 0, st, n, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.res.p);
     }
   }
 }
@@ -943,8 +1092,9 @@ int femhip_solver_profile_spmv(femhip_solver_t *s, int32_t reps, double *ms_per_
     const double vb = mx ? 4.0 : 8.0;
     const int64_t ell = (int64_t)(op.sl.ecol.n > 1 ? op.sl.ecol.n : 0);
     if (bytes)
-      *bytes = (double)op.sl.entries * vb + 16.0 * op.sl.ns + 4.0 * (double)op.sl.offs.n + 4.0 * (double)ell +
-               2.0 * vb * op.n;
+      *bytes = op.sl.sym ? (double)op.sl.entries * vb + 2.0 * vb * op.n
+                         : (double)op.sl.entries * vb + 16.0 * op.sl.ns + 4.0 * (double)op.sl.offs.n +
+                               4.0 * (double)ell + 2.0 * vb * op.n;
     return FEMHIP_OK;
   } catch (const Error &e) {
     set_error(e.msg);

@@ -126,10 +126,22 @@ namespace femhip {
 // with K = longest row and explicit column indices.  Values are laid out
 // [slice][k][lane]; padding entries hold 0 and a valid column.
 constexpr int kSlice = 64;
+//
+// Symmetric-diagonal mode (sym = true): when every row's column offsets come
+// from one global set D = -D (structured Q1 numbering), only the diagonals with
+// offset >= 0 are kept, diagonal-major with row stride `stride`; the entry
+// (r, r - o) of a lower diagonal is read as the stored (r - o, r), a shifted
+// but still contiguous load.  The matrix stream is then about half of the full
+// pattern's, as with FEMPAR's symmetric storage.
 struct Sliced {
   int32_t ns = 0;                 // number of slices
   int64_t entries = 0;            // padded entries
   int64_t dia_slices = 0;
+  bool sym = false;               // symmetric-diagonal mode
+  int32_t kplus = 0;              // stored diagonals (offsets >= 0, first is 0)
+  int64_t stride = 0;             // row stride of a stored diagonal (n rounded to 64)
+  std::vector<int32_t> h_soffs;
+  DevBuf<int32_t> soffs;          // kplus offsets, ascending
   DevBuf<int64_t> base;           // ns+1 entry offsets
   DevBuf<int32_t> meta;           // ns: K | (is_ell << 30)
   DevBuf<int32_t> aux;            // ns: offset into offs (diagonal) or ecol (ELL)

@@ -2,14 +2,111 @@
 // the CSR pattern (pattern only; values are gathered on the device through
 // `map` after every assembly).
 #include <algorithm>
+#include <cstdlib>
 #include <thread>
 
 #include "femhip_internal.h"
 
 namespace femhip {
 
+namespace {
+
+template <typename F>
+void parallel_rows(int64_t n, F &&f) {
+  const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nth; ++t)
+    th.emplace_back([&, t] { f(t, n * t / nth, n * (t + 1) / nth); });
+  for (auto &x : th) x.join();
+}
+
+bool row_has(const std::vector<int32_t> &rp, const std::vector<int32_t> &cl, int32_t r, int32_t c) {
+  return std::binary_search(cl.begin() + rp[(size_t)r], cl.begin() + rp[(size_t)r + 1], c);
+}
+
+// Symmetric-diagonal layout if the pattern is structurally symmetric and all
+// rows draw their offsets from one small symmetric set.
+bool try_build_symmetric_dia(femhip_operator &op) {
+  Sliced &S = op.sl;
+  const int32_t n = op.n;
+  const std::vector<int32_t> &rp = op.h_rowptr;
+  const std::vector<int32_t> &cl = op.h_col;
+  if (n == 0) return false;
+  const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  std::vector<std::vector<int32_t>> part(nth);
+  std::vector<char> bad(nth, 0);
+  parallel_rows(n, [&](unsigned t, int64_t lo, int64_t hi) {
+    std::vector<int32_t> &u = part[t];
+    for (int64_t r = lo; r < hi && !bad[t]; ++r) {
+      for (int32_t k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) {
+        const int32_t o = cl[(size_t)k] - (int32_t)r;
+        if (std::find(u.begin(), u.end(), o) == u.end()) {
+          u.push_back(o);
+          if (u.size() > 64) { bad[t] = 1; break; }
+        }
+        if (!row_has(rp, cl, cl[(size_t)k], (int32_t)r)) { bad[t] = 1; break; }
+      }
+    }
+  });
+  std::vector<int32_t> D;
+  for (unsigned t = 0; t < nth; ++t) {
+    if (bad[t]) return false;
+    D.insert(D.end(), part[t].begin(), part[t].end());
+  }
+  std::sort(D.begin(), D.end());
+  D.erase(std::unique(D.begin(), D.end()), D.end());
+  if (D.size() > 64) return false;
+  for (int32_t o : D)
+    if (!std::binary_search(D.begin(), D.end(), -o)) return false;
+  std::vector<int32_t> up;
+  for (int32_t o : D)
+    if (o >= 0) up.push_back(o);
+  if (up.empty() || up[0] != 0) return false;
+  const int64_t stride = ((int64_t)n + kSlice - 1) / kSlice * kSlice;
+  const int64_t E = (int64_t)up.size() * stride;
+  const int64_t stored = ((int64_t)cl.size() + n) / 2;
+  if (E > stored * 3 / 2 + 64 || E >= (int64_t)INT32_MAX) return false;
+  std::vector<int32_t> map((size_t)E, -1);
+  parallel_rows(n, [&](unsigned, int64_t lo, int64_t hi) {
+    for (int64_t r = lo; r < hi; ++r)
+      for (size_t k = 0; k < up.size(); ++k) {
+        const int64_t c = r + up[k];
+        if (c >= n) break;
+        auto b = cl.begin() + rp[(size_t)r], e = cl.begin() + rp[(size_t)r + 1];
+        auto it = std::lower_bound(b, e, (int32_t)c);
+        if (it != e && *it == c) map[(size_t)(k * stride + r)] = (int32_t)(it - cl.begin());
+      }
+  });
+  S.sym = true;
+  S.kplus = (int32_t)up.size();
+  S.stride = stride;
+  S.h_soffs = up;
+  S.ns = (int32_t)(stride / kSlice);
+  S.entries = E;
+  S.dia_slices = S.ns;
+  S.soffs.alloc(up.size());
+  FH_HIP(hipMemcpy(S.soffs.p, up.data(), sizeof(int32_t) * up.size(), hipMemcpyHostToDevice));
+  S.map.alloc((size_t)E);
+  FH_HIP(hipMemcpy(S.map.p, map.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+  // unused sliced arrays stay valid 1-element buffers
+  S.base.alloc(1);
+  S.meta.alloc(1);
+  S.aux.alloc(1);
+  S.offs.alloc(1);
+  S.ecol.alloc(1);
+  S.built = true;
+  S.filled_version = S.filled_version32 = -1;
+  return true;
+}
+
+}  // namespace
+
 void build_sliced(femhip_operator &op) {
   Sliced &S = op.sl;
+  static const char *sym_env = getenv("FEMHIP_SYMDIA");
+  const bool want_sym = sym_env ? atoi(sym_env) != 0 : false;
+  if (want_sym && try_build_symmetric_dia(op)) return;
+  S.sym = false;
   const int32_t n = op.n;
   const int32_t ns = (n + kSlice - 1) / kSlice;
   const std::vector<int32_t> &rp = op.h_rowptr;
@@ -49,7 +146,14 @@ void build_sliced(femhip_operator &op) {
   int64_t ell_entries = 0, dia = 0;
   for (int32_t s = 0; s < ns; ++s) {
     base[(size_t)s + 1] = base[(size_t)s] + (int64_t)K[(size_t)s] * kSlice;
+    FH_CHECK(K[(size_t)s] < (1 << 29), FEMHIP_ERR_INVALID, "row too long for the sliced format");
     meta[(size_t)s] = K[(size_t)s] | (is_ell[(size_t)s] << 30);
+    if (!is_ell[(size_t)s] && !offsets[(size_t)s].empty()) {
+      // bit 29: some lane's column row + offset leaves [0, n) (padding near the ends)
+      const int64_t lo = (int64_t)s * kSlice + offsets[(size_t)s].front();
+      const int64_t hi = (int64_t)s * kSlice + kSlice - 1 + offsets[(size_t)s].back();
+      if (lo < 0 || hi >= n) meta[(size_t)s] |= 1 << 29;
+    }
     if (is_ell[(size_t)s]) {
       aux[(size_t)s] = (int32_t)ell_entries;
       ell_entries += (int64_t)K[(size_t)s] * kSlice;
