@@ -89,6 +89,10 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only launch the FP64 and FP32 SpMV kernels 50 times each (for rocprofv3 --pmc runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device-to-device 5 times (counter calibration for --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -96,9 +100,28 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(256 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 8, "copies": 5}))
+        return
+
     import fempar_amd as F
     env = F.environment_t(0)
     n = args.n
+    if args.spmv_only:
+        tria, fes, di, op = make_problem(F, env, n)
+        op.compute()
+        res = {}
+        for prec in ("FP64", "FP32_REFINED"):
+            ms, by = make_solver(F, env, op, prec, args.rtol).profile_spmv(50)
+            res[prec] = {"kernel_ms": ms, "algorithmic_bytes": by, "GBs": by / ms / 1e6}
+        print(json.dumps(res))
+        return
     t0 = time.perf_counter()
     tria, fes, di, op = make_problem(F, env, n)
     t_setup = time.perf_counter() - t0
@@ -113,6 +136,10 @@ def main():
     # roofline of the dominant kernel (CG SpMV)
     spmv_ms, spmv_bytes = s64.profile_spmv(50)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"spmv_traffic_poisson3d_q1_{n}.json")
+    if os.path.exists(tpath):   # HBM bytes per launch from the committed rocprofv3 --pmc runs
+        traffic = json.load(open(tpath)).get("FP64", {}).get("hbm_bytes_per_launch")
 
     mixed = None
     if not args.no_mixed:
@@ -150,7 +177,8 @@ def main():
         "cg_iterations": r64["num_iterations"], "true_rel_residual": r64["true_residual"] / r64["rhs_norm"],
         "setup_s": t_setup,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_cg_spmv (FP64 CSR)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_spmv<double> (sliced diagonal/ELL, FP64)",
                      "kernel_ms": spmv_ms, "algorithmic_bytes": spmv_bytes},
         "mixed": mixed,
         "cpu_baseline": cpu,

@@ -62,5 +62,25 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_EXE = os.path.join(HERE, "test_poisson_hip")
+
+
+def build_fortran() -> str:
+    """Compile the ISO_C_BINDING module and the test_poisson driver with AMD flang."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    mod = os.path.join(OBJ, "fortran")
+    os.makedirs(mod, exist_ok=True)
+    obj = os.path.join(mod, "femhip_names.o")
+    for cmd in ([fc, "-O2", "-c", os.path.join(FORTRAN_DIR, "femhip_names.f90"), "-J", mod, "-o", obj],
+                [fc, "-O2", os.path.join(FORTRAN_DIR, "test_poisson_hip.f90"), obj, "-J", mod, "-L" + HERE,
+                 "-lfemhip", "-Wl,-rpath,$ORIGIN", "-o", FORTRAN_EXE]):
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError("Fortran build failed: " + " ".join(cmd) + "\n" + p.stdout + p.stderr)
+    return FORTRAN_EXE
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran())
