@@ -80,6 +80,41 @@ def cpu_baseline(n, iters_full, op_matrix=None, rhs=None):
                       f"on the full matrix (x{iters_full} iterations); t_asm={t_asm:.2f}s t_iter={t_it*1e3:.1f}ms"}
 
 
+def batched_problems(count=4096, seed=4096):
+    """Batched workload: 70 % 2-D Q1 n x n with n log-uniform in [16,256], 30 % 3-D
+    Q1 n^3 with n log-uniform in [8,48]; constant k uniform in [0.1,10]."""
+    rng = np.random.default_rng(seed)
+    is2d = rng.random(count) < 0.7
+    n2 = np.rint(np.exp(rng.uniform(np.log(16), np.log(256), count))).astype(int)
+    n3 = np.rint(np.exp(rng.uniform(np.log(8), np.log(48), count))).astype(int)
+    k = rng.uniform(0.1, 10.0, count)
+    return [(2, int(n2[i]), float(k[i]), 1.0) if is2d[i] else (3, int(n3[i]), float(k[i]), 1.0)
+            for i in range(count)]
+
+
+def run_batched(F, env, steps, rtol=1e-8):
+    problems = batched_problems()
+    b = F.batch_t(env, problems, rtol=rtol)
+    t_asm = b.assemble()
+    threads = b.autotune()
+    b.solve()   # warm-up (graph instantiation)
+    runs = [b.solve() for _ in range(max(1, steps))]
+    ms = np.array([r["ms"] for r in runs])
+    last = runs[-1]
+    lat = last["latency_us"] / 1e3
+    dofs = int(np.sum(last["num_dofs"]))
+    rel = last["true_residual"] / np.maximum(last["rhs_norm"], 1e-300)
+    out = {"problems": len(problems), "total_free_dofs": dofs, "assemble_ms": t_asm,
+           "solve_ms": float(ms.mean()), "solves_per_s": len(problems) / (ms.mean() / 1e3),
+           "dofs_per_s": dofs / ((t_asm + ms.mean()) / 1e3),
+           "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
+           "latency_ms_max": float(lat.max()), "converged": int(np.sum(last["did_converge"])),
+           "max_true_rel_residual": float(rel.max()), "threads_per_block": threads,
+           "iterations_total": int(np.sum(last["num_iterations"]))}
+    b.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +124,7 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only launch the FP64 and FP32 SpMV kernels 50 times each (for rocprofv3 --pmc runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -160,6 +196,8 @@ def main():
             "spmv_fp32_GBs": sp32_bytes / (sp32_ms * 1e-3) / 1e9,
         }
 
+    batched = None if args.no_batched else run_batched(F, env, args.steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         A = op.get_matrix()
@@ -181,6 +219,7 @@ def main():
                      "kernel": "k_spmv<double> (sliced diagonal/ELL, FP64)",
                      "kernel_ms": spmv_ms, "algorithmic_bytes": spmv_bytes},
         "mixed": mixed,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -37,7 +37,19 @@ EXPORTS = [
     "femhip_operator_apply", "femhip_operator_destroy",
     "femhip_solver_create", "femhip_solver_solve", "femhip_solver_get_solution", "femhip_solver_destroy",
     "femhip_solver_profile_spmv",
+    "femhip_batch_create", "femhip_batch_assemble", "femhip_batch_solve", "femhip_batch_get_solution",
+    "femhip_batch_get_problem_matrix", "femhip_batch_autotune", "femhip_batch_destroy",
 ]
+
+
+class BatchProblem(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("n", C.c_int32), ("k", C.c_double), ("f0", C.c_double)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("num_iterations", C.c_int32), ("updates", C.c_int32), ("did_converge", C.c_int32),
+                ("num_dofs", C.c_int32), ("true_residual", C.c_double), ("rhs_norm", C.c_double),
+                ("latency_us", C.c_double)]
 
 
 class FemhipError(RuntimeError):
@@ -98,6 +110,13 @@ def load():
         "femhip_solver_solve": [P, P, P, C.POINTER(CgResult)], "femhip_solver_get_solution": [P, P],
         "femhip_solver_destroy": [P],
         "femhip_solver_profile_spmv": [P, I32, C.POINTER(D), C.POINTER(D)],
+        "femhip_batch_create": [P, I32, C.POINTER(BatchProblem), C.POINTER(CgParams), pP],
+        "femhip_batch_assemble": [P, C.POINTER(D)],
+        "femhip_batch_solve": [P, C.POINTER(BatchResult), C.POINTER(D)],
+        "femhip_batch_get_solution": [P, I32, P],
+        "femhip_batch_get_problem_matrix": [P, I32, P, P, P, P, C.POINTER(I64)],
+        "femhip_batch_autotune": [P, C.POINTER(I32)],
+        "femhip_batch_destroy": [P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -394,3 +394,72 @@ class iterative_linear_solver_t:
             self._destroy()
         except Exception:
             pass
+
+
+class batch_t:
+    """Many independent Poisson problems (Q1, unit square/cube, constant k,
+    f = f0 k d pi^2 prod sin(pi x), homogeneous Dirichlet) assembled in one launch
+    and solved in one persistent FP32-refined launch (femhip_batch_*)."""
+
+    def __init__(self, environment: environment_t, problems, rtol=1e-8, max_num_iterations=20000, delta=0.0):
+        self._lib = N.load()
+        self.environment = environment
+        arr = (N.BatchProblem * len(problems))()
+        for i, (dim, n, k, f0) in enumerate(problems):
+            arr[i].dim, arr[i].n, arr[i].k, arr[i].f0 = int(dim), int(n), float(k), float(f0)
+        self.problems = list(problems)
+        p = N.CgParams()
+        p.rtol, p.atol, p.max_num_iterations = rtol, 0.0, max_num_iterations
+        p.stopping_criterion = N.STOPPING_CRITERIA.index("RES_RES")
+        p.preconditioner, p.precision, p.inner_rtol = N.PREC_JACOBI, N.PRECISION_FP32_REFINED, delta
+        h = C.c_void_p()
+        N.check(self._lib.femhip_batch_create(environment.h, len(problems), arr, C.byref(p), C.byref(h)))
+        self.h = h
+
+    def assemble(self):
+        ms = C.c_double()
+        N.check(self._lib.femhip_batch_assemble(self.h, C.byref(ms)))
+        return ms.value
+
+    def autotune(self):
+        t = C.c_int32()
+        N.check(self._lib.femhip_batch_autotune(self.h, C.byref(t)))
+        return t.value
+
+    def solve(self):
+        res = (N.BatchResult * len(self.problems))()
+        ms = C.c_double()
+        N.check(self._lib.femhip_batch_solve(self.h, res, C.byref(ms)))
+        out = {f: np.array([getattr(r, f) for r in res]) for f, _ in N.BatchResult._fields_}
+        out["ms"] = ms.value
+        return out
+
+    def get_solution(self, i):
+        dim, n = self.problems[i][0], self.problems[i][1]
+        x = np.empty((n - 1) ** dim)
+        N.check(self._lib.femhip_batch_get_solution(self.h, i, N.ptr(x)))
+        return x
+
+    def get_problem_matrix(self, i):
+        dim, n = self.problems[i][0], self.problems[i][1]
+        rows = (n - 1) ** dim
+        cap = rows * (9 if dim == 2 else 27)
+        irp = np.empty(rows + 1, dtype=np.int32)
+        ja = np.empty(cap, dtype=np.int32)
+        val = np.empty(cap)
+        rhs = np.empty(rows)
+        nnz = C.c_int64()
+        N.check(self._lib.femhip_batch_get_problem_matrix(self.h, i, N.ptr(irp), N.ptr(ja), N.ptr(val), N.ptr(rhs),
+                                                          C.byref(nnz)))
+        return irp, ja[: nnz.value], val[: nnz.value], rhs
+
+    def free(self):
+        if getattr(self, "h", None):
+            self._lib.femhip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

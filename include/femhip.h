@@ -205,6 +205,54 @@ int femhip_solver_profile_spmv(femhip_solver_t *solver, int32_t reps, double *ms
 int femhip_solver_get_solution(const femhip_solver_t *solver, double *x);
 int femhip_solver_destroy(femhip_solver_t *solver);
 
+/* ---- batched form: many independent small problems in one persistent launch ----
+ * Each problem is FEMPAR's test_poisson setting on its own mesh: Q1 on the unit
+ * square/cube with n cells per direction, strong Dirichlet (g = 0) on the whole
+ * boundary, constant coefficient k, source f = f0 k d pi^2 prod_i sin(pi x_i)
+ * (so u = prod_i sin(pi x_i)).  The integration plugin, numbering, assembly and
+ * CG semantics are those of the single-problem path (cited above); solves use
+ * the FP32-refined CG with reliable updates, RES_RES criterion. */
+typedef struct {
+  int32_t dim;              /* 2 or 3 */
+  int32_t n;                /* cells per direction (>= 2) */
+  double k;                 /* constant coefficient */
+  double f0;                /* source scale */
+} femhip_batch_problem_t;
+
+typedef struct {
+  int32_t num_iterations;   /* FP32 CG iterations */
+  int32_t updates;          /* reliable updates (FP64 residual replacements) */
+  int32_t did_converge;
+  int32_t num_dofs;         /* free DOFs of the problem */
+  double true_residual;     /* ||b - A x||_2 in FP64 at exit */
+  double rhs_norm;          /* ||b||_2 */
+  double latency_us;        /* completion time after the solve launch started (device clock) */
+} femhip_batch_result_t;
+
+typedef struct femhip_batch femhip_batch_t;
+
+/* Sizes the device pools for `count` problems and builds the work order
+ * (largest first). params: rtol, max_num_iterations, inner_rtol = reliable-update
+ * factor (0: 0.1); precision must be FEMHIP_PRECISION_FP32_REFINED. */
+int femhip_batch_create(femhip_ctx_t *ctx, int32_t count, const femhip_batch_problem_t *problems,
+                        const femhip_cg_params_t *params, femhip_batch_t **batch);
+/* Integrate + assemble every problem (one launch; FP64 and FP32 values, RHS, Jacobi). */
+int femhip_batch_assemble(femhip_batch_t *batch, double *ms);
+/* Solve every problem from x0 = 0 in one persistent, hipGraph-captured launch;
+ * results (count entries, may be NULL) and elapsed device ms. */
+int femhip_batch_solve(femhip_batch_t *batch, femhip_batch_result_t *results, double *ms);
+/* Solution of problem i (num_dofs values, lexicographic free-node order = the
+ * first-touch numbering of this mesh) and its assembled matrix as 1-based CSR
+ * (full storage) with the RHS, for checking. */
+int femhip_batch_get_solution(const femhip_batch_t *batch, int32_t i, double *x);
+int femhip_batch_get_problem_matrix(const femhip_batch_t *batch, int32_t i, int32_t *irp, int32_t *ja,
+                                    double *val, double *rhs, int64_t *nnz);
+/* Autotuning: time the persistent solve on a sample of the batch for each
+ * block shape and keep the fastest (also used by femhip_batch_solve when
+ * FEMHIP_AUTOTUNE is set).  Chosen threads-per-block in *threads. */
+int femhip_batch_autotune(femhip_batch_t *batch, int32_t *threads);
+int femhip_batch_destroy(femhip_batch_t *batch);
+
 #ifdef __cplusplus
 }
 #endif
