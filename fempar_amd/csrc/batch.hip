@@ -59,11 +59,10 @@ struct BTables {             // per (dim, n) class: element stiffness, weighted 
   int nd, nq;
 };
 
-__device__ __forceinline__ void diag_offset(int k, int dim, int m, int &dx, int &dy, int &dz, int &off) {
+__device__ __forceinline__ void diag_shift(int k, int dim, int &dx, int &dy, int &dz) {
   dx = k % 3 - 1;
   dy = dim > 1 ? (k / 3) % 3 - 1 : 0;
   dz = dim > 2 ? k / 9 - 1 : 0;
-  off = dx + m * (dy + m * dz);
 }
 
 // ---------------------------------------------------------------- assembly ---
@@ -101,8 +100,8 @@ __global__ __launch_bounds__(kChunk) void k_batch_assemble(const BProb *__restri
   int node[3] = {a[0] + 1, a[1] + 1, a[2] + 1};   // vertex index in [1, n-1]
   double diag = 0.0;
   for (int k = 0; k < P.K; ++k) {
-    int dd[3], off;
-    diag_offset(k, dim, m, dd[0], dd[1], dd[2], off);
+    int dd[3];
+    diag_shift(k, dim, dd[0], dd[1], dd[2]);
     bool valid = true;
     for (int d = 0; d < dim; ++d) valid = valid && (a[d] + dd[d] >= 0) && (a[d] + dd[d] < m);
     double val = 0.0;
@@ -124,7 +123,9 @@ __global__ __launch_bounds__(kChunk) void k_batch_assemble(const BProb *__restri
     }
     V[(int64_t)k * P.stride + r] = val;
     V32[(int64_t)k * P.stride + r] = (float)val;
-    if (off == 0) diag = val;
+    // the centre diagonal is k = K/2; with m = 1 other offsets are 0 as well
+    // (dx = -m dy), so the offset cannot identify it
+    if (k == (P.K >> 1)) diag = val;
   }
   // RHS: sum over the cells around the node of int f N_node
   const double h = 1.0 / P.n;
@@ -262,7 +263,9 @@ __device__ void solve_one(const BProb &P, const BPools &pl, double rtol, double 
   const double bnorm = sqrt(s3[1]);
   const double rhs_test = rtol * bnorm;   // RES_RES with x0 = 0 (cg.f90:305-307)
   double rmax = sqrt(s3[2]), rnorm64 = bnorm;
-  int it = 0, upd = 0, conv = bnorm <= rhs_test;
+  // every thread holds the same reduced values; readfirstlane makes the loop
+  // and branch conditions wave-uniform for the compiler as well
+  int it = 0, upd = 0, conv = __builtin_amdgcn_readfirstlane(bnorm <= rhs_test ? 1 : 0);
   while (!conv && it < maxit) {
     __syncthreads();   // p complete before neighbours are gathered
     double s1[1] = {P.dim == 2 ? dia_spmv32<2>(V32, S, n, P.m, p, q, BS) : dia_spmv32<3>(V32, S, n, P.m, p, q, BS)};
@@ -285,7 +288,8 @@ __device__ void solve_one(const BProb &P, const BPools &pl, double rtol, double 
     ++it;
     double rz_new = s2[0];
     const double rnorm = sqrt(s2[1]);
-    if (rnorm < delta * rmax || rnorm <= rhs_test || it >= maxit) {
+    const int do_ru = __builtin_amdgcn_readfirstlane((rnorm < delta * rmax || rnorm <= rhs_test || it >= maxit) ? 1 : 0);
+    if (do_ru) {
       // reliable update: r = b - A64 (x + d) in FP64 replaces the FP32 residual
       __syncthreads();   // d complete before neighbours are gathered
       double t3[3] = {0, 0, 0};
@@ -300,7 +304,7 @@ __device__ void solve_one(const BProb &P, const BPools &pl, double rtol, double 
       rnorm64 = sqrt(t3[0]);
       rz_new = t3[1];
       rmax = sqrt(t3[2]);
-      if (rnorm64 <= rhs_test) {
+      if (__builtin_amdgcn_readfirstlane(rnorm64 <= rhs_test ? 1 : 0)) {
         conv = 1;
         break;
       }
@@ -318,8 +322,11 @@ __device__ void solve_one(const BProb &P, const BPools &pl, double rtol, double 
       p[i] = t + z;
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
+  // Wave 0 stores the record (all its lanes store the same values): a branch
+  // on threadIdx.x == 0 here, next to the queue fetch of the caller's loop,
+  // lets the compiler thread the two lane-0 branches across the loop's back
+  // edge and park lane 0 while the other lanes loop on a stale index.
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
     out->iters = it;
     out->updates = upd;
     out->conv = conv;
@@ -337,18 +344,25 @@ __global__ void k_batch_start(int *counter, long long *t0) {
 }
 
 // Persistent solve: workgroups take problems from the queue until it is empty
-// (every workgroup reaches the exit once the counter passes `count`).
+// (every workgroup reaches the exit once the counter passes `count`).  Every
+// branch that encloses or follows a barrier is wave-uniform: the queue fetch is
+// made by the whole of wave 0 (lane 0 adds 1, the others 0) and the fetched
+// index is read back through readfirstlane.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_batch_solve(const BProb *__restrict__ probs, const int *__restrict__ order,
                                                     int count, int *counter, BPools pl, double rtol, double delta,
                                                     int maxit, BOut *out) {
   __shared__ double sm[3 * (BS / 64)];
   __shared__ int s_idx;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (;;) {
-    if (threadIdx.x == 0) s_idx = atomicAdd(counter, 1);
+    __syncthreads();   // readers of s_idx and sm from the previous problem are done
+    if (wave == 0) {
+      const int v = atomicAdd(counter, (threadIdx.x & 63) == 0 ? 1 : 0);
+      s_idx = __builtin_amdgcn_readfirstlane(v);   // lane 0's fetch
+    }
     __syncthreads();
-    const int idx = s_idx;
-    __syncthreads();
+    const int idx = __builtin_amdgcn_readfirstlane(s_idx);
     if (idx >= count) break;
     const int pi = order[idx];
     solve_one<BS>(probs[pi], pl, rtol, delta, maxit, out + pi, sm);

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 PROBLEMS = [(2, 4, 1.0, 1.0), (2, 7, 0.1, 1.0), (3, 3, 2.5, 1.0), (2, 16, 9.0, 1.0), (3, 5, 0.7, 2.0),
             (2, 33, 3.3, 1.0), (3, 8, 1.2, 1.0), (2, 2, 1.0, 1.0), (3, 11, 4.0, 0.5), (2, 64, 0.5, 1.0),
-            (3, 16, 7.5, 1.0), (2, 100, 1.0, 1.0)]
+            (3, 16, 7.5, 1.0), (2, 100, 1.0, 1.0), (3, 2, 3.0, 1.0)]   # n = 2: one row, aliased offsets
 
 
 def _oracle(dim, n, k, f0):
@@ -45,7 +45,9 @@ def test_batch_matrices_match_oracle(batch):
 
 def test_batch_solutions(batch):
     for rep in range(2):        # the hipGraph is replayed: same answers
+        print(f"[batch] solve replay {rep} of {len(PROBLEMS)} problems", flush=True)
         res = batch.solve()
+        print(f"[batch] replay {rep}: {res['ms']:.3f} ms, iterations {res['num_iterations'].tolist()}", flush=True)
         assert np.all(res["did_converge"] == 1)
         assert np.all(res["latency_us"] > 0) and res["ms"] > 0
         for i, (dim, n, k, f0) in enumerate(PROBLEMS):
@@ -64,6 +66,7 @@ def test_batch_solutions(batch):
 
 
 def test_batch_autotune(batch):
+    print("[batch] autotune", flush=True)
     t = batch.autotune()
     assert t in (256, 512, 1024)
     res = batch.solve()
