@@ -115,6 +115,54 @@ def run_batched(F, env, steps, rtol=1e-8):
     return out
 
 
+SERVE_RATES = (100, 200, 500, 1000, 2000, 5000)
+
+
+def serving_requests(count, seed=5000):
+    """Serving workload: vector-Q1 elasticity on n^3 with n log-uniform in [8,96],
+    E uniform in [1,200], nu uniform in [0.2,0.45], clamped on x = 0, body force
+    (0,0,-1)."""
+    rng = np.random.default_rng(seed)
+    n = np.rint(np.exp(rng.uniform(np.log(8), np.log(96), count))).astype(int)
+    E = rng.uniform(1.0, 200.0, count)
+    nu = rng.uniform(0.2, 0.45, count)
+    return [(int(n[i]), float(E[i]), float(nu[i])) for i in range(count)]
+
+
+def run_serving(F, env, count, rates=SERVE_RATES, rtol=1e-8):
+    """Open-loop Poisson arrivals at each rate: the same `count` requests are
+    submitted with arrival times from exponential gaps (seed 5000 + rate); the
+    server admits them as slots and HBM budget free up.  Latency = arrival to
+    solution on the host."""
+    reqs = serving_requests(count)
+    out = {"requests_per_rate": count, "rates": {}}
+    for lam in rates:
+        srv = F.serve_t(env, hbm_budget_bytes=96 << 30, max_active=8, setup_threads=8, rtol=rtol)
+        gaps = np.random.default_rng(5000 + lam).exponential(1.0 / lam, count)
+        t0 = srv.now() + 0.02
+        arrivals = t0 + np.cumsum(gaps) - gaps[0]
+        tickets = [srv.submit(n, E, nu, arrival_s=float(a)) for (n, E, nu), a in zip(reqs, arrivals)]
+        res = [srv.wait(t, timeout_s=600) for t in tickets]
+        end = srv.now()
+        st = srv.stats()
+        srv.free()
+        lat = np.array([r["latency_ms"] for r in res])
+        dofs = sum(r["num_dofs"] for r in res)
+        span = end - arrivals[0]
+        out["rates"][str(lam)] = {
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "max_ms": float(lat.max()), "solves_per_s": count / span, "dofs_per_s": dofs / span,
+            "mean_queue_ms": float(np.mean([r["queue_ms"] for r in res])),
+            "mean_setup_ms": float(np.mean([r["setup_ms"] for r in res])),
+            "mean_solve_ms": float(np.mean([r["solve_ms"] for r in res])),
+            "converged": int(sum(r["did_converge"] for r in res)),
+            "max_true_rel_residual": float(max(r["true_residual"] / max(r["rhs_norm"], 1e-300) for r in res)),
+            "peak_active": st["peak_active"], "running_peak": st["running_peak"],
+            "hbm_peak_GB": st["hbm_peak"] / 1e9}
+        print(f"[serve] rate {lam}/s: {out['rates'][str(lam)]}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +173,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32, help="requests per arrival rate")
+    ap.add_argument("--serving-only", action="store_true", help="only the serving sweep (printed as JSON)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only launch the FP64 and FP32 SpMV kernels 50 times each (for rocprofv3 --pmc runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -149,6 +200,9 @@ def main():
     import fempar_amd as F
     env = F.environment_t(0)
     n = args.n
+    if args.serving_only:
+        print(json.dumps(run_serving(F, env, args.serve_requests)))
+        return
     if args.spmv_only:
         tria, fes, di, op = make_problem(F, env, n)
         op.compute()
@@ -197,6 +251,7 @@ def main():
         }
 
     batched = None if args.no_batched else run_batched(F, env, args.steps)
+    serving = None if args.no_serving else run_serving(F, env, args.serve_requests)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -220,6 +275,7 @@ def main():
                      "kernel_ms": spmv_ms, "algorithmic_bytes": spmv_bytes},
         "mixed": mixed,
         "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

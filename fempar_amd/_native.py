@@ -39,6 +39,9 @@ EXPORTS = [
     "femhip_solver_profile_spmv",
     "femhip_batch_create", "femhip_batch_assemble", "femhip_batch_solve", "femhip_batch_get_solution",
     "femhip_batch_get_problem_matrix", "femhip_batch_autotune", "femhip_batch_destroy",
+    "femhip_serve_create", "femhip_serve_estimate_bytes", "femhip_serve_submit", "femhip_serve_wait",
+    "femhip_serve_get_solution", "femhip_serve_release", "femhip_serve_stats", "femhip_serve_now",
+    "femhip_serve_destroy",
 ]
 
 
@@ -81,6 +84,30 @@ class CgResult(C.Structure):
                 ("true_residual", C.c_double), ("rhs_norm", C.c_double), ("solve_ms", C.c_double)]
 
 
+class ServeParams(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_double), ("max_active", C.c_int32), ("setup_threads", C.c_int32),
+                ("cg", CgParams)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("n", C.c_int32), ("E", C.c_double), ("nu", C.c_double), ("body", C.c_double * 3),
+                ("arrival_s", C.c_double)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("num_iterations", C.c_int32), ("outer_iterations", C.c_int32),
+                ("did_converge", C.c_int32), ("num_dofs", C.c_int32), ("pad", C.c_int32),
+                ("true_residual", C.c_double), ("rhs_norm", C.c_double), ("queue_ms", C.c_double),
+                ("setup_ms", C.c_double), ("solve_ms", C.c_double), ("latency_ms", C.c_double),
+                ("hbm_bytes", C.c_double), ("hbm_estimate", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("queued", C.c_int32), ("active", C.c_int32), ("peak_active", C.c_int32),
+                ("running_peak", C.c_int32), ("hbm_in_use", C.c_double), ("hbm_peak", C.c_double)]
+
+
 _lib = None
 
 
@@ -117,11 +144,22 @@ def load():
         "femhip_batch_get_problem_matrix": [P, I32, P, P, P, P, C.POINTER(I64)],
         "femhip_batch_autotune": [P, C.POINTER(I32)],
         "femhip_batch_destroy": [P],
+        "femhip_serve_create": [P, C.POINTER(ServeParams), pP],
+        "femhip_serve_estimate_bytes": [I32, I32],
+        "femhip_serve_submit": [P, C.POINTER(ServeRequest), C.POINTER(I64)],
+        "femhip_serve_wait": [P, I64, D, C.POINTER(ServeResult)],
+        "femhip_serve_get_solution": [P, I64, P],
+        "femhip_serve_release": [P, I64],
+        "femhip_serve_stats": [P, C.POINTER(ServeStats)],
+        "femhip_serve_now": [P],
+        "femhip_serve_destroy": [P],
     }
+    restype = {"femhip_last_error": C.c_char_p, "femhip_serve_estimate_bytes": C.c_double,
+               "femhip_serve_now": C.c_double}
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
-        fn.restype = C.c_char_p if name == "femhip_last_error" else C.c_int
+        fn.restype = restype.get(name, C.c_int)
     _lib = L
     return L
 

@@ -753,6 +753,7 @@ struct femhip_solver {
   DevBuf<CgResult> res;
   DevBuf<int> flag;
   CgResult *h_res[2] = {nullptr, nullptr};   // pinned polling slots
+  hipEvent_t ev[2] = {nullptr, nullptr};     // completion of the copy into each slot
   hipGraphExec_t graph = nullptr;
   int graph_kind = -1, graph_chunk = 0;
   bool prepared = false;
@@ -760,6 +761,8 @@ struct femhip_solver {
   ~femhip_solver() {
     for (auto *h : h_res)
       if (h) (void)hipHostFree(h);
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
     if (graph) (void)hipGraphExecDestroy(graph);
   }
 };
@@ -821,6 +824,7 @@ void prepare(femhip_solver &s) {
       if (jac) s.invd32.alloc(n);
     }
     for (auto &h : s.h_res) FH_HIP(hipHostMalloc((void **)&h, sizeof(CgResult), hipHostMallocDefault));
+    for (auto &e : s.ev) FH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     s.prepared = true;
   }
   // matrix-dependent work, redone whenever the operator was recomputed
@@ -876,53 +880,61 @@ void enqueue_iterations(femhip_solver &s, int chunk) {
 
 constexpr int kChunk = 16;
 
-void launch_chunk(femhip_solver &s) {
-  hipStream_t st = s.ctx->stream;
-  const int kind = mixed(s) ? 1 : 0;
+bool use_graph() {
   static const bool no_graph = getenv("FEMHIP_NO_GRAPH") != nullptr;
-  if (no_graph) {
+  return !no_graph;
+}
+
+// capture (once per precision) the chunk of kChunk iterations as a hipGraph
+void ensure_graph(femhip_solver &s) {
+  const int kind = mixed(s) ? 1 : 0;
+  if (!use_graph() || (s.graph && s.graph_kind == kind)) return;
+  if (s.graph) FH_HIP(hipGraphExecDestroy(s.graph));
+  s.graph = nullptr;
+  hipStream_t st = s.ctx->stream;
+  hipGraph_t g;
+  FH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  enqueue_iterations(s, kChunk);
+  FH_HIP(hipStreamEndCapture(st, &g));
+  FH_HIP(hipGraphInstantiate(&s.graph, g, nullptr, nullptr, 0));
+  FH_HIP(hipGraphDestroy(g));
+  s.graph_kind = kind;
+}
+
+void launch_chunk(femhip_solver &s) {
+  if (!use_graph()) {
     enqueue_iterations(s, kChunk);
     FH_HIP(hipGetLastError());
     return;
   }
-  if (!s.graph || s.graph_kind != kind) {
-    if (s.graph) FH_HIP(hipGraphExecDestroy(s.graph));
-    s.graph = nullptr;
-    hipGraph_t g;
-    FH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    enqueue_iterations(s, kChunk);
-    FH_HIP(hipStreamEndCapture(st, &g));
-    FH_HIP(hipGraphInstantiate(&s.graph, g, nullptr, nullptr, 0));
-    FH_HIP(hipGraphDestroy(g));
-    s.graph_kind = kind;
-  }
-  FH_HIP(hipGraphLaunch(s.graph, st));
+  ensure_graph(s);
+  FH_HIP(hipGraphLaunch(s.graph, s.ctx->stream));
+}
+
+// one chunk, then an asynchronous copy of the result block into pinned slot
+void push_chunk(femhip_solver &s, int slot) {
+  hipStream_t st = s.ctx->stream;
+  launch_chunk(s);
+  FH_HIP(hipMemcpyAsync(s.h_res[slot], s.res.p, sizeof(CgResult), hipMemcpyDeviceToHost, st));
+  FH_HIP(hipEventRecord(s.ev[slot], st));
 }
 
 // launch chunks until the device state says done; one chunk of look-ahead
 void iterate(femhip_solver &s, int maxit) {
-  hipStream_t st = s.ctx->stream;
-  hipEvent_t ev[2];
-  FH_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  FH_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   int queued = 0, cur = 0;
-  auto push = [&](int slot) {
-    launch_chunk(s);
-    queued += kChunk;
-    FH_HIP(hipMemcpyAsync(s.h_res[slot], s.res.p, sizeof(CgResult), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipEventRecord(ev[slot], st));
-  };
-  push(0);
+  push_chunk(s, 0);
+  queued += kChunk;
   for (;;) {
     const bool more = queued < maxit;
-    if (more) push(cur ^ 1);
-    FH_HIP(hipEventSynchronize(ev[cur]));
+    if (more) {
+      push_chunk(s, cur ^ 1);
+      queued += kChunk;
+    }
+    FH_HIP(hipEventSynchronize(s.ev[cur]));
     if (s.h_res[cur]->done || !more) break;
     cur ^= 1;
   }
-  FH_HIP(hipStreamSynchronize(st));
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
+  FH_HIP(hipStreamSynchronize(s.ctx->stream));
 }
 
 void upload_state(femhip_solver &s) {
@@ -935,7 +947,8 @@ void upload_state(femhip_solver &s) {
   FH_HIP(hipMemcpyAsync(s.st.p, &h, sizeof(h), hipMemcpyHostToDevice, s.ctx->stream));
 }
 
-void solve(femhip_solver &s, bool x_zero, femhip_cg_result_t *out) {
+// initial residual, direction and device state (x and b already in place)
+void begin(femhip_solver &s, bool x_zero) {
   hipStream_t st = s.ctx->stream;
   femhip_operator &op = *s.op;
   const SlicedDev m = sliced_dev(op);
@@ -951,7 +964,10 @@ void solve(femhip_solver &s, bool x_zero, femhip_cg_result_t *out) {
   hipLaunchKernelGGL(mixed(s) ? k_ru_init_fin : k_init_fin, dim3(1), dim3(kThreads), 0, st, s.part_i.p, s.g_mat,
                      s.st.p, s.res.p);
   FH_HIP(hipGetLastError());
-  iterate(s, s.prm.max_num_iterations);
+}
+
+// result of the finished iteration (the stream has been synchronized)
+void read_result(femhip_solver &s, femhip_cg_result_t *out) {
   CgResult r;
   FH_HIP(hipMemcpy(&r, s.res.p, sizeof(r), hipMemcpyDeviceToHost));
   out->num_iterations = r.iter;
@@ -960,6 +976,12 @@ void solve(femhip_solver &s, bool x_zero, femhip_cg_result_t *out) {
   out->error_estimate = r.est;
   out->rhs_convergence_test = r.rhs_test;
   out->rhs_norm = r.bnorm;
+}
+
+void solve(femhip_solver &s, bool x_zero, femhip_cg_result_t *out) {
+  begin(s, x_zero);
+  iterate(s, s.prm.max_num_iterations);
+  read_result(s, out);
 }
 
 double true_residual(femhip_solver &s) {
@@ -977,6 +999,51 @@ double true_residual(femhip_solver &s) {
 }
 
 }  // namespace
+
+// ------------------------------------------- non-blocking driver (server) ---
+
+int solver_chunk() { return kChunk; }
+
+void solver_start(femhip_solver &s) {
+  prepare(s);
+  FH_CHECK(s.n > 0, FEMHIP_ERR_STATE, "operator without free DOFs");
+  hipStream_t st = s.ctx->stream;
+  FH_HIP(hipMemcpyAsync(s.b.p, s.op->rhs.p, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s.n, s.x.p);
+  begin(s, true);
+  ensure_graph(s);
+}
+
+void solver_push(femhip_solver &s, int slot) { push_chunk(s, slot); }
+
+int solver_poll(femhip_solver &s, int slot) {
+  const hipError_t e = hipEventQuery(s.ev[slot]);
+  if (e == hipErrorNotReady) return 0;
+  FH_HIP(e);
+  return s.h_res[slot]->done ? 2 : 1;
+}
+
+void solver_finish(femhip_solver &s, femhip_cg_result_t *out) {
+  FH_HIP(hipStreamSynchronize(s.ctx->stream));
+  std::memset(out, 0, sizeof(*out));
+  read_result(s, out);
+  out->true_residual = true_residual(s);
+  out->status = out->did_converge ? FEMHIP_OK : FEMHIP_ERR_NOT_CONVERGED;
+  s.have_solution = true;
+}
+
+void solver_copy_solution(const femhip_solver &s, double *x) {
+  FH_HIP(hipMemcpy(x, s.x.p, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToHost));
+}
+
+size_t solver_device_bytes(const femhip_solver &s) {
+  size_t b = s.st.bytes() + s.res.bytes() + s.flag.bytes();
+  for (auto *v : {&s.x, &s.b, &s.r, &s.p, &s.q, &s.invd, &s.part_i, &s.part_s, &s.part_u, &s.part_r, &s.hist})
+    b += v->bytes();
+  for (auto *v : {&s.r32, &s.p32, &s.q32, &s.d32, &s.invd32}) b += v->bytes();
+  return b;
+}
+
 }  // namespace femhip
 
 extern "C" {

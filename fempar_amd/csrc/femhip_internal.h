@@ -195,4 +195,29 @@ void assemble(femhip_operator &op, const double *fixed_host, float *ms);
 void upload_physics(femhip_operator &op);
 void spmv_full(femhip_operator &op, const double *d_x, double *d_y);
 
+// non-blocking CG driver (cg.hip) used by the request server: solver_start
+// prepares the solver format and enqueues the initial residual for b = the
+// operator's RHS and x0 = 0; solver_push enqueues one chunk of solver_chunk()
+// iterations followed by a copy of the result block into pinned slot 0 or 1;
+// solver_poll returns 0 while that copy is pending, else 2 once the device
+// state says done and 1 otherwise; solver_finish waits for the stream.
+int solver_chunk();
+void solver_start(femhip_solver &s);
+void solver_push(femhip_solver &s, int slot);
+int solver_poll(femhip_solver &s, int slot);
+void solver_finish(femhip_solver &s, femhip_cg_result_t *out);
+void solver_copy_solution(const femhip_solver &s, double *x);
+size_t solver_device_bytes(const femhip_solver &s);
+
+// device memory held by an operator (CSR, solver format, tables) and its space
+inline size_t operator_device_bytes(const femhip_operator &op) {
+  size_t b = op.rowptr.bytes() + op.col.bytes() + op.val.bytes() + op.rhs.bytes() + op.val32.bytes() +
+             op.upper_map.bytes() + op.k_cell.bytes() + op.tab.bytes() + op.fixed.bytes();
+  const Sliced &s = op.sl;
+  b += s.soffs.bytes() + s.base.bytes() + s.meta.bytes() + s.aux.bytes() + s.offs.bytes() + s.ecol.bytes() +
+       s.map.bytes() + s.v64.bytes() + s.v32.bytes();
+  if (op.space) b += op.space->d_dofs.bytes();
+  return b;
+}
+
 }  // namespace femhip

@@ -1,0 +1,502 @@
+// serve.cpp -- request loop with continuous batching for elasticity solves.
+//
+// Threads:
+//  * the scheduler admits arrived requests (first come, first served) while a
+//    slot is free and the request's device-memory estimate fits in the budget,
+//    and interleaves the CG chunks of all running requests: whenever the result
+//    copy of a request's earlier chunk has landed it either retires the request
+//    (device state says done, or the iteration budget is queued) or queues the
+//    next chunk on the request's slot stream, keeping two chunks in flight;
+//  * setup workers turn an admitted request into device state on its slot's
+//    context (numbering, CSR pattern, upload, integration + assembly, solver
+//    format, initial residual, graph capture) and retire finished requests
+//    (result, true residual, solution to the host, device memory freed).
+// Each slot owns a femhip context with its own non-blocking stream, so a
+// request's kernels, reductions and launch shapes are those of a solve alone.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "femhip_internal.h"
+
+using namespace femhip;
+
+namespace {
+
+enum ReqState { QUEUED, SETUP, RUNNING, RETIRING, DONE };
+
+struct Req {
+  int64_t ticket = 0;
+  femhip_serve_request_t rq{};
+  double arrival = 0.0, estimate = 0.0;
+  std::atomic<int> state{QUEUED};
+  int slot = -1;
+  femhip_space_t *space = nullptr;
+  femhip_operator_t *op = nullptr;
+  femhip_solver_t *solver = nullptr;
+  int cur = 0;
+  int64_t queued_its = 0;
+  double t_admit = 0.0, t_run = 0.0, t_done = 0.0;
+  femhip_serve_result_t res{};
+  std::vector<double> x;
+};
+using ReqP = std::shared_ptr<Req>;
+
+}  // namespace
+
+struct femhip_server {
+  femhip_ctx *ctx = nullptr;
+  femhip_serve_params_t prm{};
+  std::chrono::steady_clock::time_point epoch;
+  std::vector<femhip_ctx *> slot_ctx;
+  std::vector<int> free_slots;
+  std::mutex mu;
+  std::condition_variable cv_sched, cv_work, cv_done;
+  std::map<int64_t, ReqP> reqs;              // every request not yet released
+  std::multimap<std::pair<double, int64_t>, ReqP> pending;   // by (arrival, ticket)
+  std::deque<ReqP> setup_q, ready_q, retire_q;
+  int64_t next_ticket = 1, submitted = 0, completed = 0, failed = 0;
+  int active = 0, peak_active = 0, running_peak = 0;
+  double hbm_in_use = 0.0, hbm_peak = 0.0;
+  bool stop = false;
+  std::thread sched;
+  std::vector<std::thread> workers;
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - epoch).count();
+  }
+};
+
+namespace {
+
+// Device memory bound for a request: space DOF table, CSR (rowptr, col, FP64
+// values, RHS), sliced solver copy (map, FP64 + FP32 values, ELL columns,
+// per-slice data), FP64 and FP32 Krylov vectors and the residual history.  The
+// sliced entries are bounded by 64-row slices of at most 1.25 x 81 + 1
+// diagonals (the diagonal-slice criterion) or 81 ELL columns.
+double estimate_bytes(int32_t n, int32_t maxit) {
+  const double nn = n, ndof = 3.0 * nn * (nn + 1) * (nn + 1), cells = nn * nn * nn;
+  const double nnz = 81.0 * ndof;
+  const double slices = std::ceil(ndof / 64.0);
+  const double entries = 102.0 * 64.0 * slices;
+  const double csr = 4.0 * (ndof + 1) + 12.0 * nnz + 8.0 * ndof;
+  const double sliced = 20.0 * entries + 4.0 * 102.0 * slices + 24.0 * slices;
+  const double vecs = 48.0 * ndof + 20.0 * ndof + 8.0 * std::max(1, maxit);
+  const double other = 96.0 * cells + 24.0 * (nn + 1) * (nn + 1) + (4 << 20);
+  return csr + sliced + vecs + other;
+}
+
+void free_request_objects(Req &r) {
+  if (r.solver) femhip_solver_destroy(r.solver);
+  if (r.op) femhip_operator_destroy(r.op);
+  if (r.space) femhip_space_destroy(r.space);
+  r.solver = nullptr;
+  r.op = nullptr;
+  r.space = nullptr;
+}
+
+void check_rc(int rc) {
+  if (rc != FEMHIP_OK) throw Error{rc, femhip_last_error()};
+}
+
+// numbering, pattern, assembly and solver start on the request's slot context
+void setup_request(femhip_server &S, Req &r) {
+  femhip_ctx *c = S.slot_ctx[(size_t)r.slot];
+  FH_HIP(hipSetDevice(c->device));
+  femhip_mesh_desc_t m{};
+  m.dim = 3;
+  m.n[0] = m.n[1] = m.n[2] = r.rq.n;
+  for (int k = 0; k < 3; ++k) {
+    m.limits[2 * k] = 0.0;
+    m.limits[2 * k + 1] = 1.0;
+  }
+  m.order = 1;
+  m.num_components = 3;
+  m.dirichlet = FEMHIP_BC_X0_FACE;
+  check_rc(femhip_space_create_structured(c, &m, &r.space));
+  check_rc(femhip_operator_create(c, r.space, 0, &r.op));
+  femhip_physics_t ph{};
+  ph.kind = FEMHIP_PHYS_ELASTICITY;
+  ph.E = r.rq.E;
+  ph.nu = r.rq.nu;
+  for (int k = 0; k < 3; ++k) ph.body[k] = r.rq.body[k];
+  check_rc(femhip_operator_set_physics(r.op, &ph));
+  check_rc(femhip_operator_compute(r.op, nullptr, nullptr));
+  check_rc(femhip_solver_create(c, r.op, &S.prm.cg, &r.solver));
+  solver_start(*r.solver);
+  FH_HIP(hipStreamSynchronize(c->stream));
+  r.res.hbm_bytes = (double)(operator_device_bytes(*r.op) + solver_device_bytes(*r.solver));
+  r.res.num_dofs = r.op->n;
+}
+
+// result, true residual and solution copy, then the device memory is freed
+void retire_request(femhip_server &S, Req &r) {
+  (void)S;
+  femhip_cg_result_t cg{};
+  solver_finish(*r.solver, &cg);
+  r.x.resize((size_t)r.res.num_dofs);
+  solver_copy_solution(*r.solver, r.x.data());
+  r.res.status = cg.status;
+  r.res.num_iterations = cg.num_iterations;
+  r.res.outer_iterations = cg.outer_iterations;
+  r.res.did_converge = cg.did_converge;
+  r.res.true_residual = cg.true_residual;
+  r.res.rhs_norm = cg.rhs_norm;
+}
+
+void finish_locked(femhip_server &S, Req &r, int status_if_failed) {
+  r.t_done = S.now();
+  if (status_if_failed != FEMHIP_OK) {
+    r.res.status = status_if_failed;
+    ++S.failed;
+  }
+  r.res.queue_ms = (r.t_admit - r.arrival) * 1e3;
+  r.res.setup_ms = (r.t_run > 0 ? r.t_run - r.t_admit : 0.0) * 1e3;
+  r.res.solve_ms = (r.t_run > 0 ? r.t_done - r.t_run : 0.0) * 1e3;
+  r.res.latency_ms = (r.t_done - r.arrival) * 1e3;
+  r.res.hbm_estimate = r.estimate;
+  r.state = DONE;
+  ++S.completed;
+  S.free_slots.push_back(r.slot);
+  S.hbm_in_use -= r.estimate;
+  --S.active;
+  S.cv_done.notify_all();
+  S.cv_sched.notify_all();
+}
+
+void worker_loop(femhip_server &S) {
+  (void)hipSetDevice(S.ctx->device);
+  for (;;) {
+    ReqP r;
+    bool retire = false;
+    {
+      std::unique_lock<std::mutex> lk(S.mu);
+      // exit only once nothing admitted is left to set up or retire
+      S.cv_work.wait(lk, [&] { return !S.retire_q.empty() || !S.setup_q.empty() || (S.stop && S.active == 0); });
+      if (!S.retire_q.empty()) {
+        r = S.retire_q.front();
+        S.retire_q.pop_front();
+        retire = true;
+      } else if (!S.setup_q.empty()) {
+        r = S.setup_q.front();
+        S.setup_q.pop_front();
+      } else {
+        return;
+      }
+    }
+    int status = FEMHIP_OK;
+    std::string msg;
+    try {
+      if (retire) retire_request(S, *r);
+      else setup_request(S, *r);
+    } catch (const Error &e) {
+      status = e.code;
+      msg = e.msg;
+    } catch (...) {
+      status = FEMHIP_ERR_OOM;
+    }
+    if (retire || status != FEMHIP_OK) {
+      if (status != FEMHIP_OK && r->solver) (void)hipStreamSynchronize(S.slot_ctx[(size_t)r->slot]->stream);
+      free_request_objects(*r);
+      std::lock_guard<std::mutex> lk(S.mu);
+      finish_locked(S, *r, status);
+    } else {
+      std::lock_guard<std::mutex> lk(S.mu);
+      S.ready_q.push_back(r);
+      S.cv_sched.notify_all();
+    }
+  }
+}
+
+void scheduler_loop(femhip_server &S) {
+  (void)hipSetDevice(S.ctx->device);
+  const int chunk = solver_chunk();
+  const int64_t maxit = S.prm.cg.max_num_iterations;
+  std::vector<ReqP> running;
+  for (;;) {
+    std::vector<ReqP> started;
+    {
+      std::unique_lock<std::mutex> lk(S.mu);
+      if (S.stop && running.empty() && S.active == 0) return;
+      // admissions: arrived requests in order while a slot and memory are free
+      const double t = S.now();
+      while (!S.stop && !S.pending.empty()) {
+        auto it = S.pending.begin();
+        ReqP r = it->second;
+        if (r->arrival > t || S.free_slots.empty() || S.hbm_in_use + r->estimate > S.prm.hbm_budget_bytes) break;
+        S.pending.erase(it);
+        r->slot = S.free_slots.back();
+        S.free_slots.pop_back();
+        r->state = SETUP;
+        r->t_admit = t;
+        S.hbm_in_use += r->estimate;
+        S.hbm_peak = std::max(S.hbm_peak, S.hbm_in_use);
+        ++S.active;
+        S.peak_active = std::max(S.peak_active, S.active);
+        S.setup_q.push_back(r);
+        S.cv_work.notify_one();
+      }
+      while (!S.ready_q.empty()) {
+        started.push_back(S.ready_q.front());
+        S.ready_q.pop_front();
+      }
+      if (running.empty() && started.empty()) {
+        // nothing on the device: sleep until work arrives or the next arrival time
+        double wait_s = 0.05;
+        if (!S.pending.empty()) wait_s = std::min(wait_s, std::max(0.0, S.pending.begin()->second->arrival - t));
+        S.cv_sched.wait_for(lk, std::chrono::duration<double>(wait_s));
+        continue;
+      }
+    }
+    for (auto &r : started) {
+      try {
+        solver_push(*r->solver, 0);
+        r->queued_its = chunk;
+        if (r->queued_its < maxit) {
+          solver_push(*r->solver, 1);
+          r->queued_its += chunk;
+        }
+        r->cur = 0;
+        r->state = RUNNING;
+        r->t_run = S.now();
+        running.push_back(r);
+      } catch (const Error &e) {
+        r->res.status = e.code;
+        std::lock_guard<std::mutex> lk(S.mu);
+        r->state = RETIRING;
+        S.retire_q.push_back(r);
+        S.cv_work.notify_one();
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(S.mu);
+      S.running_peak = std::max(S.running_peak, (int)running.size());
+    }
+    bool progress = !started.empty();
+    for (size_t i = 0; i < running.size();) {
+      Req &r = *running[i];
+      int st = 0;
+      try {
+        st = solver_poll(*r.solver, r.cur);
+      } catch (const Error &) {
+        st = 2;   // the retire step reports the error
+      }
+      if (st == 0) {
+        ++i;
+        continue;
+      }
+      progress = true;
+      if (st == 2 || r.queued_its >= maxit) {
+        std::lock_guard<std::mutex> lk(S.mu);
+        r.state = RETIRING;
+        S.retire_q.push_back(running[i]);
+        S.cv_work.notify_one();
+        running[i] = running.back();
+        running.pop_back();
+        continue;
+      }
+      try {
+        solver_push(*r.solver, r.cur);   // the slot's previous result has been read
+        r.queued_its += chunk;
+        r.cur ^= 1;
+      } catch (const Error &) {
+        std::lock_guard<std::mutex> lk(S.mu);
+        r.state = RETIRING;
+        S.retire_q.push_back(running[i]);
+        S.cv_work.notify_one();
+        running[i] = running.back();
+        running.pop_back();
+        continue;
+      }
+      ++i;
+    }
+    if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+double femhip_serve_estimate_bytes(int32_t n, int32_t max_num_iterations) {
+  return estimate_bytes(n, max_num_iterations);
+}
+
+int femhip_serve_create(femhip_ctx_t *ctx, const femhip_serve_params_t *params, femhip_server_t **server) {
+  try {
+    FH_CHECK(ctx && params && server, FEMHIP_ERR_INVALID, "null argument");
+    const femhip_serve_params_t &p = *params;
+    FH_CHECK(p.max_active >= 1 && p.max_active <= 64, FEMHIP_ERR_INVALID, "max_active must be in [1, 64]");
+    FH_CHECK(p.setup_threads >= 1 && p.setup_threads <= 64, FEMHIP_ERR_INVALID, "setup_threads must be in [1, 64]");
+    FH_CHECK(p.hbm_budget_bytes > 0, FEMHIP_ERR_INVALID, "hbm_budget_bytes must be positive");
+    FH_CHECK(p.cg.precision == FEMHIP_PRECISION_FP32_REFINED, FEMHIP_ERR_INVALID, "the server runs FP32_REFINED");
+    FH_CHECK(p.cg.max_num_iterations >= 1 && p.cg.rtol >= 0 && p.cg.atol >= 0, FEMHIP_ERR_INVALID,
+             "bad solver parameters");
+    FH_CHECK(p.cg.stopping_criterion == FEMHIP_RES_RES || p.cg.stopping_criterion == FEMHIP_RES_RHS,
+             FEMHIP_ERR_INVALID, "FP32_REFINED supports the RES_RES and RES_RHS criteria");
+    FH_CHECK(p.cg.preconditioner == FEMHIP_PREC_IDENTITY || p.cg.preconditioner == FEMHIP_PREC_JACOBI,
+             FEMHIP_ERR_INVALID, "unknown preconditioner");
+    FH_CHECK(p.cg.inner_rtol >= 0 && p.cg.inner_rtol < 1, FEMHIP_ERR_INVALID, "reliable-update factor in [0,1)");
+    FH_HIP(hipSetDevice(ctx->device));
+    auto *S = new femhip_server();
+    S->ctx = ctx_retain(ctx);
+    S->prm = p;
+    S->epoch = std::chrono::steady_clock::now();
+    try {
+      for (int i = 0; i < p.max_active; ++i) {
+        femhip_ctx_t *c = nullptr;
+        check_rc(femhip_ctx_create(ctx->device, &c));
+        S->slot_ctx.push_back(c);
+        S->free_slots.push_back(p.max_active - 1 - i);
+      }
+      S->sched = std::thread(scheduler_loop, std::ref(*S));
+      for (int i = 0; i < p.setup_threads; ++i) S->workers.emplace_back(worker_loop, std::ref(*S));
+    } catch (...) {
+      femhip_serve_destroy(S);
+      throw;
+    }
+    *server = S;
+    return FEMHIP_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
+int femhip_serve_submit(femhip_server_t *S, const femhip_serve_request_t *rq, int64_t *ticket) {
+  try {
+    FH_CHECK(S && rq && ticket, FEMHIP_ERR_INVALID, "null argument");
+    FH_CHECK(rq->n >= 1 && rq->n <= 256, FEMHIP_ERR_INVALID, "request needs 1 <= n <= 256");
+    FH_CHECK(rq->E > 0 && rq->nu > -1.0 && rq->nu < 0.5, FEMHIP_ERR_INVALID, "invalid E or nu");
+    const double est = estimate_bytes(rq->n, S->prm.cg.max_num_iterations);
+    FH_CHECK(est <= S->prm.hbm_budget_bytes, FEMHIP_ERR_ADMISSION,
+             "request needs an estimated " + std::to_string((long long)est) +
+                 " device bytes, more than the whole HBM budget");
+    auto r = std::make_shared<Req>();
+    r->rq = *rq;
+    r->estimate = est;
+    std::lock_guard<std::mutex> lk(S->mu);
+    FH_CHECK(!S->stop, FEMHIP_ERR_STATE, "server is shutting down");
+    r->ticket = S->next_ticket++;
+    r->arrival = rq->arrival_s >= 0 ? rq->arrival_s : S->now();
+    S->reqs[r->ticket] = r;
+    S->pending.emplace(std::make_pair(r->arrival, r->ticket), r);
+    ++S->submitted;
+    *ticket = r->ticket;
+    S->cv_sched.notify_all();
+    return FEMHIP_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
+int femhip_serve_wait(femhip_server_t *S, int64_t ticket, double timeout_s, femhip_serve_result_t *result) {
+  try {
+    FH_CHECK(S && result, FEMHIP_ERR_INVALID, "null argument");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    FH_CHECK(it != S->reqs.end(), FEMHIP_ERR_INVALID, "unknown or released ticket");
+    ReqP r = it->second;
+    auto done = [&] { return r->state == DONE; };
+    if (timeout_s < 0) S->cv_done.wait(lk, done);
+    else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done))
+      throw Error{FEMHIP_ERR_STATE, "request not finished within the timeout"};
+    *result = r->res;
+    return FEMHIP_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
+int femhip_serve_get_solution(femhip_server_t *S, int64_t ticket, double *x) {
+  try {
+    FH_CHECK(S && x, FEMHIP_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    FH_CHECK(it != S->reqs.end(), FEMHIP_ERR_INVALID, "unknown or released ticket");
+    FH_CHECK(it->second->state == DONE, FEMHIP_ERR_STATE, "request not finished");
+    FH_CHECK(!it->second->x.empty(), FEMHIP_ERR_STATE, "request failed: no solution");
+    std::memcpy(x, it->second->x.data(), sizeof(double) * it->second->x.size());
+    return FEMHIP_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
+int femhip_serve_release(femhip_server_t *S, int64_t ticket) {
+  try {
+    FH_CHECK(S, FEMHIP_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    FH_CHECK(it != S->reqs.end(), FEMHIP_ERR_INVALID, "unknown or released ticket");
+    FH_CHECK(it->second->state == DONE, FEMHIP_ERR_STATE, "request not finished");
+    S->reqs.erase(it);
+    return FEMHIP_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
+int femhip_serve_stats(femhip_server_t *S, femhip_serve_stats_t *st) {
+  try {
+    FH_CHECK(S && st, FEMHIP_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    st->submitted = S->submitted;
+    st->completed = S->completed;
+    st->failed = S->failed;
+    st->queued = (int32_t)S->pending.size();
+    st->active = S->active;
+    st->peak_active = S->peak_active;
+    st->running_peak = S->running_peak;
+    st->hbm_in_use = S->hbm_in_use;
+    st->hbm_peak = S->hbm_peak;
+    return FEMHIP_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
+double femhip_serve_now(femhip_server_t *S) { return S ? S->now() : 0.0; }
+
+int femhip_serve_destroy(femhip_server_t *S) {
+  if (!S) return FEMHIP_OK;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+    for (auto &kv : S->pending) {   // never admitted: dropped
+      kv.second->state = DONE;
+      kv.second->res.status = FEMHIP_ERR_STATE;
+    }
+    S->pending.clear();
+    S->cv_sched.notify_all();
+    S->cv_done.notify_all();
+  }
+  if (S->sched.joinable()) S->sched.join();   // returns once every admitted request has retired
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->cv_work.notify_all();
+  }
+  for (auto &w : S->workers)
+    if (w.joinable()) w.join();
+  for (auto *c : S->slot_ctx) femhip_ctx_destroy(c);
+  ctx_release(S->ctx);
+  delete S;
+  return FEMHIP_OK;
+}
+
+}  // extern "C"

@@ -463,3 +463,71 @@ class batch_t:
             self.free()
         except Exception:
             pass
+
+
+class serve_t:
+    """Request loop with continuous batching (femhip_serve_*): each request is
+    a vector-Q1 linear elasticity solve on the unit cube with n cells per
+    direction, clamped on x = 0, constant body force, FP32-refined Jacobi-PCG."""
+
+    def __init__(self, environment: environment_t, hbm_budget_bytes=64 << 30, max_active=8, setup_threads=4,
+                 rtol=1e-8, max_num_iterations=20000, delta=0.0):
+        self._lib = N.load()
+        self.environment = environment
+        p = N.ServeParams()
+        p.hbm_budget_bytes, p.max_active, p.setup_threads = float(hbm_budget_bytes), max_active, setup_threads
+        p.cg.rtol, p.cg.atol, p.cg.max_num_iterations = rtol, 0.0, max_num_iterations
+        p.cg.stopping_criterion = N.STOPPING_CRITERIA.index("RES_RES")
+        p.cg.preconditioner, p.cg.precision, p.cg.inner_rtol = N.PREC_JACOBI, N.PRECISION_FP32_REFINED, delta
+        self.max_num_iterations = max_num_iterations
+        h = C.c_void_p()
+        N.check(self._lib.femhip_serve_create(environment.h, C.byref(p), C.byref(h)))
+        self.h = h
+        self._dofs = {}
+
+    def estimate_bytes(self, n):
+        return self._lib.femhip_serve_estimate_bytes(int(n), self.max_num_iterations)
+
+    def now(self):
+        return self._lib.femhip_serve_now(self.h)
+
+    def submit(self, n, E, nu, body=(0.0, 0.0, -1.0), arrival_s=-1.0):
+        rq = N.ServeRequest()
+        rq.n, rq.E, rq.nu, rq.arrival_s = int(n), float(E), float(nu), float(arrival_s)
+        for i in range(3):
+            rq.body[i] = float(body[i])
+        t = C.c_int64()
+        N.check(self._lib.femhip_serve_submit(self.h, C.byref(rq), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout_s=-1.0):
+        r = N.ServeResult()
+        N.check(self._lib.femhip_serve_wait(self.h, ticket, float(timeout_s), C.byref(r)))
+        out = {f: getattr(r, f) for f, _ in N.ServeResult._fields_ if f != "pad"}
+        self._dofs[ticket] = r.num_dofs
+        return out
+
+    def get_solution(self, ticket):
+        x = np.empty(self._dofs[ticket])
+        N.check(self._lib.femhip_serve_get_solution(self.h, ticket, N.ptr(x)))
+        return x
+
+    def release(self, ticket):
+        N.check(self._lib.femhip_serve_release(self.h, ticket))
+        self._dofs.pop(ticket, None)
+
+    def stats(self):
+        s = N.ServeStats()
+        N.check(self._lib.femhip_serve_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in N.ServeStats._fields_}
+
+    def free(self):
+        if getattr(self, "h", None):
+            self._lib.femhip_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

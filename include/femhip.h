@@ -253,6 +253,88 @@ int femhip_batch_get_problem_matrix(const femhip_batch_t *batch, int32_t i, int3
 int femhip_batch_autotune(femhip_batch_t *batch, int32_t *threads);
 int femhip_batch_destroy(femhip_batch_t *batch);
 
+/* ---- serving form: a request loop with continuous batching ----
+ * A request is one linear elasticity solve (the irreducible integration of
+ * Tests/Par/par_test_pb_bddc_linear_elasticity/modules/
+ * sbm_irreducible_discrete_integration.i90:64-130 with lambda, mu from E, nu as
+ * in par_test_elasticity_constitutive_models.f90:34-37) on the unit cube with n
+ * hexahedral Q1 cells per direction, vector Q1 (3 DOFs per node), all
+ * components clamped (g = 0) on the x = 0 face, constant body force; the space,
+ * operator and CG are those of femhip_space_create_structured,
+ * femhip_operator_compute and femhip_solver_solve with FP32_REFINED precision.
+ *
+ * The server owns `max_active` slots, each with its own HIP stream.  A request
+ * is admitted when its arrival time has come, a slot is free and its device
+ * memory estimate fits in what is left of `hbm_budget_bytes` (first come,
+ * first served).  Admitted requests are set up by host worker threads (DOF
+ * numbering, pattern, upload, device assembly) and then iterate on the device
+ * in hipGraph chunks of CG iterations; a scheduler thread interleaves the
+ * chunks of every running request, so requests that arrive while others are
+ * on the GPU join as slots and memory free up, and leave as they converge.
+ * Mesh, matrix and Krylov state stay on the device until the request ends;
+ * its device memory is then released and the solution kept on the host until
+ * femhip_serve_release.  A request's arithmetic does not depend on the other
+ * requests in flight: its result is the same alone and in a mix. */
+typedef struct {
+  double hbm_budget_bytes;  /* device memory admitted requests may hold together */
+  int32_t max_active;       /* slots (requests set up or iterating at once), >= 1 */
+  int32_t setup_threads;    /* host threads that set up and retire requests, >= 1 */
+  femhip_cg_params_t cg;    /* rtol, max_num_iterations, inner_rtol; FP32_REFINED, RES_RES/RES_RHS */
+} femhip_serve_params_t;
+
+typedef struct {
+  int32_t n;                /* cells per direction of the unit cube, 1..256 */
+  double E;                 /* Young's modulus (> 0) */
+  double nu;                /* Poisson ratio (-1, 0.5) */
+  double body[3];           /* body force */
+  double arrival_s;         /* arrival time in s since femhip_serve_create; < 0: now */
+} femhip_serve_request_t;
+
+typedef struct {
+  int32_t status;           /* FEMHIP_OK, FEMHIP_ERR_NOT_CONVERGED or the setup error */
+  int32_t num_iterations;   /* FP32 CG iterations */
+  int32_t outer_iterations; /* reliable updates */
+  int32_t did_converge;
+  int32_t num_dofs;         /* free DOFs */
+  int32_t pad;
+  double true_residual;     /* ||b - A x||_2 in FP64 at exit */
+  double rhs_norm;          /* ||b||_2 */
+  double queue_ms;          /* arrival -> admitted */
+  double setup_ms;          /* admitted -> first CG chunk queued (numbering, pattern, assembly) */
+  double solve_ms;          /* first chunk -> solution on the host */
+  double latency_ms;        /* arrival -> solution on the host */
+  double hbm_bytes;         /* device memory the request held */
+  double hbm_estimate;      /* the admission estimate */
+} femhip_serve_result_t;
+
+typedef struct {
+  int64_t submitted, completed, failed;
+  int32_t queued, active, peak_active, running_peak;  /* running_peak: most requests iterating at once */
+  double hbm_in_use, hbm_peak;   /* admitted estimates (the budget's currency) */
+} femhip_serve_stats_t;
+
+typedef struct femhip_server femhip_server_t;
+
+int femhip_serve_create(femhip_ctx_t *ctx, const femhip_serve_params_t *params, femhip_server_t **server);
+/* Device memory the server would reserve for a request of n cells per
+ * direction (what admission compares with the budget). */
+double femhip_serve_estimate_bytes(int32_t n, int32_t max_num_iterations);
+/* Queue a request; FEMHIP_ERR_ADMISSION if its estimate exceeds the whole
+ * budget (it could never be admitted), FEMHIP_ERR_INVALID for bad fields. */
+int femhip_serve_submit(femhip_server_t *server, const femhip_serve_request_t *request, int64_t *ticket);
+/* Wait up to timeout_s (< 0: forever) for a request; FEMHIP_OK with the
+ * result, or FEMHIP_ERR_STATE if it has not finished in time. */
+int femhip_serve_wait(femhip_server_t *server, int64_t ticket, double timeout_s, femhip_serve_result_t *result);
+/* Solution of a finished request (num_dofs values, numbering of the space). */
+int femhip_serve_get_solution(femhip_server_t *server, int64_t ticket, double *x);
+/* Forget a finished request (its host solution). */
+int femhip_serve_release(femhip_server_t *server, int64_t ticket);
+int femhip_serve_stats(femhip_server_t *server, femhip_serve_stats_t *stats);
+/* Seconds since femhip_serve_create on the server's clock (arrival times). */
+double femhip_serve_now(femhip_server_t *server);
+/* Stops admitting, lets admitted requests finish, drops queued ones. */
+int femhip_serve_destroy(femhip_server_t *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "femhip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char|void)\s*\**\s*(femhip_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char|void|double)\s*\**\s*(femhip_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_entry_points():
@@ -48,3 +48,18 @@ def test_invalid_arguments_are_rejected_without_device():
     lib = N.load()
     assert lib.femhip_space_info(None, None, None, None, None, None) == N.ERR_INVALID
     assert b"null" in lib.femhip_last_error()
+    assert lib.femhip_serve_submit(None, None, None) == N.ERR_INVALID
+
+
+def test_serve_admission_estimate_bounds_the_stored_data():
+    """The admission estimate of an elasticity request covers at least the CSR
+    (12 B per stored entry, 81 entries per interior row of vector Q1) plus the
+    FP64 and FP32 sliced copies, and grows with n."""
+    from fempar_amd import _native as N
+    lib = N.load()
+    est = [lib.femhip_serve_estimate_bytes(n, 20000) for n in (1, 2, 8, 32, 96)]
+    assert all(a < b for a, b in zip(est, est[1:]))
+    for n, e in zip((8, 32, 96), est[2:]):
+        ndof = 3 * n * (n + 1) ** 2
+        assert e >= ndof * 81 * (12 + 12)
+    assert est[-1] < 16e9   # the largest serving request (n = 96) stays well inside one GPU

@@ -1,0 +1,118 @@
+"""Serving form: elasticity requests through the continuous-batching server.
+
+Gates: each solution within 1e-6 relative of the oracle's FP64 CG solution
+(to 1e-12) and its FP64 true residual <= rtol ||b|| on the oracle's matrix
+(vector Q1, clamped on x = 0, body force (0, 0, -1); the oracle restates
+sbm_irreducible_discrete_integration.i90:64-130 and cg.f90:101-407); every
+result bit-identical alone and in a mix of requests in flight together;
+admission never exceeds the HBM budget and refuses a request larger than it."""
+import numpy as np
+import pytest
+
+import fempar_amd as F
+from fempar_amd import _native as N
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-8
+
+
+def _oracle(n, E, nu):
+    sp = O.Space(O.make_problem(3, (n, n, n), 1, 3, O.BC_X0))
+    ph = O.make_physics(kind=O.PHYS_ELASTICITY, E=E, nu=nu, body=(0, 0, -1))
+    return sp.assemble(ph, False)
+
+
+def test_serve_matches_oracle(env):
+    srv = F.serve_t(env, max_active=4, setup_threads=2, rtol=RTOL)
+    try:
+        cases = [(2, 1.0, 0.3), (3, 50.0, 0.2), (4, 120.0, 0.45), (5, 7.5, 0.33)]
+        tickets = [srv.submit(n, E, nu) for n, E, nu in cases]
+        for t, (n, E, nu) in zip(tickets, cases):
+            print(f"[serve] waiting for n={n} E={E} nu={nu}", flush=True)
+            res = srv.wait(t, timeout_s=60)
+            assert res["status"] == N.OK and res["did_converge"] == 1, res
+            irp, ja, val, rhs = _oracle(n, E, nu)
+            assert res["num_dofs"] == len(rhs) == 3 * n * (n + 1) ** 2
+            x = srv.get_solution(t)
+            r = rhs - O.spmv(irp, ja, val, x, False)
+            assert np.linalg.norm(r) <= 1.0001 * RTOL * np.linalg.norm(rhs), (n, np.linalg.norm(r))
+            assert abs(res["true_residual"] - np.linalg.norm(r)) <= 1e-6 * np.linalg.norm(rhs)
+            x_ref, info = O.cg(irp, ja, val, rhs, False, precond="jacobi", rtol=1e-12, maxit=50000)
+            assert info["converged"]
+            assert np.linalg.norm(x - x_ref) <= 1e-6 * np.linalg.norm(x_ref), n
+            assert 0 < res["hbm_bytes"] <= res["hbm_estimate"]
+            assert res["latency_ms"] >= res["solve_ms"] > 0
+            srv.release(t)
+    finally:
+        srv.free()
+
+
+def test_serve_alone_vs_mixed(env):
+    cases = [(3, 10.0, 0.25), (12, 80.0, 0.3), (6, 2.0, 0.4), (16, 150.0, 0.22), (4, 33.0, 0.44),
+             (9, 1.5, 0.35), (14, 60.0, 0.28), (5, 190.0, 0.21)]
+    srv = F.serve_t(env, max_active=8, setup_threads=4, rtol=RTOL)
+    try:
+        alone = []
+        for n, E, nu in cases:
+            t = srv.submit(n, E, nu)
+            res = srv.wait(t, timeout_s=60)
+            assert res["did_converge"] == 1, (n, res)
+            alone.append((res, srv.get_solution(t)))
+            srv.release(t)
+        print("[serve] alone done; submitting the mix", flush=True)
+        tickets = [srv.submit(n, E, nu) for n, E, nu in cases]
+        for t, (n, E, nu), (ra, xa) in zip(tickets, cases, alone):
+            res = srv.wait(t, timeout_s=120)
+            x = srv.get_solution(t)
+            assert res["num_iterations"] == ra["num_iterations"], n
+            assert np.array_equal(x, xa), n          # same bits alone and in the mix
+            srv.release(t)
+        st = srv.stats()
+        print(f"[serve] stats {st}", flush=True)
+        assert st["completed"] == 2 * len(cases) and st["failed"] == 0
+        assert st["running_peak"] >= 2              # requests iterated together
+    finally:
+        srv.free()
+
+
+def test_serve_admission_budget(env):
+    probe = F.serve_t(env, max_active=1, setup_threads=1)
+    est4, est8 = probe.estimate_bytes(4), probe.estimate_bytes(8)
+    probe.free()
+    budget = 1.5 * est4                              # room for one n = 4 request at a time
+    srv = F.serve_t(env, hbm_budget_bytes=budget, max_active=4, setup_threads=2, rtol=RTOL)
+    try:
+        with pytest.raises(N.FemhipError) as ei:
+            srv.submit(8, 10.0, 0.3)
+        assert ei.value.code == N.ERR_ADMISSION and est8 > budget
+        tickets = [srv.submit(4, 10.0 + i, 0.3) for i in range(3)]
+        for t in tickets:
+            assert srv.wait(t, timeout_s=60)["did_converge"] == 1
+            srv.release(t)
+        st = srv.stats()
+        assert st["peak_active"] == 1 and st["hbm_peak"] <= budget
+        assert st["hbm_in_use"] == 0 and st["active"] == 0
+    finally:
+        srv.free()
+
+
+def test_serve_errors(env):
+    srv = F.serve_t(env, max_active=2, setup_threads=1)
+    try:
+        for bad in [dict(n=0, E=1.0, nu=0.3), dict(n=4, E=-1.0, nu=0.3), dict(n=4, E=1.0, nu=0.5)]:
+            with pytest.raises(N.FemhipError) as ei:
+                srv.submit(**bad)
+            assert ei.value.code == N.ERR_INVALID
+        with pytest.raises(N.FemhipError):
+            srv.wait(12345, timeout_s=0.1)
+        # a future arrival is not admitted before its time
+        t = srv.submit(2, 1.0, 0.3, arrival_s=srv.now() + 0.5)
+        with pytest.raises(N.FemhipError) as ei:
+            srv.wait(t, timeout_s=0.1)
+        assert ei.value.code == N.ERR_STATE
+        res = srv.wait(t, timeout_s=30)
+        assert res["did_converge"] == 1 and res["queue_ms"] < 100.0
+    finally:
+        srv.free()
