@@ -31,9 +31,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <tuple>
+#include <type_traits>
 
 #include "femhip_internal.h"
 
@@ -849,10 +852,40 @@ void prepare(femhip_solver &s) {
   }
 }
 
+// Launches a kernel on a stream, or appends it as a kernel node after the
+// previous node of a graph under construction.  Graphs are built node by node
+// rather than by stream capture: a capture in one thread makes synchronous
+// legacy-stream calls of other threads (and the capture itself) fail, and the
+// request server sets up and retires requests on several threads at once.
+struct Launcher {
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphNode_t last = nullptr;
+
+  template <typename... P, typename... A>
+  void operator()(void (*k)(P...), dim3 grid, dim3 block, A... a) {
+    std::tuple<std::decay_t<P>...> vals(static_cast<std::decay_t<P>>(a)...);
+    auto args = std::apply([](auto &...v) { return std::array<void *, sizeof...(P)>{(void *)&v...}; }, vals);
+    if (!graph) {
+      FH_HIP(hipLaunchKernel((const void *)k, grid, block, args.data(), 0, stream));
+      return;
+    }
+    hipKernelNodeParams prm{};
+    prm.func = (void *)k;
+    prm.gridDim = grid;
+    prm.blockDim = block;
+    prm.sharedMemBytes = 0;
+    prm.kernelParams = args.data();
+    prm.extra = nullptr;
+    hipGraphNode_t node;
+    FH_HIP(hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &prm));
+    last = node;
+  }
+};
+
 // Enqueue `chunk` iterations (chunk even: iteration parity restarts at 0).
-void enqueue_iterations(femhip_solver &s, int chunk) {
+void enqueue_iterations(femhip_solver &s, int chunk, Launcher &L) {
   femhip_operator &op = *s.op;
-  hipStream_t st = s.ctx->stream;
   const SlicedDev m = sliced_dev(op);
   const int32_t n = s.n;
   const bool jac = s.prm.preconditioner == FEMHIP_PREC_JACOBI;
@@ -861,19 +894,17 @@ void enqueue_iterations(femhip_solver &s, int chunk) {
     const int par = i & 1;
     if (!mixed(s)) {
       const double *invd = jac ? s.invd.p : nullptr;
-      hipLaunchKernelGGL((k_spmv<double, double>), GM, B, 0, st, m, op.sl.v64.p, s.p.p, s.q.p, s.st.p, par,
-                         s.part_s.p);
-      hipLaunchKernelGGL((k_update<double, double>), GV, B, 0, st, n, s.x.p, s.r.p, s.p.p, s.q.p, invd, s.st.p, par,
-                         s.part_s.p, s.g_mat, s.part_u.p);
-      hipLaunchKernelGGL((k_dir<double>), GV, B, 0, st, n, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec,
-                         s.hist.p, s.res.p);
+      L(k_spmv<double, double>, GM, B, m, op.sl.v64.p, s.p.p, s.q.p, s.st.p, par, s.part_s.p);
+      L(k_update<double, double>, GV, B, n, s.x.p, s.r.p, s.p.p, s.q.p, invd, s.st.p, par, s.part_s.p, s.g_mat,
+        s.part_u.p);
+      L(k_dir<double>, GV, B, n, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.hist.p, s.res.p);
     } else {
       const float *invd = jac ? s.invd32.p : nullptr;
-      hipLaunchKernelGGL(k_ru_spmv, GM, B, 0, st, m, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.d32.p,
-                         s.r32.p, invd, s.st.p, par, s.part_s.p);
-      hipLaunchKernelGGL(k_ru_update, GV, B, 0, st, n, s.x.p, s.d32.p, s.r32.p, s.p32.p, s.q32.p, invd, s.st.p, par,
-                         s.part_s.p, s.g_mat, s.part_u.p);
-      hipLaunchKernelGGL(k_ru_dir, GV, B, 0, st, n, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.res.p);
+      L(k_ru_spmv, GM, B, m, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.d32.p, s.r32.p, invd,
+        s.st.p, par, s.part_s.p);
+      L(k_ru_update, GV, B, n, s.x.p, s.d32.p, s.r32.p, s.p32.p, s.q32.p, invd, s.st.p, par, s.part_s.p, s.g_mat,
+        s.part_u.p);
+      L(k_ru_dir, GV, B, n, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.res.p);
     }
   }
 }
@@ -885,26 +916,30 @@ bool use_graph() {
   return !no_graph;
 }
 
-// capture (once per precision) the chunk of kChunk iterations as a hipGraph
+// build (once per precision) the chunk of kChunk iterations as a hipGraph
 void ensure_graph(femhip_solver &s) {
   const int kind = mixed(s) ? 1 : 0;
   if (!use_graph() || (s.graph && s.graph_kind == kind)) return;
   if (s.graph) FH_HIP(hipGraphExecDestroy(s.graph));
   s.graph = nullptr;
-  hipStream_t st = s.ctx->stream;
-  hipGraph_t g;
-  FH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  enqueue_iterations(s, kChunk);
-  FH_HIP(hipStreamEndCapture(st, &g));
-  FH_HIP(hipGraphInstantiate(&s.graph, g, nullptr, nullptr, 0));
-  FH_HIP(hipGraphDestroy(g));
+  Launcher L;
+  FH_HIP(hipGraphCreate(&L.graph, 0));
+  try {
+    enqueue_iterations(s, kChunk, L);
+    FH_HIP(hipGraphInstantiate(&s.graph, L.graph, nullptr, nullptr, 0));
+  } catch (...) {
+    (void)hipGraphDestroy(L.graph);
+    throw;
+  }
+  FH_HIP(hipGraphDestroy(L.graph));
   s.graph_kind = kind;
 }
 
 void launch_chunk(femhip_solver &s) {
   if (!use_graph()) {
-    enqueue_iterations(s, kChunk);
-    FH_HIP(hipGetLastError());
+    Launcher L;
+    L.stream = s.ctx->stream;
+    enqueue_iterations(s, kChunk, L);
     return;
   }
   ensure_graph(s);

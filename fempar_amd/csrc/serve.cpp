@@ -49,6 +49,7 @@ struct Req {
   double t_admit = 0.0, t_run = 0.0, t_done = 0.0;
   femhip_serve_result_t res{};
   std::vector<double> x;
+  std::string error;   // message of a failed setup or retire
 };
 using ReqP = std::shared_ptr<Req>;
 
@@ -196,19 +197,30 @@ void worker_loop(femhip_server &S) {
     }
     int status = FEMHIP_OK;
     std::string msg;
+    {
+      std::lock_guard<std::mutex> lk(S.mu);
+      if (retire && !r->error.empty()) {   // the scheduler's launch or poll failed
+        status = r->res.status;
+        msg = r->error;
+      }
+    }
     try {
-      if (retire) retire_request(S, *r);
-      else setup_request(S, *r);
+      if (status == FEMHIP_OK) {
+        if (retire) retire_request(S, *r);
+        else setup_request(S, *r);
+      }
     } catch (const Error &e) {
       status = e.code;
       msg = e.msg;
     } catch (...) {
       status = FEMHIP_ERR_OOM;
+      msg = "host allocation failed";
     }
     if (retire || status != FEMHIP_OK) {
       if (status != FEMHIP_OK && r->solver) (void)hipStreamSynchronize(S.slot_ctx[(size_t)r->slot]->stream);
       free_request_objects(*r);
       std::lock_guard<std::mutex> lk(S.mu);
+      if (status != FEMHIP_OK && r->error.empty()) r->error = std::string(retire ? "retire: " : "setup: ") + msg;
       finish_locked(S, *r, status);
     } else {
       std::lock_guard<std::mutex> lk(S.mu);
@@ -271,8 +283,9 @@ void scheduler_loop(femhip_server &S) {
         r->t_run = S.now();
         running.push_back(r);
       } catch (const Error &e) {
-        r->res.status = e.code;
         std::lock_guard<std::mutex> lk(S.mu);
+        r->res.status = e.code;
+        r->error = "launch: " + e.msg;
         r->state = RETIRING;
         S.retire_q.push_back(r);
         S.cv_work.notify_one();
@@ -288,8 +301,11 @@ void scheduler_loop(femhip_server &S) {
       int st = 0;
       try {
         st = solver_poll(*r.solver, r.cur);
-      } catch (const Error &) {
-        st = 2;   // the retire step reports the error
+      } catch (const Error &e) {
+        std::lock_guard<std::mutex> lk(S.mu);
+        r.res.status = e.code;
+        r.error = "poll: " + e.msg;
+        st = 2;
       }
       if (st == 0) {
         ++i;
@@ -309,8 +325,10 @@ void scheduler_loop(femhip_server &S) {
         solver_push(*r.solver, r.cur);   // the slot's previous result has been read
         r.queued_its += chunk;
         r.cur ^= 1;
-      } catch (const Error &) {
+      } catch (const Error &e) {
         std::lock_guard<std::mutex> lk(S.mu);
+        r.res.status = e.code;
+        r.error = "launch: " + e.msg;
         r.state = RETIRING;
         S.retire_q.push_back(running[i]);
         S.cv_work.notify_one();
@@ -413,6 +431,7 @@ int femhip_serve_wait(femhip_server_t *S, int64_t ticket, double timeout_s, femh
     else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done))
       throw Error{FEMHIP_ERR_STATE, "request not finished within the timeout"};
     *result = r->res;
+    set_error(r->error);   // femhip_last_error() names a failed request's cause
     return FEMHIP_OK;
   } catch (const Error &e) {
     set_error(e.msg);
@@ -427,7 +446,7 @@ int femhip_serve_get_solution(femhip_server_t *S, int64_t ticket, double *x) {
     auto it = S->reqs.find(ticket);
     FH_CHECK(it != S->reqs.end(), FEMHIP_ERR_INVALID, "unknown or released ticket");
     FH_CHECK(it->second->state == DONE, FEMHIP_ERR_STATE, "request not finished");
-    FH_CHECK(!it->second->x.empty(), FEMHIP_ERR_STATE, "request failed: no solution");
+    FH_CHECK(!it->second->x.empty(), FEMHIP_ERR_STATE, "request failed, no solution: " + it->second->error);
     std::memcpy(x, it->second->x.data(), sizeof(double) * it->second->x.size());
     return FEMHIP_OK;
   } catch (const Error &e) {

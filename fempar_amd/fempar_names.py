@@ -504,6 +504,7 @@ class serve_t:
         r = N.ServeResult()
         N.check(self._lib.femhip_serve_wait(self.h, ticket, float(timeout_s), C.byref(r)))
         out = {f: getattr(r, f) for f, _ in N.ServeResult._fields_ if f != "pad"}
+        out["error"] = self._lib.femhip_last_error().decode(errors="replace") if r.status not in (N.OK, 4) else ""
         self._dofs[ticket] = r.num_dofs
         return out
 

@@ -65,6 +65,7 @@ def test_serve_alone_vs_mixed(env):
         tickets = [srv.submit(n, E, nu) for n, E, nu in cases]
         for t, (n, E, nu), (ra, xa) in zip(tickets, cases, alone):
             res = srv.wait(t, timeout_s=120)
+            assert res["status"] == N.OK, (n, res["status"], res["error"])
             x = srv.get_solution(t)
             assert res["num_iterations"] == ra["num_iterations"], n
             assert np.array_equal(x, xa), n          # same bits alone and in the mix
