@@ -58,6 +58,56 @@ def run_steps(op, solver, steps):
     return np.array(t_asm), np.array(t_cg), np.array(wall), last
 
 
+def grf_k_cell(n, sigma=1.5, ell=0.1, seed=20261202):
+    """k per cell = exp(mean of g over the cell's 8 vertices), g a Gaussian
+    random field on the (n+1)^3 vertex grid of the unit cube: white noise from
+    default_rng(seed) filtered in Fourier space by the square root of the
+    squared-exponential spectrum (correlation length ell), normalised to mean 0
+    and standard deviation sigma.  Array axes are (z, y, x), so the flattened
+    cell array is x-fastest like the mesh's cells."""
+    m = n + 1
+    w = np.random.default_rng(seed).standard_normal((m, m, m))
+    kf = 2 * np.pi * np.fft.fftfreq(m, d=1.0 / n)
+    kr = 2 * np.pi * np.fft.rfftfreq(m, d=1.0 / n)
+    k2 = kf[:, None, None] ** 2 + kf[None, :, None] ** 2 + kr[None, None, :] ** 2
+    g = np.fft.irfftn(np.fft.rfftn(w) * np.exp(-(ell ** 2) * k2 / 4.0), s=w.shape, axes=(0, 1, 2))
+    g = (g - g.mean()) / g.std() * sigma
+    c = sum(g[a:m - 1 + a, b:m - 1 + b, d:m - 1 + d] for a in (0, 1) for b in (0, 1) for d in (0, 1)) / 8.0
+    return np.exp(c).ravel()
+
+
+def measure_config(F, env, n, order, di, rtol, steps, warmup, with_mixed=True):
+    """Assembly + Jacobi-PCG DOFs/s of one configuration in FP64 and FP32-refined."""
+    tria = F.serial_triangulation_t().create(env, {
+        "STRUCT_HEX_MESH_GENERATOR_NUM_DIMS": 3, "STRUCT_HEX_MESH_GENERATOR_NUM_CELLS_X_DIM": [n, n, n],
+        "STRUCT_HEX_MESH_GENERATOR_DOMAIN_LIMITS": [0, 1, 0, 1, 0, 1]})
+    t0 = time.perf_counter()
+    fes = F.serial_fe_space_t().create(tria, order=order)
+    op = F.fe_affine_operator_t()
+    op.create("CSR", [True], fes, di)
+    out = {"n_free_dofs": fes.num_free_dofs, "nnz_full": op.nnz_full, "setup_s": time.perf_counter() - t0}
+    xs, solvers = {}, []
+    for prec in ("FP64", "FP32_REFINED") if with_mixed else ("FP64",):
+        s = make_solver(F, env, op, prec, rtol)
+        solvers.append(s)
+        for _ in range(warmup):
+            run_steps(op, s, 1)
+        ta, tc, wall, r = run_steps(op, s, steps)
+        xs[prec] = s.get_solution()
+        out[prec] = {"value": fes.num_free_dofs / float(np.mean(ta + tc)), "t_assemble_ms": float(np.mean(ta) * 1e3),
+                     "t_cg_ms": float(np.mean(tc) * 1e3), "iterations": r["num_iterations"],
+                     "outer_iterations": r["outer_iterations"], "true_rel_residual": r["true_residual"] / r["rhs_norm"]}
+    if with_mixed:
+        out["speedup_mixed_vs_fp64"] = out["FP32_REFINED"]["value"] / out["FP64"]["value"]
+        out["rel_err_mixed_vs_fp64"] = float(np.linalg.norm(xs["FP32_REFINED"] - xs["FP64"]) /
+                                             np.linalg.norm(xs["FP64"]))
+    for s in solvers:
+        s.free()
+    op.free()
+    fes.free()
+    return out
+
+
 def cpu_baseline(n, iters_full, op_matrix=None, rhs=None):
     """Oracle (CPU restatement of the reference algorithms), 1 thread: assembly
     of a z-slab of 1/8 of the cells scaled x8, plus 10 Jacobi-PCG iterations on
@@ -174,6 +224,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-q2", action="store_true", help="skip the 64^3 Q2 configuration")
+    ap.add_argument("--no-grf", action="store_true", help="skip the 192^3 random-field mixed-precision configuration")
+    ap.add_argument("--configs-only", action="store_true", help="only the Q2 and random-field configurations")
     ap.add_argument("--serve-requests", type=int, default=32, help="requests per arrival rate")
     ap.add_argument("--serving-only", action="store_true", help="only the serving sweep (printed as JSON)")
     ap.add_argument("--spmv-only", action="store_true",
@@ -202,6 +255,27 @@ def main():
     n = args.n
     if args.serving_only:
         print(json.dumps(run_serving(F, env, args.serve_requests)))
+        return
+
+    def other_configs():
+        res = {}
+        if not args.no_q2:
+            di = F.poisson_discrete_integration_t(k="sin3", f="manufactured")
+            res["poisson3d_q2_64"] = measure_config(F, env, 64, 2, di, args.rtol, args.steps, args.warmup)
+            print(f"[bench] q2 64^3: {res['poisson3d_q2_64']}", file=sys.stderr, flush=True)
+        if not args.no_grf:
+            t0 = time.perf_counter()
+            kc = grf_k_cell(192)
+            di = F.poisson_discrete_integration_t(k="cell", k_cell=kc, f="const", f0=1.0)
+            r = measure_config(F, env, 192, 1, di, args.rtol, 1, 1)
+            r["k_contrast"] = float(kc.max() / kc.min())
+            r["field_s"] = time.perf_counter() - t0 - r["setup_s"]
+            res["poisson3d_q1_192_grf"] = r
+            print(f"[bench] grf 192^3: {r}", file=sys.stderr, flush=True)
+        return res
+
+    if args.configs_only:
+        print(json.dumps(other_configs()))
         return
     if args.spmv_only:
         tria, fes, di, op = make_problem(F, env, n)
@@ -252,6 +326,7 @@ def main():
 
     batched = None if args.no_batched else run_batched(F, env, args.steps)
     serving = None if args.no_serving else run_serving(F, env, args.serve_requests)
+    configs = other_configs()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -276,6 +351,7 @@ def main():
         "mixed": mixed,
         "batched": batched,
         "serving": serving,
+        "configs": configs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
