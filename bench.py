@@ -76,7 +76,7 @@ def grf_k_cell(n, sigma=1.5, ell=0.1, seed=20261202):
     return np.exp(c).ravel()
 
 
-def measure_config(F, env, n, order, di, rtol, steps, warmup, with_mixed=True):
+def measure_config(F, env, n, order, di, rtol, steps, warmup, with_mixed=True, exact=None):
     """Assembly + Jacobi-PCG DOFs/s of one configuration in FP64 and FP32-refined."""
     tria = F.serial_triangulation_t().create(env, {
         "STRUCT_HEX_MESH_GENERATOR_NUM_DIMS": 3, "STRUCT_HEX_MESH_GENERATOR_NUM_CELLS_X_DIM": [n, n, n],
@@ -97,6 +97,8 @@ def measure_config(F, env, n, order, di, rtol, steps, warmup, with_mixed=True):
         out[prec] = {"value": fes.num_free_dofs / float(np.mean(ta + tc)), "t_assemble_ms": float(np.mean(ta) * 1e3),
                      "t_cg_ms": float(np.mean(tc) * 1e3), "iterations": r["num_iterations"],
                      "outer_iterations": r["outer_iterations"], "true_rel_residual": r["true_residual"] / r["rhs_norm"]}
+        if exact:
+            out[prec]["error_vs_u"] = s.error_norms(exact)
     if with_mixed:
         out["speedup_mixed_vs_fp64"] = out["FP32_REFINED"]["value"] / out["FP64"]["value"]
         out["rel_err_mixed_vs_fp64"] = float(np.linalg.norm(xs["FP32_REFINED"] - xs["FP64"]) /
@@ -261,7 +263,8 @@ def main():
         res = {}
         if not args.no_q2:
             di = F.poisson_discrete_integration_t(k="sin3", f="manufactured")
-            res["poisson3d_q2_64"] = measure_config(F, env, 64, 2, di, args.rtol, args.steps, args.warmup)
+            res["poisson3d_q2_64"] = measure_config(F, env, 64, 2, di, args.rtol, args.steps, args.warmup,
+                                                    exact="sin")
             print(f"[bench] q2 64^3: {res['poisson3d_q2_64']}", file=sys.stderr, flush=True)
         if not args.no_grf:
             t0 = time.perf_counter()
@@ -294,6 +297,7 @@ def main():
         run_steps(op, s64, 1)
     ta, tc, wall, r64 = run_steps(op, s64, args.steps)
     x64 = s64.get_solution()
+    err64 = s64.error_norms("sin")   # vs the manufactured u = sin sin sin
     ms_step = float(np.mean(wall) * 1e3)
     value = fes.num_free_dofs / float(np.mean(ta + tc))
 
@@ -322,6 +326,7 @@ def main():
             "rel_err_vs_fp64": float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64)),
             "true_rel_residual": r32["true_residual"] / r32["rhs_norm"],
             "spmv_fp32_GBs": sp32_bytes / (sp32_ms * 1e-3) / 1e9,
+            "error_vs_u": s32.error_norms("sin"),
         }
 
     batched = None if args.no_batched else run_batched(F, env, args.steps)
@@ -343,6 +348,7 @@ def main():
                    "cells": fes.num_cells},
         "t_assemble_ms": float(np.mean(ta) * 1e3), "t_cg_ms": float(np.mean(tc) * 1e3),
         "cg_iterations": r64["num_iterations"], "true_rel_residual": r64["true_residual"] / r64["rhs_norm"],
+        "error_vs_u": err64,
         "setup_s": t_setup,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

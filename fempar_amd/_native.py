@@ -36,7 +36,7 @@ EXPORTS = [
     "femhip_operator_set_physics", "femhip_operator_compute", "femhip_operator_get_values",
     "femhip_operator_apply", "femhip_operator_destroy",
     "femhip_solver_create", "femhip_solver_solve", "femhip_solver_get_solution", "femhip_solver_destroy",
-    "femhip_solver_profile_spmv",
+    "femhip_solver_profile_spmv", "femhip_solver_error_norms",
     "femhip_batch_create", "femhip_batch_assemble", "femhip_batch_solve", "femhip_batch_get_solution",
     "femhip_batch_get_problem_matrix", "femhip_batch_autotune", "femhip_batch_destroy",
     "femhip_serve_create", "femhip_serve_estimate_bytes", "femhip_serve_submit", "femhip_serve_wait",
@@ -82,6 +82,14 @@ class CgResult(C.Structure):
     _fields_ = [("num_iterations", C.c_int32), ("did_converge", C.c_int32), ("outer_iterations", C.c_int32),
                 ("status", C.c_int32), ("error_estimate", C.c_double), ("rhs_convergence_test", C.c_double),
                 ("true_residual", C.c_double), ("rhs_norm", C.c_double), ("solve_ms", C.c_double)]
+
+
+class ErrorNorms(C.Structure):
+    _fields_ = [("l2_norm", C.c_double), ("h1_seminorm", C.c_double), ("h1_norm", C.c_double),
+                ("linfty_norm", C.c_double)]
+
+
+EXACT_SIN, EXACT_LINEAR = 0, 1
 
 
 class ServeParams(C.Structure):
@@ -137,6 +145,7 @@ def load():
         "femhip_solver_solve": [P, P, P, C.POINTER(CgResult)], "femhip_solver_get_solution": [P, P],
         "femhip_solver_destroy": [P],
         "femhip_solver_profile_spmv": [P, I32, C.POINTER(D), C.POINTER(D)],
+        "femhip_solver_error_norms": [P, I32, C.POINTER(ErrorNorms)],
         "femhip_batch_create": [P, I32, C.POINTER(BatchProblem), C.POINTER(CgParams), pP],
         "femhip_batch_assemble": [P, C.POINTER(D)],
         "femhip_batch_solve": [P, C.POINTER(BatchResult), C.POINTER(D)],

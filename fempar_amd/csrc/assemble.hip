@@ -15,10 +15,16 @@
 // On a uniform mesh every cell is a translate of the first one, so the element
 // matrix factors as  elmat_c = sum_q k_c(x_q) * M_q  with M_q = detJ w_q G_q^T G_q
 // fixed per space (nq small nd x nd matrices), and elvec_c = sum_q f_c(x_q) Nw_q.
-// Cells are processed colour by colour (2^dim colours by cell-index parity):
+//
+// Scalar Q1 (the CG workloads) is assembled owner-computes: one thread per
+// free row gathers the element entries of the cells around its vertex and
+// writes each value of its row once.  Other spaces use a coloured scatter:
+// cells are processed colour by colour (2^dim colours by cell-index parity),
 // cells of one colour share no DOF, so every (row, col) slot is updated by at
-// most one thread per launch -- no atomics, and the result is deterministic.
+// most one thread per launch.  Both are free of atomics and deterministic.
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 #include "femhip_internal.h"
 
@@ -213,6 +219,189 @@ __global__ __launch_bounds__(CPB *ND) void k_assemble_const(
   rhs[gi] += bacc;
 }
 
+// ---------------------------------------------- owner-computes scalar Q1 ---
+// Every free DOF of a scalar Q1 space sits on one vertex, and its CSR row holds
+// the free DOFs of the 3^d vertices around it.  One thread per row sums, over
+// the 2^d cells around its vertex in ascending cell order (the order in which
+// the reference's COO append + stable sort sums duplicates), the element
+// entries coupling it to each neighbour, and writes every value of the row
+// once: no colours, no search, no read-modify-write.
+
+struct VtxGrid {
+  int dim;
+  int n[3];    // cells per direction (1 beyond dim)
+  int nv[3];   // vertices per direction (1 beyond dim)
+};
+
+__device__ __forceinline__ void vtx_coords(const VtxGrid &g, int32_t v, int a[3]) {
+  a[0] = v % g.nv[0];
+  a[1] = (v / g.nv[0]) % g.nv[1];
+  a[2] = v / (g.nv[0] * g.nv[1]);
+}
+
+// vertex -> signed DOF id and free row -> vertex, from the cell DOF lists
+template <int DIM>
+__global__ void k_gather_vertices(VtxGrid g, int64_t ncells, const int32_t *__restrict__ dofs,
+                                  int32_t *__restrict__ vtx_dof, int32_t *__restrict__ row_vtx) {
+  constexpr int ND = 1 << DIM;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells * ND) return;
+  const int64_t cell = t / ND;
+  const int l = (int)(t % ND);
+  const int ci = (int)(cell % g.n[0]), cj = (int)((cell / g.n[0]) % g.n[1]), ck = (int)(cell / ((int64_t)g.n[0] * g.n[1]));
+  const int a = ci + (l & 1), b = cj + ((l >> 1) & 1), c = DIM > 2 ? ck + ((l >> 2) & 1) : 0;
+  const int32_t v = a + g.nv[0] * (b + g.nv[1] * c);
+  const int32_t id = dofs[t];
+  vtx_dof[v] = id;             // every cell around v writes the same id
+  if (id >= 0) row_vtx[id] = v;
+}
+
+// CSR slot of each neighbour offset code (dx+1) + 3(dy+1) + 9(dz+1) of a row;
+// checks that the row holds exactly the free neighbours, in ascending order
+template <int DIM>
+__global__ void k_gather_slots(VtxGrid g, int32_t n, const int32_t *__restrict__ vtx_dof,
+                               const int32_t *__restrict__ row_vtx, const int32_t *__restrict__ rowptr,
+                               const int32_t *__restrict__ col, uint8_t *__restrict__ slot, int *__restrict__ bad) {
+  constexpr int K = DIM == 2 ? 9 : 27;
+  constexpr int KS = DIM == 2 ? 16 : 32;   // slot row stride (bytes)
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int a[3];
+  vtx_coords(g, row_vtx[r], a);
+  int32_t ids[K];
+  int cnt = 0;
+  for (int k = 0; k < K; ++k) {
+    const int d[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
+    bool in = true;
+    for (int e = 0; e < DIM; ++e) in = in && a[e] + d[e] >= 0 && a[e] + d[e] < g.nv[e];
+    ids[k] = in ? vtx_dof[(a[0] + d[0]) + g.nv[0] * ((a[1] + d[1]) + g.nv[1] * (DIM > 2 ? a[2] + d[2] : 0))] : -1;
+    cnt += ids[k] >= 0;
+  }
+  const int32_t b = rowptr[r], len = rowptr[r + 1] - b;
+  bool ok = cnt == len;
+  for (int k = 0; k < KS; ++k) {
+    uint8_t s = 255;
+    if (k < K && ids[k] >= 0) {
+      int rank = 0;
+      for (int k2 = 0; k2 < K; ++k2) rank += ids[k2] >= 0 && ids[k2] < ids[k];
+      ok = ok && rank < len && col[b + rank] == ids[k];   // col read only inside the row
+      s = (uint8_t)rank;
+    }
+    slot[(int64_t)r * KS + k] = s;
+  }
+  if (!ok) atomicOr(bad, 1);
+}
+
+// On a uniform mesh the d-th coordinate of quadrature point q of cell (i,j,k)
+// depends only on (i_d, the d-th 1-D Gauss index of q), so the trigonometric
+// factors of the coefficient and source are tabulated per direction:
+// trig[off[d] + i_d * 2 + a] = {sin(pi x), cos(pi x), sin(2 pi x), cos(2 pi x)}.
+struct TrigTab {
+  const double4 *t;
+  int off[3];
+};
+
+// k and f at a quadrature point from the tables: the same expressions as
+// k_at and f_at (the coefficient variant of poisson_discrete_integration.f90)
+template <int DIM>
+__device__ __forceinline__ void coef_tab(const PhysDev &ph, const TrigTab &tt, const int ijk[3], int q, int64_t cell,
+                                         double &k, double &f) {
+  double su[3] = {1, 1, 1}, cu[3] = {0, 0, 0}, s2[3] = {1, 1, 1}, c2[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const double4 v = tt.t[tt.off[d] + ijk[d] * 2 + ((q >> d) & 1)];
+    su[d] = v.x;
+    cu[d] = v.y;
+    s2[d] = v.z;
+    c2[d] = v.w;
+  }
+  if (ph.k_kind == FEMHIP_K_SIN3) k = 1.0 + 0.5 * (s2[0] * s2[1] * s2[2]);
+  else if (ph.k_kind == FEMHIP_K_CELL) k = ph.k_cell[cell];
+  else k = ph.k0;
+  if (ph.f_kind == FEMHIP_F_MANUFACTURED) {
+    f = k * DIM * kPi * kPi * (su[0] * su[1] * su[2]);
+    if (ph.k_kind == FEMHIP_K_SIN3) {
+      double dot = kPi * c2[0] * s2[1] * s2[2] * kPi * cu[0] * su[1] * su[2] +
+                   kPi * s2[0] * c2[1] * s2[2] * kPi * su[0] * cu[1] * su[2];
+      if (DIM == 3) dot += kPi * s2[0] * s2[1] * c2[2] * kPi * su[0] * su[1] * cu[2];
+      f -= dot;
+    }
+  } else if (ph.f_kind == FEMHIP_F_CELL_SIN) {
+    f = ph.f0 * k * DIM * kPi * kPi * (su[0] * su[1] * su[2]);
+  } else {
+    f = ph.f0;
+  }
+}
+
+// One thread per free row; a block owns RB consecutive rows, whose CSR values
+// are contiguous: they are staged in LDS and stored with coalesced writes.
+template <int DIM, int RB>
+__global__ __launch_bounds__(RB) void k_assemble_gather(
+    VtxGrid g, PhysDev ph, TrigTab tt, int32_t n, const int32_t *__restrict__ row_vtx,
+    const int32_t *__restrict__ vtx_dof, const uint8_t *__restrict__ slot, const int32_t *__restrict__ rowptr,
+    const double *__restrict__ tab, const double *__restrict__ fixed, double *__restrict__ val,
+    double *__restrict__ rhs) {
+  constexpr int ND = 1 << DIM, NQ = ND, K = DIM == 2 ? 9 : 27, KS = DIM == 2 ? 16 : 32;
+  __shared__ double sv[RB * K];
+  const double *M = tab;                 // M[NQ][ND][ND]
+  const double *Nw = tab + NQ * ND * ND; // Nw[NQ][ND]
+  const int32_t r0 = blockIdx.x * RB, r = r0 + threadIdx.x;
+  const int32_t b0 = rowptr[r0], b1 = rowptr[min(n, r0 + RB)];
+  if (r < n) {
+  int a[3];
+  vtx_coords(g, row_vtx[r], a);
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  double bacc = 0.0;
+#pragma unroll
+  for (int e = 0; e < ND; ++e) {          // cells around the vertex, ascending index
+    const int ex = e & 1, ey = (e >> 1) & 1, ez = (e >> 2) & 1;
+    const int ci = a[0] - 1 + ex, cj = a[1] - 1 + ey, ck = DIM > 2 ? a[2] - 1 + ez : 0;
+    if (ci < 0 || ci >= g.n[0] || cj < 0 || cj >= g.n[1] || (DIM > 2 && (ck < 0 || ck >= g.n[2]))) continue;
+    const int64_t cell = ci + (int64_t)g.n[0] * (cj + (int64_t)g.n[1] * ck);
+    const int ijk[3] = {ci, cj, ck};
+    const int il = (1 - ex) + 2 * (1 - ey) + (DIM > 2 ? 4 * (1 - ez) : 0);   // the vertex in this cell
+    double kv[NQ], fv[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) coef_tab<DIM>(ph, tt, ijk, q, cell, kv[q], fv[q]);
+    double be = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) be += fv[q] * Nw[q * ND + il];
+    bacc += be;
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+      const int dx = ex + (j & 1) - 1, dy = ey + ((j >> 1) & 1) - 1, dz = DIM > 2 ? ez + ((j >> 2) & 1) - 1 : -1;
+      const int code = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) v += kv[q] * M[(q * ND + il) * ND + j];
+      acc[code] += v;
+    }
+  }
+  const uint8_t *sr = slot + (int64_t)r * KS;
+  const int32_t base = rowptr[r] - b0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint8_t s = sr[k];
+    if (s != 255) {
+      sv[base + s] = acc[k];
+    } else if (fixed) {   // a fixed neighbour lifts its value into the RHS
+      const int d[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
+      bool in = true;
+      for (int e = 0; e < DIM; ++e) in = in && a[e] + d[e] >= 0 && a[e] + d[e] < g.nv[e];
+      if (in) {
+        const int32_t id = vtx_dof[(a[0] + d[0]) + g.nv[0] * ((a[1] + d[1]) + g.nv[1] * (DIM > 2 ? a[2] + d[2] : 0))];
+        if (id < 0) bacc -= acc[k] * fixed[-id - 1];
+      }
+    }
+  }
+  rhs[r] = bacc;
+  }
+  __syncthreads();
+  for (int32_t e = threadIdx.x; e < b1 - b0; e += RB) val[b0 + e] = sv[e];   // rows <= K entries each
+}
+
 __global__ void k_spmv_csr(int32_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                            const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y) {
   int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -255,6 +444,105 @@ void launch_const(femhip_operator &op, const MeshDev &m, const double *d_fixed) 
     hipLaunchKernelGGL((k_assemble_const<ND, CPB>), dim3((unsigned)blocks), dim3(CPB * ND), 0, op.ctx->stream, m,
                        color, cnt, s.d_dofs.p, op.rowptr.p, op.col.p, op.tab.p, d_fixed, op.val.p, op.rhs.p);
   }
+}
+
+VtxGrid vtx_grid(const femhip_mesh_desc_t &d) {
+  VtxGrid g{};
+  g.dim = d.dim;
+  for (int k = 0; k < 3; ++k) {
+    g.n[k] = k < d.dim ? d.n[k] : 1;
+    g.nv[k] = k < d.dim ? d.n[k] + 1 : 1;
+  }
+  return g;
+}
+
+// One-time set-up of the owner-computes path (scalar Q1 Poisson only); the
+// maps are validated against the CSR pattern, and any mismatch keeps the
+// operator on the coloured scatter.
+void setup_gather(femhip_operator &op) {
+  femhip_space &s = *op.space;
+  const RefFE &fe = s.fe;
+  const int dim = fe.dim;
+  static const bool off = getenv("FEMHIP_NO_GATHER") != nullptr;
+  if (off || op.tab_kind != 0 || fe.order != 1 || fe.ncomp != 1 || fe.nd != (1 << dim) || op.n <= 0) {
+    op.gather = -1;
+    return;
+  }
+  hipStream_t st = op.ctx->stream;
+  const VtxGrid g = vtx_grid(s.mesh);
+  const int64_t nverts = (int64_t)g.nv[0] * g.nv[1] * g.nv[2];
+  const int ks = dim == 2 ? 16 : 32;
+  op.vtx_dof.alloc((size_t)nverts);
+  op.row_vtx.alloc((size_t)op.n);
+  op.slot.alloc((size_t)op.n * ks);
+  DevBuf<int> bad;
+  bad.alloc(1);
+  FH_HIP(hipMemsetAsync(op.row_vtx.p, 0xff, sizeof(int32_t) * (size_t)op.n, st));   // -1: no vertex
+  FH_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
+  const int64_t t = s.ncells * fe.nd;
+  const unsigned gb = (unsigned)((t + 255) / 256), rb = (unsigned)((op.n + 255) / 256);
+  if (dim == 2) {
+    hipLaunchKernelGGL(k_gather_vertices<2>, dim3(gb), dim3(256), 0, st, g, s.ncells, s.d_dofs.p, op.vtx_dof.p,
+                       op.row_vtx.p);
+    hipLaunchKernelGGL(k_gather_slots<2>, dim3(rb), dim3(256), 0, st, g, op.n, op.vtx_dof.p, op.row_vtx.p,
+                       op.rowptr.p, op.col.p, op.slot.p, bad.p);
+  } else {
+    hipLaunchKernelGGL(k_gather_vertices<3>, dim3(gb), dim3(256), 0, st, g, s.ncells, s.d_dofs.p, op.vtx_dof.p,
+                       op.row_vtx.p);
+    hipLaunchKernelGGL(k_gather_slots<3>, dim3(rb), dim3(256), 0, st, g, op.n, op.vtx_dof.p, op.row_vtx.p,
+                       op.rowptr.p, op.col.p, op.slot.p, bad.p);
+  }
+  FH_HIP(hipGetLastError());
+  int h_bad = 1;
+  FH_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  FH_HIP(hipStreamSynchronize(st));
+  if (h_bad) {
+    op.vtx_dof.release();
+    op.row_vtx.release();
+    op.slot.release();
+    op.gather = -1;
+    return;
+  }
+  // trig factors at the 1-D quadrature points of every cell layer: the d-th
+  // coordinate of a quadrature point is x_i (1 - xi) / 2 + x_{i+1} (1 + xi) / 2
+  // (the geometric Q1 map of sbm_cell_map.i90:673-687 along one direction)
+  std::vector<double> h;
+  for (int d = 0; d < dim; ++d) {
+    op.trig_off[d] = (int)(h.size() / 4);
+    const int nd = s.mesh.n[d];
+    const double lo = s.mesh.limits[2 * d], hi = s.mesh.limits[2 * d + 1];
+    for (int i = 0; i < nd; ++i)
+      for (int a = 0; a < 2; ++a) {
+        const double xi = fe.qp[d][a << d];   // quadrature point with 1-D index a along d, 0 elsewhere
+        const double x0 = (hi - lo) * ((double)i / (double)nd) + lo;
+        const double x1 = (hi - lo) * ((double)(i + 1) / (double)nd) + lo;
+        const double x = x0 * (0.5 * (1.0 - xi)) + x1 * (0.5 * (1.0 + xi));
+        h.push_back(std::sin(kPi * x));
+        h.push_back(std::cos(kPi * x));
+        h.push_back(std::sin(2 * kPi * x));
+        h.push_back(std::cos(2 * kPi * x));
+      }
+  }
+  op.trig.alloc(h.size());
+  FH_HIP(hipMemcpyAsync(op.trig.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, st));
+  FH_HIP(hipStreamSynchronize(st));
+  op.gather = 1;
+}
+
+constexpr int kGatherRows = 128;
+
+void launch_gather(femhip_operator &op, const PhysDev &ph, const double *d_fixed) {
+  femhip_space &s = *op.space;
+  hipStream_t st = op.ctx->stream;
+  const VtxGrid g = vtx_grid(s.mesh);
+  const TrigTab tt{reinterpret_cast<const double4 *>(op.trig.p), {op.trig_off[0], op.trig_off[1], op.trig_off[2]}};
+  const unsigned rb = (unsigned)((op.n + kGatherRows - 1) / kGatherRows);
+  if (s.fe.dim == 2)
+    hipLaunchKernelGGL((k_assemble_gather<2, kGatherRows>), dim3(rb), dim3(kGatherRows), 0, st, g, ph, tt, op.n,
+                       op.row_vtx.p, op.vtx_dof.p, op.slot.p, op.rowptr.p, op.tab.p, d_fixed, op.val.p, op.rhs.p);
+  else
+    hipLaunchKernelGGL((k_assemble_gather<3, kGatherRows>), dim3(rb), dim3(kGatherRows), 0, st, g, ph, tt, op.n,
+                       op.row_vtx.p, op.vtx_dof.p, op.slot.p, op.rowptr.p, op.tab.p, d_fixed, op.val.p, op.rhs.p);
 }
 
 }  // namespace
@@ -327,18 +615,28 @@ void assemble(femhip_operator &op, const double *fixed_host, float *ms) {
     FH_HIP(hipMemcpyAsync(op.fixed.p, fixed_host, sizeof(double) * (size_t)s.n_fixed, hipMemcpyHostToDevice, st));
     d_fixed = op.fixed.p;
   }
+  const RefFE &fe = s.fe;
+  // the gather maps depend on the space only; whether they apply depends on the physics
+  const bool poisson_q1 = op.tab_kind == 0 && fe.order == 1 && fe.ncomp == 1 && fe.nd == (1 << fe.dim);
+  if (poisson_q1 && op.gather == 0) setup_gather(op);
+  const bool gather = poisson_q1 && op.gather == 1;
   hipEvent_t e0, e1;
   FH_HIP(hipEventCreate(&e0));
   FH_HIP(hipEventCreate(&e1));
   FH_HIP(hipEventRecord(e0, st));
-  int zb = (int)std::min<int64_t>(4096, (op.nnz_full + 255) / 256 + 1);
-  hipLaunchKernelGGL(k_fill_zero, dim3(zb), dim3(256), 0, st, op.val.p, op.nnz_full);
-  hipLaunchKernelGGL(k_fill_zero, dim3(std::max(1, std::min(1024, (op.n + 255) / 256))), dim3(256), 0, st, op.rhs.p,
-                     (int64_t)op.n);
   MeshDev m = mesh_dev(s.mesh);
   PhysDev ph{op.phys.kind, op.phys.k_kind, op.phys.f_kind, op.phys.k0, op.phys.f0, op.k_cell.p};
-  const RefFE &fe = s.fe;
-  if (op.tab_kind == 0) {
+  if (gather) {   // every value and RHS entry is written once: no clearing pass
+    launch_gather(op, ph, d_fixed);
+  } else {
+    int zb = (int)std::min<int64_t>(4096, (op.nnz_full + 255) / 256 + 1);
+    hipLaunchKernelGGL(k_fill_zero, dim3(zb), dim3(256), 0, st, op.val.p, op.nnz_full);
+    hipLaunchKernelGGL(k_fill_zero, dim3(std::max(1, std::min(1024, (op.n + 255) / 256))), dim3(256), 0, st,
+                       op.rhs.p, (int64_t)op.n);
+  }
+  if (gather) {
+    // done above
+  } else if (op.tab_kind == 0) {
     if (fe.dim == 2 && fe.nd == 4) launch_poisson<2, 4, 4, 64>(op, m, ph, d_fixed);
     else if (fe.dim == 2 && fe.nd == 9) launch_poisson<2, 9, 9, 28>(op, m, ph, d_fixed);
     else if (fe.dim == 3 && fe.nd == 8) launch_poisson<3, 8, 8, 32>(op, m, ph, d_fixed);

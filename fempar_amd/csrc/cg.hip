@@ -1217,6 +1217,24 @@ int femhip_solver_get_solution(const femhip_solver_t *s, double *x) {
   }
 }
 
+int femhip_solver_error_norms(femhip_solver_t *s, int32_t exact, femhip_error_norms_t *norms) {
+  try {
+    FH_CHECK(s && norms, FEMHIP_ERR_INVALID, "null argument");
+    FH_CHECK(s->have_solution, FEMHIP_ERR_STATE, "no solution yet");
+    FH_HIP(hipSetDevice(s->ctx->device));
+    double out[4] = {0, 0, 0, 0};
+    error_norms(*s->op, s->x.p, exact, out);
+    norms->l2_norm = out[0];
+    norms->h1_seminorm = out[1];
+    norms->h1_norm = out[2];
+    norms->linfty_norm = out[3];
+    return FEMHIP_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
 int femhip_solver_destroy(femhip_solver_t *s) {
   if (s) {
     femhip_ctx *c = s->ctx;

@@ -180,6 +180,15 @@ struct femhip_operator {
   int tab_kind = 0;
   bool computed = false;
   femhip::DevBuf<double> fixed;             // fixed values on device
+  // owner-computes assembly of scalar Q1 (one row per free vertex): vertex ->
+  // signed DOF id, free row -> vertex, per row the CSR slot of each of the 3^d
+  // neighbour offsets (255: none), and per direction the sin/cos factors of
+  // the coefficients at the 1-D quadrature points of every cell layer
+  int gather = 0;                           // 0 not set up, 1 in use, -1 not applicable
+  femhip::DevBuf<int32_t> vtx_dof, row_vtx;
+  femhip::DevBuf<uint8_t> slot;
+  femhip::DevBuf<double> trig;              // 4 values per (direction, layer, 1-D point)
+  int trig_off[3] = {0, 0, 0};
 };
 
 namespace femhip {
@@ -194,6 +203,10 @@ void build_sliced(femhip_operator &op);
 void assemble(femhip_operator &op, const double *fixed_host, float *ms);
 void upload_physics(femhip_operator &op);
 void spmv_full(femhip_operator &op, const double *d_x, double *d_y);
+
+// error norms of the FE function with free values d_x (norms.hip):
+// out = {l2_norm, h1_seminorm, h1_norm, linfty_norm}
+void error_norms(femhip_operator &op, const double *d_x, int exact, double out[4]);
 
 // non-blocking CG driver (cg.hip) used by the request server: solver_start
 // prepares the solver format and enqueues the initial residual for b = the
@@ -212,7 +225,8 @@ size_t solver_device_bytes(const femhip_solver &s);
 // device memory held by an operator (CSR, solver format, tables) and its space
 inline size_t operator_device_bytes(const femhip_operator &op) {
   size_t b = op.rowptr.bytes() + op.col.bytes() + op.val.bytes() + op.rhs.bytes() + op.val32.bytes() +
-             op.upper_map.bytes() + op.k_cell.bytes() + op.tab.bytes() + op.fixed.bytes();
+             op.upper_map.bytes() + op.k_cell.bytes() + op.tab.bytes() + op.fixed.bytes() + op.vtx_dof.bytes() +
+             op.row_vtx.bytes() + op.slot.bytes() + op.trig.bytes();
   const Sliced &s = op.sl;
   b += s.soffs.bytes() + s.base.bytes() + s.meta.bytes() + s.aux.bytes() + s.offs.bytes() + s.ecol.bytes() +
        s.map.bytes() + s.v64.bytes() + s.v32.bytes();

@@ -368,6 +368,14 @@ class iterative_linear_solver_t:
         N.check(self._lib.femhip_solver_get_solution(self.h, N.ptr(x)))
         return x
 
+    def error_norms(self, exact="sin"):
+        """error_norms_scalar_t%compute of the last solution: l2_norm, h1_seminorm,
+        h1_norm, linfty_norm against u = prod sin(pi x_d) ("sin") or x + y (+ z) ("linear")."""
+        r = N.ErrorNorms()
+        N.check(self._lib.femhip_solver_error_norms(self.h, {"sin": N.EXACT_SIN, "linear": N.EXACT_LINEAR}[exact],
+                                                    C.byref(r)))
+        return {f: getattr(r, f) for f, _ in N.ErrorNorms._fields_}
+
     def profile_spmv(self, reps=50):
         """Average duration (ms) and algorithmic bytes of one SpMV launch of this solver."""
         self._ensure()

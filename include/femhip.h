@@ -203,6 +203,19 @@ int femhip_solver_profile_spmv(femhip_solver_t *solver, int32_t reps, double *ms
                                double *bytes_per_launch);
 /* copy the last solution to the host (n values) */
 int femhip_solver_get_solution(const femhip_solver_t *solver, double *x);
+
+/* Error of the last solution (fixed DOFs: the values given to
+ * femhip_operator_compute, 0 if none) against an exact solution, scalar
+ * spaces; replaces error_norms_scalar_t%compute
+ * (Integration/ErrorNorms/sbm_error_norms_scalar.i90:67-241) for the norm
+ * types l2_norm, h1_seminorm, h1_norm and linfty_norm
+ * (Integration/ErrorNorms/error_norms.f90:43-52). */
+enum { FEMHIP_EXACT_SIN = 0,      /* u = prod_d sin(pi x_d), the manufactured solution */
+       FEMHIP_EXACT_LINEAR = 1 }; /* u = x + y (+ z) (poisson_analytical_functions.f90:77-104) */
+typedef struct {
+  double l2_norm, h1_seminorm, h1_norm, linfty_norm;
+} femhip_error_norms_t;
+int femhip_solver_error_norms(femhip_solver_t *solver, int32_t exact, femhip_error_norms_t *norms);
 int femhip_solver_destroy(femhip_solver_t *solver);
 
 /* ---- batched form: many independent small problems in one persistent launch ----

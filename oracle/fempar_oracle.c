@@ -897,3 +897,64 @@ void orc_cg(int n, const int32_t *irp, const int32_t *ja, const double *val, int
   res->rhs_test = rhs_test;
   free(r); free(z); free(p); free(Ap); free(inv);
 }
+
+/* ---- error norms (Integration/ErrorNorms/sbm_error_norms_scalar.i90:67-241) ---- */
+
+static void exact_value(int exact, int d, const double x[3], double *u, double g[3]) {
+  g[0] = g[1] = g[2] = 0.0;
+  if (exact == ORC_EXACT_LINEAR) {   /* poisson_analytical_functions.f90:77-104 */
+    *u = 0.0;
+    for (int k = 0; k < d; ++k) {
+      *u += x[k];
+      g[k] = 1.0;
+    }
+    return;
+  }
+  double s[3] = {1.0, 1.0, 1.0}, c[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < d; ++k) {
+    s[k] = sin(PI * x[k]);
+    c[k] = cos(PI * x[k]);
+  }
+  *u = s[0] * s[1] * s[2];
+  for (int k = 0; k < d; ++k) {
+    double v = PI * c[k];
+    for (int e = 0; e < d; ++e)
+      if (e != k) v *= s[e];
+    g[k] = v;
+  }
+}
+
+void orc_error_norms(const orc_space_t *s, const double *x_free, const double *fixed_values, int exact,
+                     double out[4]) {
+  const int nd = s->nd, d = s->d;
+  double values_norm = 0.0, gradients_norm = 0.0, linf = 0.0;
+  double xq[MAXNQ][3], nodal[MAXND];
+  for (int64_t c = 0; c < s->ncells; ++c) {   /* fe cell iterator, cells in order (:94-110) */
+    qpoint_coords(s, c, xq);
+    const int32_t *cd = s->dofs + c * nd;
+    for (int i = 0; i < nd; ++i)   /* fe_cell_function%update: nodal values of the cell */
+      nodal[i] = cd[i] > 0 ? x_free[cd[i] - 1] : (fixed_values ? fixed_values[-cd[i] - 1] : 0.0);
+    double vc = 0.0, gc = 0.0;
+    for (int q = 0; q < s->nq; ++q) {
+      double uh = 0.0, guh[3] = {0.0, 0.0, 0.0}, u, gu[3];
+      for (int i = 0; i < nd; ++i) {   /* values and gradients at the quadrature point */
+        uh += nodal[i] * s->shp[i][q];
+        for (int k = 0; k < 3; ++k) guh[k] += nodal[i] * s->grad[i][k][q];
+      }
+      exact_value(exact, d, xq[q], &u, gu);
+      const double e = u - uh;
+      double ge2 = 0.0;
+      for (int k = 0; k < 3; ++k) ge2 += (gu[k] - guh[k]) * (gu[k] - guh[k]);
+      const double dv = s->detj[q] * s->qw[q];
+      vc += (e * e) * dv;      /* l2_norm, h1_norm (:200-205) */
+      gc += ge2 * dv;          /* h1_norm, h1_seminorm (:220-225) */
+      if (fabs(e) > linf) linf = fabs(e);
+    }
+    values_norm += vc;          /* update_norm (error_norms.f90:160-172): sum over cells */
+    gradients_norm += gc;
+  }
+  out[0] = sqrt(values_norm);   /* finalize_norm (error_norms.f90:174-221): square roots */
+  out[1] = sqrt(gradients_norm);
+  out[2] = sqrt(values_norm + gradients_norm);
+  out[3] = linf;
+}

@@ -122,6 +122,18 @@ void orc_cg(int n, const int32_t *irp, const int32_t *ja, const double *val, int
             int precond, int criterion, double rtol, double atol, int maxit, const double *b,
             double *x, orc_cg_result_t *res, double *history /* maxit entries or NULL */);
 
+/* exact solutions for the error norms */
+enum { ORC_EXACT_SIN = 0,      /* u = prod_d sin(pi x_d)   (the manufactured solution)     */
+       ORC_EXACT_LINEAR = 1 }; /* u = x + y (+ z)          (poisson_analytical_functions.f90:77-104) */
+
+/* error_norms_scalar_t%compute (Integration/ErrorNorms/sbm_error_norms_scalar.i90:67-241)
+   of the FE function with free values x_free (n_free) and fixed values
+   fixed_values (n_fixed, NULL: 0) against `exact`, scalar spaces:
+   out[0] = l2_norm, out[1] = h1_seminorm, out[2] = h1_norm, out[3] = linfty_norm
+   (norm types of Integration/ErrorNorms/error_norms.f90:43-52). */
+void orc_error_norms(const orc_space_t *s, const double *x_free, const double *fixed_values, int exact,
+                     double out[4]);
+
 #ifdef __cplusplus
 }
 #endif

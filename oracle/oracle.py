@@ -20,6 +20,7 @@ PHYS_POISSON, PHYS_ELASTICITY = 0, 1
 K_CONST, K_SIN3, K_CELL = 0, 1, 2
 F_CONST, F_MANUFACTURED, F_CELL_SIN = 0, 1, 2
 G_ZERO, G_LINEAR = 0, 1
+EXACT_SIN, EXACT_LINEAR = 0, 1
 
 CRITERIA = ["RES_NRMGIVEN_RHS_NRMGIVEN", "RES_NRMGIVEN_RES_NRMGIVEN", "DELTA_RHS", "DELTA_DELTA",
             "RES_RES", "RES_RHS", "DELTA_RHS_AND_RES_RES", "DELTA_RHS_AND_RES_RHS",
@@ -77,6 +78,7 @@ def lib():
         L.orc_extract_diagonal.argtypes = [C.c_int, P, P, P, P]
         L.orc_cg.argtypes = [C.c_int, P, P, P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                              C.c_int, P, P, C.POINTER(CgResult), P]
+        L.orc_error_norms.argtypes = [P, P, P, C.c_int, P]
         _lib = L
     return _lib
 
@@ -150,6 +152,16 @@ class Space:
         shp = np.empty((self.nq, nnode)); dshp = np.empty((self.nq, nnode, d))
         lib().orc_reference_tables(self.h, _ptr(qp), _ptr(qw), _ptr(shp), _ptr(dshp))
         return qp, qw, shp, dshp
+
+    def error_norms(self, x_free, fixed_values=None, exact="sin"):
+        """l2_norm, h1_seminorm, h1_norm, linfty_norm of u - u_h
+        (error_norms_scalar_t%compute) for u = prod sin(pi x_d) or x + y (+ z)."""
+        x = np.ascontiguousarray(x_free, dtype=np.float64)
+        fx = None if fixed_values is None else np.ascontiguousarray(fixed_values, dtype=np.float64)
+        out = np.zeros(4)
+        lib().orc_error_norms(self.h, _ptr(x), _ptr(fx), {"sin": EXACT_SIN, "linear": EXACT_LINEAR}[exact],
+                              _ptr(out))
+        return {"l2_norm": out[0], "h1_seminorm": out[1], "h1_norm": out[2], "linfty_norm": out[3]}
 
     def element(self, phys, cell):
         em = np.empty((self.nd, self.nd), dtype=np.float64, order="F")

@@ -129,6 +129,31 @@ def test_cg_solution_matches_oracle(env, name, prec):
     assert np.linalg.norm(solver.get_solution() - x_ref) <= SOL_RTOL * np.linalg.norm(x_ref)
 
 
+@pytest.mark.parametrize("name,exact", [("2d_q1_64_sin", "sin"), ("2d_q2_12", "sin"), ("3d_q1_20_sin3", "sin"),
+                                        ("3d_q2_8_sin3", "sin"), ("3d_q1_linear_g", "linear"),
+                                        ("2d_q2_linear_g", "linear")])
+def test_error_norms_match_oracle(env, name, exact):
+    """Device error norms of the device solution equal the oracle's norms of the
+    same vector within 1e-10 relative (1e-12 absolute when the FE space holds u)."""
+    print(f"[norms] {name} {exact}", flush=True)
+    osp, oph, fes, op, di = _build(env, name)
+    g = osp.fixed_values(oph)
+    op.set_fixed_dof_values(g if osp.n_fixed else None)
+    op.compute()
+    solver = F.iterative_linear_solver_t().create(env)
+    solver.set_type_from_string("CG")
+    solver.set_parameters_from_pl({"ILS_RTOL": 1e-12, "ILS_MAX_NUM_ITERATIONS": 20000})
+    solver.set_operators(op.get_tangent(), F.jacobi_preconditioner_t(op))
+    x = solver.solve(op.get_translation())
+    got = solver.error_norms(exact)
+    ref = osp.error_norms(x, g if osp.n_fixed else None, exact=exact)
+    for key in ("l2_norm", "h1_seminorm", "h1_norm", "linfty_norm"):
+        if exact == "linear":
+            assert got[key] < 1e-10 and ref[key] < 1e-10, key
+        else:
+            assert abs(got[key] - ref[key]) <= 1e-10 * ref[key], (key, got[key], ref[key])
+
+
 @pytest.mark.parametrize("crit", O.CRITERIA)
 def test_stopping_criteria_match_oracle(env, crit):
     osp, oph, fes, op, di = _build(env, "2d_q1_64_sin")

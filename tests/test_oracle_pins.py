@@ -317,3 +317,48 @@ def test_cg_reference_semantics():
     r0 = np.linalg.norm(rhs)
     assert abs(info["rhs_test"] - 1e-8 * r0) < 1e-20
     assert info["history"][-1] <= 1e-8 * r0 < info["history"][-2]
+
+
+@pytest.mark.parametrize("dim,order,n", [(2, 1, 32), (2, 2, 16), (3, 1, 16), (3, 2, 8)])
+def test_error_norms_of_zero_match_closed_forms(dim, order, n):
+    """u_h = 0: ||u||_L2 = 2^(-d/2) and |u|_H1 = sqrt(d pi^2 / 2^d) for
+    u = prod sin(pi x_d); composite Gauss quadrature integrates sin^2 on uniform
+    cells exactly, so the norms hold to rounding; linfty is below max u = 1."""
+    sp = O.Space(O.make_problem(dim, [n] * dim, order))
+    r = sp.error_norms(np.zeros(sp.n_free))
+    assert abs(r["l2_norm"] - 0.5 ** (dim / 2)) < 1e-12
+    assert abs(r["h1_seminorm"] - np.sqrt(dim * np.pi ** 2 / 2 ** dim)) < 1e-12
+    assert abs(r["h1_norm"] - np.hypot(r["l2_norm"], r["h1_seminorm"])) < 1e-14
+    assert 0.99 < r["linfty_norm"] < 1.0
+
+
+@pytest.mark.parametrize("dim,order", [(2, 1), (3, 1), (2, 2)])
+def test_error_norms_vanish_for_the_linear_solution(dim, order):
+    """test_poisson checks the error norms of u = x + y (+ z) (test_poisson_driver.f90:562-600):
+    the FE space holds u, so every norm of u - u_h is at rounding level."""
+    n = 4
+    sp = O.Space(O.make_problem(dim, [n] * dim, order))
+    ph = O.make_physics(f_kind=O.F_CONST, f0=0.0, g_kind=O.G_LINEAR)
+    g = sp.fixed_values(ph)
+    irp, ja, val, rhs = sp.assemble(ph, True, fixed_values=g)
+    x, info = O.cg(irp, ja, val, rhs, True, precond="identity", rtol=1e-14, maxit=5000)
+    r = sp.error_norms(x, g, exact="linear")
+    assert max(r.values()) < 1e-10
+    r0 = sp.error_norms(np.zeros(sp.n_free), g, exact="linear")   # a wrong u_h is seen
+    assert r0["l2_norm"] > 0.1 and r0["h1_seminorm"] > 0.1
+
+
+@pytest.mark.parametrize("order,l2_rate,h1_rate", [(1, 4.0, 2.0), (2, 8.0, 4.0)])
+def test_error_norms_converge_at_the_theoretical_rates(order, l2_rate, h1_rate):
+    """-lap u = f with u = sin(pi x) sin(pi y): halving h divides the L2 error by
+    2^(p+1) and the H1 seminorm error by 2^p."""
+    errs = []
+    for n in (8, 16, 32):
+        sp = O.Space(O.make_problem(2, (n, n), order))
+        ph = O.make_physics(k_kind=O.K_CONST, k0=1.0, f_kind=O.F_MANUFACTURED)
+        irp, ja, val, rhs = sp.assemble(ph, False)
+        x, info = O.cg(irp, ja, val, rhs, False, precond="jacobi", rtol=1e-13, maxit=100000)
+        errs.append(sp.error_norms(x))
+    for a, b in zip(errs, errs[1:]):
+        assert abs(a["l2_norm"] / b["l2_norm"] / l2_rate - 1) < 0.02
+        assert abs(a["h1_seminorm"] / b["h1_seminorm"] / h1_rate - 1) < 0.02
