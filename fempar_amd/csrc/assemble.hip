@@ -547,6 +547,37 @@ void launch_gather(femhip_operator &op, const PhysDev &ph, const double *d_fixed
 
 }  // namespace
 
+// Element matrix (row-major nd x nd) and body-force vector of the irreducible
+// elasticity plugin for a vector FE (sbm_irreducible_discrete_integration.i90:64-130):
+// elmat(i,j) += dV grad(phi_i) : sigma(phi_j), sigma = lambda tr(eps) I + 2 mu eps,
+// lambda, mu from E, nu as par_test_elasticity_constitutive_models.f90:34-37.
+void elasticity_element(const RefFE &fe, const femhip_physics_t &ph, double *Ke, double *Fe) {
+  const int nd = fe.nd, nq = fe.nq, nn = fe.nnode;
+  const double E = ph.E, nu = ph.nu;
+  const double lambda = (nu * E) / ((1 + nu) * (1 - 2 * nu));
+  const double mu = E / (2 * (1 + nu));
+  for (int i = 0; i < nd * nd; ++i) Ke[i] = 0.0;
+  for (int i = 0; i < nd; ++i) Fe[i] = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    double dV = fe.detj[q] * fe.qw[q];
+    for (int i = 0; i < nd; ++i) {
+      int ci = i / nn, ai = i % nn;
+      for (int j = 0; j < nd; ++j) {
+        int cj = j / nn, aj = j % nn;
+        double gi[3], gj[3];
+        for (int k = 0; k < 3; ++k) {
+          gi[k] = fe.grad[ai][k][q];
+          gj[k] = fe.grad[aj][k][q];
+        }
+        double divdiv = gi[ci] * gj[cj];
+        double epseps = 0.5 * ((ci == cj ? gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2] : 0.0) + gi[cj] * gj[ci]);
+        Ke[i * nd + j] += dV * (lambda * divdiv + 2 * mu * epseps);
+      }
+      Fe[i] += dV * ph.body[ci] * fe.shp[ai][q];
+    }
+  }
+}
+
 // Element tables for the operator's physics (host, once per set_physics).
 void upload_physics(femhip_operator &op) {
   const RefFE &fe = op.space->fe;
@@ -575,29 +606,8 @@ void upload_physics(femhip_operator &op) {
     }
   } else if (ph.kind == FEMHIP_PHYS_ELASTICITY) {
     FH_CHECK(fe.ncomp == dim && dim == 3, FEMHIP_ERR_INVALID, "elasticity plugin needs a 3-D vector FE space");
-    const double E = ph.E, nu = ph.nu;
-    const double lambda = (nu * E) / ((1 + nu) * (1 - 2 * nu));
-    const double mu = E / (2 * (1 + nu));
     t.assign((size_t)(nd * nd + nd), 0.0);
-    for (int q = 0; q < nq; ++q) {
-      double dV = fe.detj[q] * fe.qw[q];
-      for (int i = 0; i < nd; ++i) {
-        int ci = i / nn, ai = i % nn;
-        for (int j = 0; j < nd; ++j) {
-          int cj = j / nn, aj = j % nn;
-          // grad(phi_i) : sigma(phi_j),  sigma = lambda tr(eps) I + 2 mu eps
-          double gi[3], gj[3];
-          for (int k = 0; k < 3; ++k) {
-            gi[k] = fe.grad[ai][k][q];
-            gj[k] = fe.grad[aj][k][q];
-          }
-          double divdiv = gi[ci] * gj[cj];
-          double epseps = 0.5 * ((ci == cj ? gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2] : 0.0) + gi[cj] * gj[ci]);
-          t[(size_t)(i * nd + j)] += dV * (lambda * divdiv + 2 * mu * epseps);
-        }
-        t[(size_t)(nd * nd + i)] += dV * ph.body[ci] * fe.shp[ai][q];
-      }
-    }
+    elasticity_element(fe, ph, t.data(), t.data() + nd * nd);
     op.tab_kind = 1;
   } else {
     throw Error{FEMHIP_ERR_INVALID, "unknown physics kind"};

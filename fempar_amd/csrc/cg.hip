@@ -699,6 +699,33 @@ __global__ void k_jacobi(int32_t n, const int32_t *__restrict__ rowptr, const in
   if (invd32) invd32[i] = (float)inv;
 }
 
+// The same from the sliced copy, for operators held without a CSR: the entry
+// of offset 0 of a diagonal slice, or the first ELL entry whose column is the
+// row (real entries precede the padding).
+__global__ void k_jacobi_sliced(SlicedDev m, const double *__restrict__ v, double *__restrict__ invd,
+                                float *__restrict__ invd32, int *zero_diag) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  const int32_t s = i / kSlice, lane = i % kSlice;
+  const int32_t meta = m.meta[s];
+  const int32_t K = meta & ((1 << 29) - 1);
+  const bool ell = (meta >> 30) & 1;
+  const int64_t b = m.base[s];
+  const int32_t ax = m.aux[s];
+  double d = 0.0;
+  for (int32_t k = 0; k < K; ++k) {
+    const bool hit = ell ? m.ecol[ax + (int64_t)k * kSlice + lane] == i : m.offs[ax + k] == 0;
+    if (hit) {
+      d = v[b + (int64_t)k * kSlice + lane];
+      break;
+    }
+  }
+  if (d == 0.0) *zero_diag = 1;
+  const double inv = 1.0 / d;
+  invd[i] = inv;
+  if (invd32) invd32[i] = (float)inv;
+}
+
 // gather CSR values into the sliced layout (FP64 and/or FP32)
 __global__ void k_fill_sliced(int64_t E, const int32_t *__restrict__ map, const double *__restrict__ val,
                               double *__restrict__ v64, float *__restrict__ v32) {
@@ -843,8 +870,12 @@ void prepare(femhip_solver &s) {
   if (jac && n > 0) {
     int zero = 0;
     FH_HIP(hipMemcpyAsync(s.flag.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_jacobi, dim3((n + 255) / 256), dim3(256), 0, st, n, op.rowptr.p, op.col.p, op.val.p,
-                       s.invd.p, mx ? s.invd32.p : nullptr, s.flag.p);
+    if (op.structured)   // no CSR: the diagonal comes from the sliced copy
+      hipLaunchKernelGGL(k_jacobi_sliced, dim3((n + 255) / 256), dim3(256), 0, st, sliced_dev(op), op.sl.v64.p,
+                         s.invd.p, mx ? s.invd32.p : nullptr, s.flag.p);
+    else
+      hipLaunchKernelGGL(k_jacobi, dim3((n + 255) / 256), dim3(256), 0, st, n, op.rowptr.p, op.col.p, op.val.p,
+                         s.invd.p, mx ? s.invd32.p : nullptr, s.flag.p);
     FH_HIP(hipGetLastError());
     FH_HIP(hipMemcpyAsync(&zero, s.flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
     FH_HIP(hipStreamSynchronize(st));

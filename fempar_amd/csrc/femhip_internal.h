@@ -189,6 +189,11 @@ struct femhip_operator {
   femhip::DevBuf<uint8_t> slot;
   femhip::DevBuf<double> trig;              // 4 values per (direction, layer, 1-D point)
   int trig_off[3] = {0, 0, 0};
+  // structured stencil operator (no CSR, no space): rows are component-major
+  // blocks of `blk` rows (the `nodes` free nodes, padded to a multiple of 64),
+  // values assembled in place into the sliced layout
+  bool structured = false;
+  int32_t blk = 0, nodes = 0;
 };
 
 namespace femhip {
@@ -203,6 +208,15 @@ void build_sliced(femhip_operator &op);
 void assemble(femhip_operator &op, const double *fixed_host, float *ms);
 void upload_physics(femhip_operator &op);
 void spmv_full(femhip_operator &op, const double *d_x, double *d_y);
+
+// element matrix (row-major) and body-force vector of the elasticity plugin
+void elasticity_element(const RefFE &fe, const femhip_physics_t &ph, double *Ke, double *Fe);
+
+// Structured elasticity operator on the unit cube with n cells per direction,
+// clamped on x = 0 (stencil.hip): numbering, solver format, values, RHS built
+// on the device of ctx; row (c, node) = c * blk + node, node lexicographic
+// over the free vertices (x index 1..n fastest, then y, then z).
+femhip_operator *create_structured_elasticity(femhip_ctx *ctx, int32_t n, const femhip_physics_t &ph);
 
 // error norms of the FE function with free values d_x (norms.hip):
 // out = {l2_norm, h1_seminorm, h1_norm, linfty_norm}

@@ -8,9 +8,9 @@
 //    (device state says done, or the iteration budget is queued) or queues the
 //    next chunk on the request's slot stream, keeping two chunks in flight;
 //  * setup workers turn an admitted request into device state on its slot's
-//    context (numbering, CSR pattern, upload, integration + assembly, solver
-//    format, initial residual, graph capture) and retire finished requests
-//    (result, true residual, solution to the host, device memory freed).
+//    context (structured operator built and assembled on the device, Jacobi,
+//    initial residual, chunk graph) and retire finished requests (result,
+//    true residual, solution to the host, device memory freed).
 // Each slot owns a femhip context with its own non-blocking stream, so a
 // request's kernels, reductions and launch shapes are those of a solve alone.
 #include <hip/hip_runtime.h>
@@ -80,21 +80,17 @@ struct femhip_server {
 
 namespace {
 
-// Device memory bound for a request: space DOF table, CSR (rowptr, col, FP64
-// values, RHS), sliced solver copy (map, FP64 + FP32 values, ELL columns,
-// per-slice data), FP64 and FP32 Krylov vectors and the residual history.  The
-// sliced entries are bounded by 64-row slices of at most 1.25 x 81 + 1
-// diagonals (the diagonal-slice criterion) or 81 ELL columns.
+// Device memory bound for a request on the structured stencil operator: 81
+// FP64 + FP32 values per row, RHS, per-slice data, FP64 and FP32 Krylov
+// vectors, block partials and the residual history (rows = 3 x the free
+// vertices padded to a multiple of 64).
 double estimate_bytes(int32_t n, int32_t maxit) {
-  const double nn = n, ndof = 3.0 * nn * (nn + 1) * (nn + 1), cells = nn * nn * nn;
-  const double nnz = 81.0 * ndof;
-  const double slices = std::ceil(ndof / 64.0);
-  const double entries = 102.0 * 64.0 * slices;
-  const double csr = 4.0 * (ndof + 1) + 12.0 * nnz + 8.0 * ndof;
-  const double sliced = 20.0 * entries + 4.0 * 102.0 * slices + 24.0 * slices;
-  const double vecs = 48.0 * ndof + 20.0 * ndof + 8.0 * std::max(1, maxit);
-  const double other = 96.0 * cells + 24.0 * (nn + 1) * (nn + 1) + (4 << 20);
-  return csr + sliced + vecs + other;
+  const double nn = n, nodes = nn * (nn + 1) * (nn + 1);
+  const double rows = 3.0 * std::ceil(nodes / 64.0) * 64.0;
+  const double values = 81.0 * rows * 12.0;
+  const double slices = rows / 64.0;
+  const double vecs = 8.0 * rows + 48.0 * rows + 20.0 * rows + 8.0 * std::max(1, maxit);
+  return values + 24.0 * slices + vecs + (4 << 20);
 }
 
 void free_request_objects(Req &r) {
@@ -110,43 +106,36 @@ void check_rc(int rc) {
   if (rc != FEMHIP_OK) throw Error{rc, femhip_last_error()};
 }
 
-// numbering, pattern, assembly and solver start on the request's slot context
+// operator (built on the device), solver format and solver start on the
+// request's slot context
 void setup_request(femhip_server &S, Req &r) {
   femhip_ctx *c = S.slot_ctx[(size_t)r.slot];
   FH_HIP(hipSetDevice(c->device));
-  femhip_mesh_desc_t m{};
-  m.dim = 3;
-  m.n[0] = m.n[1] = m.n[2] = r.rq.n;
-  for (int k = 0; k < 3; ++k) {
-    m.limits[2 * k] = 0.0;
-    m.limits[2 * k + 1] = 1.0;
-  }
-  m.order = 1;
-  m.num_components = 3;
-  m.dirichlet = FEMHIP_BC_X0_FACE;
-  check_rc(femhip_space_create_structured(c, &m, &r.space));
-  check_rc(femhip_operator_create(c, r.space, 0, &r.op));
   femhip_physics_t ph{};
   ph.kind = FEMHIP_PHYS_ELASTICITY;
   ph.E = r.rq.E;
   ph.nu = r.rq.nu;
   for (int k = 0; k < 3; ++k) ph.body[k] = r.rq.body[k];
-  check_rc(femhip_operator_set_physics(r.op, &ph));
-  check_rc(femhip_operator_compute(r.op, nullptr, nullptr));
+  r.op = create_structured_elasticity(c, r.rq.n, ph);
   check_rc(femhip_solver_create(c, r.op, &S.prm.cg, &r.solver));
   solver_start(*r.solver);
   FH_HIP(hipStreamSynchronize(c->stream));
   r.res.hbm_bytes = (double)(operator_device_bytes(*r.op) + solver_device_bytes(*r.solver));
-  r.res.num_dofs = r.op->n;
+  r.res.num_dofs = 3 * r.op->nodes;
 }
 
-// result, true residual and solution copy, then the device memory is freed
+// result, true residual and solution copy (node-interleaved: 3 node + c), then
+// the device memory is freed
 void retire_request(femhip_server &S, Req &r) {
   (void)S;
   femhip_cg_result_t cg{};
   solver_finish(*r.solver, &cg);
+  std::vector<double> xi((size_t)r.op->n);
+  solver_copy_solution(*r.solver, xi.data());
+  const int32_t blk = r.op->blk, nodes = r.op->nodes;
   r.x.resize((size_t)r.res.num_dofs);
-  solver_copy_solution(*r.solver, r.x.data());
+  for (int c = 0; c < 3; ++c)
+    for (int32_t v = 0; v < nodes; ++v) r.x[(size_t)3 * v + c] = xi[(size_t)c * blk + v];
   r.res.status = cg.status;
   r.res.num_iterations = cg.num_iterations;
   r.res.outer_iterations = cg.outer_iterations;

@@ -272,15 +272,17 @@ int femhip_batch_destroy(femhip_batch_t *batch);
  * sbm_irreducible_discrete_integration.i90:64-130 with lambda, mu from E, nu as
  * in par_test_elasticity_constitutive_models.f90:34-37) on the unit cube with n
  * hexahedral Q1 cells per direction, vector Q1 (3 DOFs per node), all
- * components clamped (g = 0) on the x = 0 face, constant body force; the space,
- * operator and CG are those of femhip_space_create_structured,
- * femhip_operator_compute and femhip_solver_solve with FP32_REFINED precision.
+ * components clamped (g = 0) on the x = 0 face, constant body force, solved
+ * by the FP32-refined Jacobi-PCG of femhip_solver_solve.  Since every cell
+ * has the same element matrix, the operator is built on the device as a
+ * 27-point block stencil in the solver's sliced format (no DOF lists, no CSR
+ * pattern, no host pass over the matrix).
  *
  * The server owns `max_active` slots, each with its own HIP stream.  A request
  * is admitted when its arrival time has come, a slot is free and its device
  * memory estimate fits in what is left of `hbm_budget_bytes` (first come,
- * first served).  Admitted requests are set up by host worker threads (DOF
- * numbering, pattern, upload, device assembly) and then iterate on the device
+ * first served).  Admitted requests are set up by worker threads (operator
+ * build and assembly on the slot's stream) and then iterate on the device
  * in hipGraph chunks of CG iterations; a scheduler thread interleaves the
  * chunks of every running request, so requests that arrive while others are
  * on the GPU join as slots and memory free up, and leave as they converge.
@@ -338,7 +340,9 @@ int femhip_serve_submit(femhip_server_t *server, const femhip_serve_request_t *r
 /* Wait up to timeout_s (< 0: forever) for a request; FEMHIP_OK with the
  * result, or FEMHIP_ERR_STATE if it has not finished in time. */
 int femhip_serve_wait(femhip_server_t *server, int64_t ticket, double timeout_s, femhip_serve_result_t *result);
-/* Solution of a finished request (num_dofs values, numbering of the space). */
+/* Solution of a finished request: num_dofs values, entry 3 v + c is
+ * component c of free vertex v, the free vertices (x index 1..n) numbered
+ * x-fastest, then y, then z. */
 int femhip_serve_get_solution(femhip_server_t *server, int64_t ticket, double *x);
 /* Forget a finished request (its host solution). */
 int femhip_serve_release(femhip_server_t *server, int64_t ticket);
