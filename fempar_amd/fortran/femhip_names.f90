@@ -62,6 +62,12 @@ module femhip_names
     real(c_double)     :: error_estimate, rhs_convergence_test, true_residual, rhs_norm, solve_ms
   end type
 
+  integer(c_int32_t), parameter, public :: FEMHIP_EXACT_SIN = 0, FEMHIP_EXACT_LINEAR = 1
+
+  type, bind(c), public :: femhip_error_norms_t
+    real(c_double) :: l2_norm, h1_seminorm, h1_norm, linfty_norm
+  end type
+
   public :: femhip_ctx_create, femhip_ctx_destroy, femhip_last_error, femhip_check
   public :: femhip_space_create_structured, femhip_space_create_from_dofs, femhip_space_info
   public :: femhip_space_get_dofs, femhip_space_interpolate_dirichlet, femhip_space_destroy
@@ -69,6 +75,7 @@ module femhip_names
   public :: femhip_operator_set_physics, femhip_operator_compute, femhip_operator_get_values
   public :: femhip_operator_apply, femhip_operator_destroy
   public :: femhip_solver_create, femhip_solver_solve, femhip_solver_get_solution, femhip_solver_destroy
+  public :: femhip_solver_error_norms
 
   interface
     integer(c_int) function femhip_ctx_create(device, ctx) bind(c, name="femhip_ctx_create")
@@ -182,6 +189,13 @@ module femhip_names
     integer(c_int) function femhip_solver_destroy(solver) bind(c, name="femhip_solver_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: solver
+    end function
+    integer(c_int) function femhip_solver_error_norms(solver, exact, norms) &
+        bind(c, name="femhip_solver_error_norms")
+      import :: c_int, c_ptr, c_int32_t, femhip_error_norms_t
+      type(c_ptr), value :: solver
+      integer(c_int32_t), value :: exact
+      type(femhip_error_norms_t), intent(out) :: norms
     end function
   end interface
 

@@ -16,6 +16,7 @@ program test_poisson_hip
   type(femhip_physics_t) :: phys
   type(femhip_cg_params_t) :: prm
   type(femhip_cg_result_t) :: res
+  type(femhip_error_norms_t) :: en
   integer(c_int64_t) :: num_cells, nnz, nnz_full
   integer(c_int32_t) :: nd, nq, n_free, n_fixed, n
   integer(c_int32_t), allocatable, target :: gids(:)
@@ -98,11 +99,16 @@ program test_poisson_hip
   end do
   write(*, '(a,i0,a,i0,a,i0,a,i0,a,i0,a,es10.3)') 'test_poisson_hip dim=', dim, ' n=', ncell, ' order=', order, &
       ' free=', n_free, ' cg_iterations=', res%num_iterations, ' max_nodal_error=', err
+  ! error norms of u - u_h, as test_poisson_driver's check_solution computes them
+  call femhip_check(femhip_solver_error_norms(solver, FEMHIP_EXACT_LINEAR, en), 'error_norms')
+  write(*, '(a,es10.3,a,es10.3,a,es10.3)') 'l2_norm=', en%l2_norm, ' h1_seminorm=', en%h1_seminorm, &
+      ' linfty_norm=', en%linfty_norm
   call femhip_check(femhip_solver_destroy(solver), 'solver_destroy')
   call femhip_check(femhip_operator_destroy(op), 'operator_destroy')
   call femhip_check(femhip_space_destroy(space), 'space_destroy')
   call femhip_check(femhip_ctx_destroy(ctx), 'ctx_destroy')
-  if (res%did_converge /= 1 .or. err > 1.0e-6_c_double) then
+  if (res%did_converge /= 1 .or. err > 1.0e-6_c_double .or. en%h1_norm > 1.0e-6_c_double &
+      .or. en%linfty_norm > 1.0e-6_c_double) then
     write(*, '(a)') 'FAILED'
     error stop 1
   end if
