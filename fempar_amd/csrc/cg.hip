@@ -72,7 +72,14 @@ struct SlicedDev {
   int64_t stride;
   const int32_t *soffs;
   int32_t order;               // traversal order of wave_slices
+  const int32_t *srow;         // rows of the lanes of permuted slices
 };
+
+// Row of lane `lane` of slice s (meta bit 28: the slice's rows are a sorted
+// window's, listed in srow; a lane without a row gets n).
+__device__ __forceinline__ int slice_row_index(const SlicedDev &m, int s, int lane) {
+  return ((m.meta[s] >> 28) & 1) ? m.srow[s * kSlice + lane] : s * kSlice + lane;
+}
 
 __device__ __forceinline__ bool is_delta(int c) {
   return c == FEMHIP_DELTA_RHS || c == FEMHIP_DELTA_DELTA || c >= FEMHIP_DELTA_RHS_AND_RES_RES;
@@ -189,7 +196,7 @@ __device__ __forceinline__ void slice_rows(const SlicedDev &m, const T *__restri
   }
   const int64_t b = m.base[s];
   const int meta = m.meta[s];
-  const int K = meta & 0x1fffffff;
+  const int K = meta & 0x0fffffff;
   const int row = s * kSlice + lane;
   for (int j = 0; j < NX; ++j) acc[j] = 0;
   const T *vv = v + b + lane;
@@ -256,7 +263,7 @@ __global__ __launch_bounds__(kThreads) void k_init(SlicedDev m, const T *__restr
   double acc[4] = {0, 0, 0, 0};
   for (int s = s0; s < s1; s += sstep) {
     T ax = x_zero ? (T)0 : slice_row<T, T>(m, v, x, s, lane);
-    const int row = s * kSlice + lane;
+    const int row = slice_row_index(m, s, lane);
     if (row < m.n) {
       T bi = b[row], ri = bi - ax;
       T zi = invd ? invd[row] * ri : ri;
@@ -339,7 +346,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SlicedDev m, const T *__restr
   A pq = 0, pp = 0;
   for (int s = s0; s < s1; s += sstep) {
     const A acc = slice_row<T, A>(m, v, p, s, lane);
-    const int row = s * kSlice + lane;
+    const int row = slice_row_index(m, s, lane);
     if (row < m.n) {
       q[row] = (T)acc;
       const A pi = (A)p[row];
@@ -452,7 +459,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_init(SlicedDev m, const double 
   double acc[4] = {0, 0, 0, 0};
   for (int s = s0; s < s1; s += sstep) {
     const double ax = x_zero ? 0.0 : slice_row<double, double>(m, v64, x, s, lane);
-    const int row = s * kSlice + lane;
+    const int row = slice_row_index(m, s, lane);
     if (row < m.n) {
       const double bi = b[row], ri = bi - ax;
       r64[row] = ri;
@@ -520,7 +527,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_spmv(SlicedDev m, const float *
     float pq = 0.f, pp = 0.f;
     for (int s = s0; s < s1; s += sstep) {
       const float a = slice_row<float, float>(m, v32, p, s, lane);
-      const int row = s * kSlice + lane;
+      const int row = slice_row_index(m, s, lane);
       if (row < m.n) {
         q[row] = a;
         const float pi = p[row];
@@ -536,7 +543,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_spmv(SlicedDev m, const float *
       const float *const ds[1] = {d};
       double ad[1];
       slice_rows<1, double, double, float>(m, v64, ds, s, lane, ad);
-      const int row = s * kSlice + lane;
+      const int row = slice_row_index(m, s, lane);
       if (row < m.n) {
         const double ri = b[row] - ax - ad[0];
         const float rf = (float)ri;
@@ -708,7 +715,7 @@ __global__ void k_jacobi_sliced(SlicedDev m, const double *__restrict__ v, doubl
   if (i >= m.n) return;
   const int32_t s = i / kSlice, lane = i % kSlice;
   const int32_t meta = m.meta[s];
-  const int32_t K = meta & ((1 << 29) - 1);
+  const int32_t K = meta & ((1 << 28) - 1);
   const bool ell = (meta >> 30) & 1;
   const int64_t b = m.base[s];
   const int32_t ax = m.aux[s];
@@ -757,7 +764,7 @@ __global__ __launch_bounds__(kThreads) void k_true_res(SlicedDev m, const double
   double acc[1] = {0};
   for (int s = s0; s < s1; s += sstep) {
     const double ax = slice_row<double, double>(m, v64, x, s, lane);
-    const int row = s * kSlice + lane;
+    const int row = slice_row_index(m, s, lane);
     if (row < m.n) {
       const double ri = b[row] - ax;
       acc[0] += ri * ri;
@@ -813,6 +820,7 @@ SlicedDev sliced_dev(femhip_operator &op) {
   m.kplus = op.sl.kplus;
   m.stride = op.sl.stride;
   m.soffs = op.sl.soffs.p;
+  m.srow = op.sl.srow.p;
   static const char *ord = getenv("FEMHIP_SLICE_ORDER");
   m.order = ord ? atoi(ord) : 1;
   return m;
@@ -1221,13 +1229,14 @@ int femhip_solver_profile_spmv(femhip_solver_t *s, int32_t reps, double *ms_per_
     (void)hipEventDestroy(e1);
     if (ms_per_launch) *ms_per_launch = ms / reps;
     // algorithmic bytes of the sliced format: values + per-slice metadata +
-    // diagonal offsets + ELL indices + one read of p and one write of q
+    // diagonal offsets + ELL indices + row lists of sorted windows + one read
+    // of p and one write of q
     const double vb = mx ? 4.0 : 8.0;
     const int64_t ell = (int64_t)(op.sl.ecol.n > 1 ? op.sl.ecol.n : 0);
     if (bytes)
       *bytes = op.sl.sym ? (double)op.sl.entries * vb + 2.0 * vb * op.n
                          : (double)op.sl.entries * vb + 16.0 * op.sl.ns + 4.0 * (double)op.sl.offs.n +
-                               4.0 * (double)ell + 2.0 * vb * op.n;
+                               4.0 * (double)ell + 4.0 * kSlice * (double)op.sl.perm_slices + 2.0 * vb * op.n;
     return FEMHIP_OK;
   } catch (const Error &e) {
     set_error(e.msg);

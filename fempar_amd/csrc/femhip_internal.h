@@ -124,8 +124,13 @@ namespace femhip {
 // diagonals: K offsets per slice and K x 64 values, no column indices, and the
 // x gathers of one diagonal are contiguous.  Other slices are stored as ELL
 // with K = longest row and explicit column indices.  Values are laid out
-// [slice][k][lane]; padding entries hold 0 and a valid column.
+// [slice][k][lane]; padding entries hold 0 and a valid column.  Where rows of
+// very different lengths mix (Q2: 27 to 125 entries), the rows of each
+// 4096-row window are sorted by length before they are cut into ELL slices,
+// so a slice's rows have similar lengths (little padding); such slices carry
+// the row of each lane in `srow`.
 constexpr int kSlice = 64;
+constexpr int kSortWindow = 4096;
 //
 // Symmetric-diagonal mode (sym = true): when every row's column offsets come
 // from one global set D = -D (structured Q1 numbering), only the diagonals with
@@ -137,13 +142,15 @@ struct Sliced {
   int32_t ns = 0;                 // number of slices
   int64_t entries = 0;            // padded entries
   int64_t dia_slices = 0;
+  int64_t perm_slices = 0;        // ELL slices over sorted windows
   bool sym = false;               // symmetric-diagonal mode
   int32_t kplus = 0;              // stored diagonals (offsets >= 0, first is 0)
   int64_t stride = 0;             // row stride of a stored diagonal (n rounded to 64)
   std::vector<int32_t> h_soffs;
   DevBuf<int32_t> soffs;          // kplus offsets, ascending
   DevBuf<int64_t> base;           // ns+1 entry offsets
-  DevBuf<int32_t> meta;           // ns: K | (is_ell << 30)
+  DevBuf<int32_t> meta;           // ns: K | (is_ell << 30) | (clamp << 29) | (permuted << 28)
+  DevBuf<int32_t> srow;           // ns x 64: row of each lane of a permuted slice (n: none)
   DevBuf<int32_t> aux;            // ns: offset into offs (diagonal) or ecol (ELL)
   DevBuf<int32_t> offs;           // diagonal offsets
   DevBuf<int32_t> ecol;           // ELL column indices
@@ -242,7 +249,8 @@ inline size_t operator_device_bytes(const femhip_operator &op) {
              op.upper_map.bytes() + op.k_cell.bytes() + op.tab.bytes() + op.fixed.bytes() + op.vtx_dof.bytes() +
              op.row_vtx.bytes() + op.slot.bytes() + op.trig.bytes();
   const Sliced &s = op.sl;
-  b += s.soffs.bytes() + s.base.bytes() + s.meta.bytes() + s.aux.bytes() + s.offs.bytes() + s.ecol.bytes() +
+  b += s.soffs.bytes() + s.base.bytes() + s.meta.bytes() + s.srow.bytes() + s.aux.bytes() + s.offs.bytes() +
+       s.ecol.bytes() +
        s.map.bytes() + s.v64.bytes() + s.v32.bytes();
   if (op.space) b += op.space->d_dofs.bytes();
   return b;

@@ -107,6 +107,7 @@ void build_sliced(femhip_operator &op) {
   const bool want_sym = sym_env ? atoi(sym_env) != 0 : false;
   if (want_sym && try_build_symmetric_dia(op)) return;
   S.sym = false;
+  S.perm_slices = 0;
   const int32_t n = op.n;
   const int32_t ns = (n + kSlice - 1) / kSlice;
   const std::vector<int32_t> &rp = op.h_rowptr;
@@ -140,14 +141,40 @@ void build_sliced(femhip_operator &op) {
     for (unsigned t = 0; t < nth; ++t) th.emplace_back(work, t);
     for (auto &x : th) x.join();
   }
+  // windows holding an ELL slice: rows sorted by length (longest first, ties
+  // by row), every slice of the window ELL over the sorted rows
+  constexpr int32_t wsl = kSortWindow / kSlice;   // slices per window
+  std::vector<int32_t> rowof((size_t)ns * kSlice);
+  std::vector<char> perm((size_t)ns, 0);
+  bool any_perm = false;
+  for (int32_t w0 = 0; w0 < ns; w0 += wsl) {
+    const int32_t w1 = std::min(ns, w0 + wsl);
+    const int32_t r0 = w0 * kSlice, r1 = std::min(n, w1 * kSlice);
+    bool has_ell = false;
+    for (int32_t s = w0; s < w1; ++s) has_ell = has_ell || is_ell[(size_t)s];
+    for (int32_t i = r0; i < w1 * kSlice; ++i) rowof[(size_t)i] = i < r1 ? i : n;
+    if (!has_ell) continue;
+    any_perm = true;
+    std::stable_sort(rowof.begin() + r0, rowof.begin() + r1, [&](int32_t a, int32_t b) {
+      return rp[(size_t)a + 1] - rp[(size_t)a] > rp[(size_t)b + 1] - rp[(size_t)b];
+    });
+    for (int32_t s = w0; s < w1; ++s) {
+      const int32_t first = rowof[(size_t)s * kSlice];
+      is_ell[(size_t)s] = 1;
+      perm[(size_t)s] = 1;
+      ++S.perm_slices;
+      K[(size_t)s] = first < n ? rp[(size_t)first + 1] - rp[(size_t)first] : 0;   // longest row of the slice
+      offsets[(size_t)s].clear();
+    }
+  }
   std::vector<int64_t> base((size_t)ns + 1, 0);
   std::vector<int32_t> meta((size_t)ns), aux((size_t)ns);
   std::vector<int32_t> offs;
   int64_t ell_entries = 0, dia = 0;
   for (int32_t s = 0; s < ns; ++s) {
     base[(size_t)s + 1] = base[(size_t)s] + (int64_t)K[(size_t)s] * kSlice;
-    FH_CHECK(K[(size_t)s] < (1 << 29), FEMHIP_ERR_INVALID, "row too long for the sliced format");
-    meta[(size_t)s] = K[(size_t)s] | (is_ell[(size_t)s] << 30);
+    FH_CHECK(K[(size_t)s] < (1 << 28), FEMHIP_ERR_INVALID, "row too long for the sliced format");
+    meta[(size_t)s] = K[(size_t)s] | (is_ell[(size_t)s] << 30) | (perm[(size_t)s] << 28);
     if (!is_ell[(size_t)s] && !offsets[(size_t)s].empty()) {
       // bit 29: some lane's column row + offset leaves [0, n) (padding near the ends)
       const int64_t lo = (int64_t)s * kSlice + offsets[(size_t)s].front();
@@ -171,7 +198,7 @@ void build_sliced(femhip_operator &op) {
     for (int32_t s = (int32_t)((int64_t)ns * t / nth); s < (int32_t)((int64_t)ns * (t + 1) / nth); ++s) {
       const int32_t k_s = K[(size_t)s];
       for (int32_t lane = 0; lane < kSlice; ++lane) {
-        const int32_t r = s * kSlice + lane;
+        const int32_t r = rowof[(size_t)s * kSlice + lane];
         if (r >= n) {
           if (is_ell[(size_t)s])
             for (int32_t k = 0; k < k_s; ++k) ecol[(size_t)(aux[(size_t)s] + (int64_t)k * kSlice + lane)] = n - 1;
@@ -215,6 +242,8 @@ void build_sliced(femhip_operator &op) {
   S.offs.alloc(std::max<size_t>(1, offs.size()));
   S.ecol.alloc(ecol.size());
   S.map.alloc((size_t)std::max<int64_t>(1, E));
+  S.srow.alloc(any_perm ? rowof.size() : 1);
+  if (any_perm) FH_HIP(hipMemcpy(S.srow.p, rowof.data(), sizeof(int32_t) * rowof.size(), hipMemcpyHostToDevice));
   FH_HIP(hipMemcpy(S.base.p, base.data(), sizeof(int64_t) * base.size(), hipMemcpyHostToDevice));
   if (ns) {
     FH_HIP(hipMemcpy(S.meta.p, meta.data(), sizeof(int32_t) * meta.size(), hipMemcpyHostToDevice));
