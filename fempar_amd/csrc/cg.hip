@@ -171,6 +171,25 @@ __device__ __forceinline__ void sym_rows(const SlicedDev &m, const T *__restrict
   const int hi = m.n - 1;
   for (int j = 0; j < NX; ++j) acc[j] = 0;
   const T *vr = v + row;
+  if (m.kplus == 14) {   // 27-point stencils: every load of the row in flight at once
+#pragma unroll
+    for (int k = 0; k < 14; ++k) {
+      const int o = m.soffs[k];
+      const A a = (A)vr[(int64_t)k * m.stride];
+      const int c = min(row + o, hi);
+      for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][c];
+    }
+#pragma unroll
+    for (int k = 1; k < 14; ++k) {
+      const int o = m.soffs[k];
+      const int rr = row - o;
+      const int rc = min(max(rr, 0), hi);
+      A a = (A)v[(int64_t)k * m.stride + rc];
+      if (rr < 0) a = 0;
+      for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][rc];
+    }
+    return;
+  }
   for (int k = 0; k < m.kplus; ++k) {   // diagonal and upper part
     const int o = m.soffs[k];
     const A a = (A)vr[k * m.stride];

@@ -98,14 +98,14 @@ def test_serve_alone_vs_mixed(env):
 
 def test_serve_admission_budget(env):
     probe = F.serve_t(env, max_active=1, setup_threads=1)
-    est4, est8 = probe.estimate_bytes(4), probe.estimate_bytes(8)
+    est4, est16 = probe.estimate_bytes(4), probe.estimate_bytes(16)
     probe.free()
-    budget = 1.5 * est4                              # room for one n = 4 request at a time
+    budget = 1.2 * est4                              # room for one n = 4 request at a time
     srv = F.serve_t(env, hbm_budget_bytes=budget, max_active=4, setup_threads=2, rtol=RTOL)
     try:
         with pytest.raises(N.FemhipError) as ei:
-            srv.submit(8, 10.0, 0.3)
-        assert ei.value.code == N.ERR_ADMISSION and est8 > budget
+            srv.submit(16, 10.0, 0.3)
+        assert ei.value.code == N.ERR_ADMISSION and est16 > budget
         tickets = [srv.submit(4, 10.0 + i, 0.3) for i in range(3)]
         for t in tickets:
             assert srv.wait(t, timeout_s=60)["did_converge"] == 1
