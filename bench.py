@@ -221,6 +221,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--order", type=int, default=1, help="Lagrangian order of the --spmv-only problem")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -281,7 +282,7 @@ def main():
         print(json.dumps(other_configs()))
         return
     if args.spmv_only:
-        tria, fes, di, op = make_problem(F, env, n)
+        tria, fes, di, op = make_problem(F, env, n, order=args.order)
         op.compute()
         res = {}
         for prec in ("FP64", "FP32_REFINED"):

@@ -76,8 +76,11 @@ struct SlicedDev {
 };
 
 // Row of lane `lane` of slice s (meta bit 28: the slice's rows are a sorted
-// window's, listed in srow; a lane without a row gets n).
+// window's, listed in srow; a lane without a row gets n).  The symmetric
+// layout has no per-slice metadata and formats without sorted windows no row
+// list: their rows are in order.
 __device__ __forceinline__ int slice_row_index(const SlicedDev &m, int s, int lane) {
+  if (m.sym || m.srow == nullptr) return s * kSlice + lane;
   return ((m.meta[s] >> 28) & 1) ? m.srow[s * kSlice + lane] : s * kSlice + lane;
 }
 

@@ -125,12 +125,12 @@ namespace femhip {
 // x gathers of one diagonal are contiguous.  Other slices are stored as ELL
 // with K = longest row and explicit column indices.  Values are laid out
 // [slice][k][lane]; padding entries hold 0 and a valid column.  Where rows of
-// very different lengths mix (Q2: 27 to 125 entries), the rows of each
-// 4096-row window are sorted by length before they are cut into ELL slices,
-// so a slice's rows have similar lengths (little padding); such slices carry
-// the row of each lane in `srow`.
+// very different lengths mix (Q2: 27 to 125 entries), the rows of a small
+// window of consecutive rows may be sorted by length before they are cut into
+// ELL slices, so a slice's rows have similar lengths (little padding) while
+// its gathers stay local; such slices carry the row of each lane in `srow`.
 constexpr int kSlice = 64;
-constexpr int kSortWindow = 4096;
+constexpr int kSortWindow = 512;
 //
 // Symmetric-diagonal mode (sym = true): when every row's column offsets come
 // from one global set D = -D (structured Q1 numbering), only the diagonals with

@@ -103,8 +103,10 @@ bool try_build_symmetric_dia(femhip_operator &op) {
 
 void build_sliced(femhip_operator &op) {
   Sliced &S = op.sl;
+  // symmetric-diagonal storage wherever the pattern allows it (half the matrix
+  // stream); FEMHIP_SYMDIA=0 keeps the full diagonals
   static const char *sym_env = getenv("FEMHIP_SYMDIA");
-  const bool want_sym = sym_env ? atoi(sym_env) != 0 : false;
+  const bool want_sym = sym_env ? atoi(sym_env) != 0 : true;
   if (want_sym && try_build_symmetric_dia(op)) return;
   S.sym = false;
   S.perm_slices = 0;
@@ -141,23 +143,32 @@ void build_sliced(femhip_operator &op) {
     for (unsigned t = 0; t < nth; ++t) th.emplace_back(work, t);
     for (auto &x : th) x.join();
   }
-  // windows holding an ELL slice: rows sorted by length (longest first, ties
-  // by row), every slice of the window ELL over the sorted rows
-  constexpr int32_t wsl = kSortWindow / kSlice;   // slices per window
+  // a window's rows are sorted by length (longest first, ties by row) and cut
+  // into ELL slices when that stores fewer bytes than the slices as they are
+  // (values at 8 B, ELL columns and row lists at 4 B)
+  static const char *win_env = getenv("FEMHIP_SORT_WINDOW");
+  const int32_t window = win_env ? std::max(0, atoi(win_env)) / kSlice * kSlice : kSortWindow;
+  const int32_t wsl = std::max(1, window / kSlice);   // slices per window
   std::vector<int32_t> rowof((size_t)ns * kSlice);
   std::vector<char> perm((size_t)ns, 0);
   bool any_perm = false;
   for (int32_t w0 = 0; w0 < ns; w0 += wsl) {
     const int32_t w1 = std::min(ns, w0 + wsl);
     const int32_t r0 = w0 * kSlice, r1 = std::min(n, w1 * kSlice);
-    bool has_ell = false;
-    for (int32_t s = w0; s < w1; ++s) has_ell = has_ell || is_ell[(size_t)s];
     for (int32_t i = r0; i < w1 * kSlice; ++i) rowof[(size_t)i] = i < r1 ? i : n;
-    if (!has_ell) continue;
-    any_perm = true;
-    std::stable_sort(rowof.begin() + r0, rowof.begin() + r1, [&](int32_t a, int32_t b) {
+    if (window == 0) continue;
+    int64_t as_is = 0;
+    for (int32_t s = w0; s < w1; ++s) as_is += (int64_t)K[(size_t)s] * kSlice * (is_ell[(size_t)s] ? 12 : 8);
+    std::vector<int32_t> sorted(rowof.begin() + r0, rowof.begin() + r1);
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int32_t a, int32_t b) {
       return rp[(size_t)a + 1] - rp[(size_t)a] > rp[(size_t)b + 1] - rp[(size_t)b];
     });
+    int64_t if_sorted = 0;
+    for (size_t i = 0; i < sorted.size(); i += kSlice)
+      if_sorted += (int64_t)(rp[(size_t)sorted[i] + 1] - rp[(size_t)sorted[i]]) * kSlice * 12 + 4 * kSlice;
+    if (if_sorted >= as_is) continue;
+    any_perm = true;
+    std::copy(sorted.begin(), sorted.end(), rowof.begin() + r0);
     for (int32_t s = w0; s < w1; ++s) {
       const int32_t first = rowof[(size_t)s * kSlice];
       is_ell[(size_t)s] = 1;
