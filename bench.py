@@ -37,10 +37,11 @@ def make_problem(F, env, n, order=1):
     return tria, fes, di, op
 
 
-def make_solver(F, env, op, precision, rtol):
+def make_solver(F, env, op, precision, rtol, delta=0.0):
     s = F.iterative_linear_solver_t().create(env)
     s.set_type_from_string("CG")
-    s.set_parameters_from_pl({"ILS_RTOL": rtol, "ILS_MAX_NUM_ITERATIONS": 20000, "FEMHIP_PRECISION": precision})
+    s.set_parameters_from_pl({"ILS_RTOL": rtol, "ILS_MAX_NUM_ITERATIONS": 20000, "FEMHIP_PRECISION": precision,
+                              "FEMHIP_INNER_RTOL": delta})
     s.set_operators(op.get_tangent(), F.jacobi_preconditioner_t(op))
     return s
 
@@ -222,6 +223,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--order", type=int, default=1, help="Lagrangian order of the --spmv-only problem")
+    ap.add_argument("--delta", type=float, default=0.0,
+                    help="reliable-update factor of the FP32-refined solver (0: the library default 0.1)")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -312,7 +315,7 @@ def main():
 
     mixed = None
     if not args.no_mixed:
-        s32 = make_solver(F, env, op, "FP32_REFINED", args.rtol)
+        s32 = make_solver(F, env, op, "FP32_REFINED", args.rtol, args.delta)
         for _ in range(args.warmup):
             run_steps(op, s32, 1)
         ta32, tc32, wall32, r32 = run_steps(op, s32, args.steps)
