@@ -356,7 +356,7 @@ def main():
         "setup_s": t_setup,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_spmv<double> (sliced diagonal/ELL, FP64)",
+                     "kernel": "k_spmv<double> (FP64 CG SpMV on the sliced solver copy)",
                      "kernel_ms": spmv_ms, "algorithmic_bytes": spmv_bytes},
         "mixed": mixed,
         "batched": batched,
