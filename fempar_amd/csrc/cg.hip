@@ -863,8 +863,10 @@ void prepare(femhip_solver &s) {
     const int ns = op.sl.ns;
     // about 4 slices per wavefront for the matrix kernels, multiple of 8 blocks
     // one resident wave set: about 32 waves per CU, grids a multiple of the 8 XCDs
+    // measured at 128^3: 4 blocks of 512 per CU for FP64, 2 for the FP32
+    // iterations (whose SpMV and partial reductions are cheaper per block)
     const char *bpc_env = getenv("FEMHIP_CG_BPC");
-    const int bpc = bpc_env ? std::max(1, atoi(bpc_env)) : std::max(1, 2048 / kThreads);
+    const int bpc = bpc_env ? std::max(1, atoi(bpc_env)) : std::max(1, (mx ? 1024 : 2048) / kThreads);
     const int cap = std::max(8, bpc * std::max(1, s.ctx->num_cus));
     s.g_mat = std::max(8, std::min(cap, (ns + 2 * kWaves - 1) / (2 * kWaves)));
     s.g_mat = (s.g_mat + 7) / 8 * 8;
