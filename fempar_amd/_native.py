@@ -36,7 +36,7 @@ EXPORTS = [
     "femhip_operator_set_physics", "femhip_operator_compute", "femhip_operator_get_values",
     "femhip_operator_apply", "femhip_operator_destroy",
     "femhip_solver_create", "femhip_solver_solve", "femhip_solver_get_solution", "femhip_solver_destroy",
-    "femhip_solver_profile_spmv", "femhip_solver_error_norms",
+    "femhip_solver_profile_spmv", "femhip_solver_profile_cg", "femhip_solver_error_norms",
     "femhip_batch_create", "femhip_batch_assemble", "femhip_batch_solve", "femhip_batch_get_solution",
     "femhip_batch_get_problem_matrix", "femhip_batch_autotune", "femhip_batch_destroy",
     "femhip_serve_create", "femhip_serve_estimate_bytes", "femhip_serve_submit", "femhip_serve_wait",
@@ -145,6 +145,7 @@ def load():
         "femhip_solver_solve": [P, P, P, C.POINTER(CgResult)], "femhip_solver_get_solution": [P, P],
         "femhip_solver_destroy": [P],
         "femhip_solver_profile_spmv": [P, I32, C.POINTER(D), C.POINTER(D)],
+        "femhip_solver_profile_cg": [P, C.POINTER(D), C.POINTER(I32), C.POINTER(D)],
         "femhip_solver_error_norms": [P, I32, C.POINTER(ErrorNorms)],
         "femhip_batch_create": [P, I32, C.POINTER(BatchProblem), C.POINTER(CgParams), pP],
         "femhip_batch_assemble": [P, C.POINTER(D)],

@@ -73,6 +73,7 @@ struct SlicedDev {
   const int32_t *soffs;
   int32_t order;               // traversal order of wave_slices
   const int32_t *srow;         // rows of the lanes of permuted slices
+  int32_t pstride;             // order 2: slices per step of a wave (about one plane)
 };
 
 // Row of lane `lane` of slice s (meta bit 28: the slice's rows are a sorted
@@ -146,8 +147,26 @@ __device__ __forceinline__ int xcd_block() {
 struct WaveRange {
   int s0, s1, step;
 };
-__device__ __forceinline__ WaveRange wave_slices(int ns, int order) {
+__device__ __forceinline__ WaveRange wave_slices(int ns, int order, int pstride = 0) {
   const int w = (int)(threadIdx.x >> 6);
+  if (order == 2 && pstride > 0 && (int)gridDim.x * kWaves >= pstride) {   // every column has a wave
+    // Pencils: wave (z, c) takes slices c + P j for j in its z-chunk, P = pstride
+    // (about one plane), so the lower-part reads of the symmetric layout at
+    // offset ~ -P slices touch lines the same wave loaded one step before.
+    // Logical wave ids are contiguous per XCD, so an XCD owns whole z-chunks.
+    const int P = pstride;
+    const int wid = __builtin_amdgcn_readfirstlane(xcd_block() * kWaves + w);
+    const int nw = gridDim.x * kWaves;
+    const int np = (ns + P - 1) / P;                 // steps of P slices
+    const int Z = max(1, nw / P);                    // z-chunks
+    const int L = (np + Z - 1) / Z;                  // steps per chunk
+    const int z = wid / P, c = wid % P;
+    if (z >= Z) return {0, 0, 1};
+    const int j0 = z * L, j1 = min(np, j0 + L);
+    const int s0 = c + P * j0;
+    const int s1 = min(ns, c + P * j1);
+    return {s0, max(s0, s1), P};
+  }
   if (order == 1 && gridDim.x % 8 == 0) {
     const int b = blockIdx.x, x = b % 8, per = gridDim.x / 8;
     const int wx = __builtin_amdgcn_readfirstlane((b / 8) * kWaves + w);
@@ -280,7 +299,7 @@ __global__ __launch_bounds__(kThreads) void k_init(SlicedDev m, const T *__restr
                                                    const T *__restrict__ x, int x_zero, T *__restrict__ r,
                                                    T *__restrict__ p, const T *__restrict__ invd, double *part) {
   const int lane = threadIdx.x & 63;
-  const WaveRange wr = wave_slices(m.ns, m.order);
+  const WaveRange wr = wave_slices(m.ns, m.order, m.pstride);
   const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
   double acc[4] = {0, 0, 0, 0};
   for (int s = s0; s < s1; s += sstep) {
@@ -363,7 +382,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SlicedDev m, const T *__restr
                                                    T *__restrict__ q, const CgState *S, int par, double *part) {
   if (S[par].done) return;
   const int lane = threadIdx.x & 63;
-  const WaveRange wr = wave_slices(m.ns, m.order);
+  const WaveRange wr = wave_slices(m.ns, m.order, m.pstride);
   const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
   A pq = 0, pp = 0;
   for (int s = s0; s < s1; s += sstep) {
@@ -476,7 +495,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_init(SlicedDev m, const double 
                                                       float *__restrict__ p32, float *__restrict__ d32,
                                                       const float *__restrict__ invd32, double *part) {
   const int lane = threadIdx.x & 63;
-  const WaveRange wr = wave_slices(m.ns, m.order);
+  const WaveRange wr = wave_slices(m.ns, m.order, m.pstride);
   const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
   double acc[4] = {0, 0, 0, 0};
   for (int s = s0; s < s1; s += sstep) {
@@ -542,7 +561,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_spmv(SlicedDev m, const float *
   const CgState &cur = S[par];
   if (cur.done) return;
   const int lane = threadIdx.x & 63;
-  const WaveRange wr = wave_slices(m.ns, m.order);
+  const WaveRange wr = wave_slices(m.ns, m.order, m.pstride);
   const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
   double acc[3] = {0, 0, 0};
   if (!cur.ru) {
@@ -781,7 +800,7 @@ __global__ __launch_bounds__(kThreads) void k_true_res(SlicedDev m, const double
                                                        const double *__restrict__ b, const double *__restrict__ x,
                                                        double *part) {
   const int lane = threadIdx.x & 63;
-  const WaveRange wr = wave_slices(m.ns, m.order);
+  const WaveRange wr = wave_slices(m.ns, m.order, m.pstride);
   const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
   double acc[1] = {0};
   for (int s = s0; s < s1; s += sstep) {
@@ -843,6 +862,10 @@ SlicedDev sliced_dev(femhip_operator &op) {
   m.stride = op.sl.stride;
   m.soffs = op.sl.soffs.p;
   m.srow = op.sl.srow.p;
+  // symmetric layout: pencils of one plane (the largest stored offset) per step
+  // for order 2; measured inside CG solves order 2 and order 1 run the SpMV
+  // equally fast and order 1 gives the faster FP32 iterations, so 1 is default
+  m.pstride = op.sl.sym && !op.sl.h_soffs.empty() ? std::max(1, op.sl.h_soffs.back() / kSlice) : 0;
   static const char *ord = getenv("FEMHIP_SLICE_ORDER");
   m.order = ord ? atoi(ord) : 1;
   return m;
@@ -946,30 +969,36 @@ struct Launcher {
   }
 };
 
-// Enqueue `chunk` iterations (chunk even: iteration parity restarts at 0).
-void enqueue_iterations(femhip_solver &s, int chunk, Launcher &L) {
+// Enqueue one iteration of parity `par`: SpMV, update, direction.  With
+// `ev`, events are recorded on the launcher's stream around the SpMV.
+void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = nullptr) {
   femhip_operator &op = *s.op;
   const SlicedDev m = sliced_dev(op);
   const int32_t n = s.n;
   const bool jac = s.prm.preconditioner == FEMHIP_PREC_JACOBI;
   const dim3 B(kThreads), GM(s.g_mat), GV(s.g_vec);
-  for (int i = 0; i < chunk; ++i) {
-    const int par = i & 1;
-    if (!mixed(s)) {
-      const double *invd = jac ? s.invd.p : nullptr;
-      L(k_spmv<double, double>, GM, B, m, op.sl.v64.p, s.p.p, s.q.p, s.st.p, par, s.part_s.p);
-      L(k_update<double, double>, GV, B, n, s.x.p, s.r.p, s.p.p, s.q.p, invd, s.st.p, par, s.part_s.p, s.g_mat,
-        s.part_u.p);
-      L(k_dir<double>, GV, B, n, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.hist.p, s.res.p);
-    } else {
-      const float *invd = jac ? s.invd32.p : nullptr;
-      L(k_ru_spmv, GM, B, m, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.d32.p, s.r32.p, invd,
-        s.st.p, par, s.part_s.p);
-      L(k_ru_update, GV, B, n, s.x.p, s.d32.p, s.r32.p, s.p32.p, s.q32.p, invd, s.st.p, par, s.part_s.p, s.g_mat,
-        s.part_u.p);
-      L(k_ru_dir, GV, B, n, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.res.p);
-    }
+  if (ev) FH_HIP(hipEventRecord(ev[0], L.stream));
+  if (!mixed(s)) {
+    const double *invd = jac ? s.invd.p : nullptr;
+    L(k_spmv<double, double>, GM, B, m, op.sl.v64.p, s.p.p, s.q.p, s.st.p, par, s.part_s.p);
+    if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
+    L(k_update<double, double>, GV, B, n, s.x.p, s.r.p, s.p.p, s.q.p, invd, s.st.p, par, s.part_s.p, s.g_mat,
+      s.part_u.p);
+    L(k_dir<double>, GV, B, n, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.hist.p, s.res.p);
+  } else {
+    const float *invd = jac ? s.invd32.p : nullptr;
+    L(k_ru_spmv, GM, B, m, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.d32.p, s.r32.p, invd, s.st.p,
+      par, s.part_s.p);
+    if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
+    L(k_ru_update, GV, B, n, s.x.p, s.d32.p, s.r32.p, s.p32.p, s.q32.p, invd, s.st.p, par, s.part_s.p, s.g_mat,
+      s.part_u.p);
+    L(k_ru_dir, GV, B, n, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.res.p);
   }
+}
+
+// Enqueue `chunk` iterations (chunk even: iteration parity restarts at 0).
+void enqueue_iterations(femhip_solver &s, int chunk, Launcher &L) {
+  for (int i = 0; i < chunk; ++i) enqueue_one(s, i & 1, L);
 }
 
 constexpr int kChunk = 16;
@@ -1080,6 +1109,52 @@ void solve(femhip_solver &s, bool x_zero, femhip_cg_result_t *out) {
   begin(s, x_zero);
   iterate(s, s.prm.max_num_iterations);
   read_result(s, out);
+}
+
+// One device-resident solve (b = the operator's RHS, x0 = 0) launched kernel
+// by kernel with events around every SpMV; returns the mean SpMV duration over
+// the iterations that ran (launches queued after convergence return at entry).
+void profiled_solve(femhip_solver &s, double *spmv_ms, int32_t *iterations) {
+  hipStream_t st = s.ctx->stream;
+  FH_HIP(hipMemcpyAsync(s.b.p, s.op->rhs.p, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s.n, s.x.p);
+  begin(s, true);
+  Launcher L;
+  L.stream = st;
+  std::vector<hipEvent_t> ev;
+  int launched = 0;
+  bool done = false;
+  try {
+    while (!done && launched < s.prm.max_num_iterations) {
+      for (int i = 0; i < kChunk; ++i, ++launched) {
+        hipEvent_t e[2];
+        FH_HIP(hipEventCreate(&e[0]));
+        ev.push_back(e[0]);
+        FH_HIP(hipEventCreate(&e[1]));
+        ev.push_back(e[1]);
+        enqueue_one(s, i & 1, L, e);
+      }
+      CgResult r;
+      FH_HIP(hipMemcpyAsync(&r, s.res.p, sizeof(r), hipMemcpyDeviceToHost, st));
+      FH_HIP(hipStreamSynchronize(st));
+      done = r.done != 0;
+    }
+    CgResult r;
+    FH_HIP(hipMemcpy(&r, s.res.p, sizeof(r), hipMemcpyDeviceToHost));
+    const int it = std::min(r.iter, launched);
+    double sum = 0.0;
+    for (int k = 0; k < it; ++k) {
+      float ms = 0.f;
+      FH_HIP(hipEventElapsedTime(&ms, ev[(size_t)2 * k], ev[(size_t)2 * k + 1]));
+      sum += ms;
+    }
+    *spmv_ms = it > 0 ? sum / it : 0.0;
+    *iterations = r.iter;
+  } catch (...) {
+    for (auto e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto e : ev) (void)hipEventDestroy(e);
 }
 
 double true_residual(femhip_solver &s) {
@@ -1261,6 +1336,25 @@ int femhip_solver_profile_spmv(femhip_solver_t *s, int32_t reps, double *ms_per_
       *bytes = op.sl.sym ? (double)op.sl.entries * vb + 2.0 * vb * op.n
                          : (double)op.sl.entries * vb + 16.0 * op.sl.ns + 4.0 * (double)op.sl.offs.n +
                                4.0 * (double)ell + 4.0 * kSlice * (double)op.sl.perm_slices + 2.0 * vb * op.n;
+    return FEMHIP_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
+int femhip_solver_profile_cg(femhip_solver_t *s, double *spmv_ms, int32_t *iterations, double *bytes) {
+  try {
+    FH_CHECK(s && spmv_ms && iterations, FEMHIP_ERR_INVALID, "null argument");
+    FH_HIP(hipSetDevice(s->ctx->device));
+    prepare(*s);
+    FH_CHECK(s->n > 0, FEMHIP_ERR_STATE, "empty operator");
+    profiled_solve(*s, spmv_ms, iterations);
+    s->have_solution = true;
+    double b = 0.0, ms = 0.0;
+    // algorithmic bytes: the same count as femhip_solver_profile_spmv
+    FH_CHECK(femhip_solver_profile_spmv(s, 1, &ms, &b) == FEMHIP_OK, FEMHIP_ERR_HIP, "profile_spmv failed");
+    if (bytes) *bytes = b;
     return FEMHIP_OK;
   } catch (const Error &e) {
     set_error(e.msg);

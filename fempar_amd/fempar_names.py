@@ -383,6 +383,13 @@ class iterative_linear_solver_t:
         N.check(self._lib.femhip_solver_profile_spmv(self.h, int(reps), C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def profile_cg(self):
+        """Mean SpMV duration (ms) inside one solve, iterations, algorithmic bytes per SpMV."""
+        self._ensure()
+        ms, it, by = C.c_double(), C.c_int32(), C.c_double()
+        N.check(self._lib.femhip_solver_profile_cg(self.h, C.byref(ms), C.byref(it), C.byref(by)))
+        return ms.value, it.value, by.value
+
     def get_num_iterations(self):
         return self.result["num_iterations"]
 

@@ -201,6 +201,13 @@ int femhip_solver_solve(femhip_solver_t *solver, const double *b, double *x, fem
  * row pointers + one read of p and one write of q). */
 int femhip_solver_profile_spmv(femhip_solver_t *solver, int32_t reps, double *ms_per_launch,
                                double *bytes_per_launch);
+/* Measurement aid: one device-resident solve (operator RHS, x0 = 0) launched
+ * kernel by kernel, with HIP events around every SpMV on the solver's stream;
+ * mean SpMV duration over the iterations that ran (inside the CG sequence,
+ * with the vector kernels in between), the iteration count, and the SpMV's
+ * algorithmic bytes per launch. */
+int femhip_solver_profile_cg(femhip_solver_t *solver, double *spmv_ms, int32_t *iterations,
+                             double *bytes_per_launch);
 /* copy the last solution to the host (n values) */
 int femhip_solver_get_solution(const femhip_solver_t *solver, double *x);
 
