@@ -4,8 +4,8 @@
 // cg.f90:101-238) with all its stopping criteria (cg.f90:279-407), in FP64.
 // Three kernels per iteration, each one HBM pass:
 //   spmv : q = A p ; block partials of <p,q>, ||p||^2
-//   upd  : alpha = rz/<p,q> ; x += alpha p ; r -= alpha q ; partials <r,z>, ||r||^2
-//   dir  : stopping test, beta ; p = z + beta p        (z = invD r, never stored)
+//   upd  : alpha = rz/<p,q> ; r -= alpha q ; partials <r,z>, ||r||^2
+//   dir  : stopping test, beta ; x += alpha p ; p = z + beta p   (z = invD r, never stored)
 // Scalars never leave the device.  A kernel reduces the block partials of its
 // producer at entry, every block in the same fixed order, so all blocks agree
 // bit for bit and no grid-wide fence or atomic is needed (visibility comes from
@@ -48,6 +48,10 @@ namespace {
 #endif
 constexpr int kThreads = FEMHIP_CG_THREADS;
 constexpr int kWaves = kThreads / 64;
+// Elements per thread a vector kernel loads before it reduces its producer's
+// block partials, so the two latencies overlap (the grids give about 4
+// elements per thread at 128^3).
+constexpr int kPrefetch = 4;
 
 struct CgState {
   double rz, rnrm_m, alpha, beta;
@@ -74,6 +78,7 @@ struct SlicedDev {
   int32_t order;               // traversal order of wave_slices
   const int32_t *srow;         // rows of the lanes of permuted slices
   int32_t pstride;             // order 2: slices per step of a wave (about one plane)
+  int32_t trip;                // symmetric 27-point offsets in column triplets (sym27_row)
 };
 
 // Row of lane `lane` of slice s (meta bit 28: the slice's rows are a sorted
@@ -178,20 +183,85 @@ __device__ __forceinline__ WaveRange wave_slices(int ns, int order, int pstride 
   return {(int)((int64_t)ns * wave / nw), (int)((int64_t)ns * (wave + 1) / nw), 1};
 }
 
-// Row `lane` of slice s (s wave-uniform): returns A x for NX right-hand vectors
-// x[0..NX) sharing one pass over the slice's values.  Diagonal slices use a
-// scalar base pointer per diagonal and a per-lane 32-bit byte offset, so a
-// gather is one load with no vector address arithmetic; slices whose columns
-// may leave [0, n) (flag bit 29) clamp the column.
-// Symmetric-diagonal rows: y_r = sum_k V[k][r] x[r+o_k] + sum_{k>0} V[k][r-o_k] x[r-o_k]
+// Right-hand vectors of the slice kernels.  operator()(c) is element c;
+// off(o, rb) is element row + o where rb = row * sizeof(elem): a scalar base
+// pointer per diagonal plus a per-lane 32-bit byte offset, so a gather needs no
+// vector address arithmetic.
+template <typename X>
+struct VecRef {   // x
+  using elem = X;
+  const X *__restrict__ x;
+  __device__ __forceinline__ X operator()(int c) const { return x[c]; }
+  __device__ __forceinline__ X off(int o, uint32_t rb) const { return *(const X *)((const char *)(x + o) + rb); }
+};
+struct SumRef {   // x + d: FP64 solution plus FP32 increment, the solution a reliable update makes
+  using elem = double;
+  const double *__restrict__ x;
+  const float *__restrict__ d;
+  __device__ __forceinline__ double operator()(int c) const { return x[c] + (double)d[c]; }
+  __device__ __forceinline__ double off(int o, uint32_t rb) const {
+    return *(const double *)((const char *)(x + o) + rb) + (double)*(const float *)((const char *)(d + o) + rb / 2);
+  }
+};
+
+// Row `lane` of slice s (s wave-uniform): returns (A g)_row.  Slices whose
+// columns may leave [0, n) (flag bit 29) clamp the column.
+// Symmetric-diagonal rows: y_r = sum_k V[k][r] g[r+o_k] + sum_{k>0} V[k][r-o_k] g[r-o_k]
 // (o_0 = 0).  Indices are clamped into [0, n) and out-of-range terms zeroed;
 // padding values are 0.
-template <int NX, typename T, typename A, typename X>
-__device__ __forceinline__ void sym_rows(const SlicedDev &m, const T *__restrict__ v, const X *const (&x)[NX], int s,
-                                         int lane, A (&acc)[NX]) {
+//
+// Column triplets (m.trip): the 14 stored offsets of a 27-point stencil on a
+// structured grid are 0, 1 and the triplets {d-1, d, d+1} for d = mx, P-mx, P,
+// P+mx (row length mx, plane P), so the 27 columns of a row are the triplets
+// around row + d for d = 0, +-mx, +-(P-mx), +-P, +-(P+mx).  One gather per
+// triplet loads its centre; the -1/+1 neighbours are the centres of the
+// adjacent lanes (shuffles), and the two lanes at the slice's ends take theirs
+// from wave-uniform loads.  That is 9 vector gathers of g instead of 27; the
+// terms are summed in the generic order, so the result is the same bit for bit.
+template <typename T, typename A, typename G>
+__device__ __forceinline__ A sym27_row(const SlicedDev &m, const T *__restrict__ v, const G &g, int s, int lane) {
+  using X = typename G::elem;
+  const int row0 = s * kSlice;
+  const int row = row0 + lane;
+  const int hi = m.n - 1;
+  X xv[9][3];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int d = t == 0 ? 0 : (t <= 4 ? m.soffs[3 * t] : -m.soffs[3 * (t - 4)]);
+    const X c = g(min(max(row + d, 0), hi));
+    const X hl = g(__builtin_amdgcn_readfirstlane(min(max(row0 + d - 1, 0), hi)));
+    const X hr = g(__builtin_amdgcn_readfirstlane(min(max(row0 + kSlice + d, 0), hi)));
+    const X l = __shfl_up(c, 1, 64), r = __shfl_down(c, 1, 64);
+    xv[t][0] = lane == 0 ? hl : l;
+    xv[t][1] = c;
+    xv[t][2] = lane == kSlice - 1 ? hr : r;
+  }
+  A acc = 0;
+  const T *vr = v + row;
+  // upper part: k = 0 (column row), 1 (row + 1), k >= 2: triplet (k+1)/3, member (k+1)%3
+  acc += (A)vr[0] * (A)xv[0][1];
+  acc += (A)vr[m.stride] * (A)xv[0][2];
+#pragma unroll
+  for (int k = 2; k < 14; ++k) acc += (A)vr[(int64_t)k * m.stride] * (A)xv[(k + 1) / 3][(k + 1) % 3];
+  // lower part: column row - o_k: k = 1 -> row - 1; k >= 2 -> the mirrored triplet
+#pragma unroll
+  for (int k = 1; k < 14; ++k) {
+    const int rr = row - m.soffs[k];
+    const int rc = min(max(rr, 0), hi);
+    A a = (A)v[(int64_t)k * m.stride + rc];
+    if (rr < 0) a = 0;
+    const X xx = k == 1 ? xv[0][0] : xv[(k + 1) / 3 + 4][2 - (k + 1) % 3];
+    acc += a * (A)xx;
+  }
+  return acc;
+}
+
+template <typename T, typename A, typename G>
+__device__ __forceinline__ A sym_rows(const SlicedDev &m, const T *__restrict__ v, const G &g, int s, int lane) {
+  if (m.trip) return sym27_row<T, A, G>(m, v, g, s, lane);
   const int row = s * kSlice + lane;
   const int hi = m.n - 1;
-  for (int j = 0; j < NX; ++j) acc[j] = 0;
+  A acc = 0;
   const T *vr = v + row;
   if (m.kplus == 14) {   // 27-point stencils: every load of the row in flight at once
 #pragma unroll
@@ -199,7 +269,7 @@ __device__ __forceinline__ void sym_rows(const SlicedDev &m, const T *__restrict
       const int o = m.soffs[k];
       const A a = (A)vr[(int64_t)k * m.stride];
       const int c = min(row + o, hi);
-      for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][c];
+      acc += a * (A)g(c);
     }
 #pragma unroll
     for (int k = 1; k < 14; ++k) {
@@ -208,15 +278,15 @@ __device__ __forceinline__ void sym_rows(const SlicedDev &m, const T *__restrict
       const int rc = min(max(rr, 0), hi);
       A a = (A)v[(int64_t)k * m.stride + rc];
       if (rr < 0) a = 0;
-      for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][rc];
+      acc += a * (A)g(rc);
     }
-    return;
+    return acc;
   }
   for (int k = 0; k < m.kplus; ++k) {   // diagonal and upper part
     const int o = m.soffs[k];
     const A a = (A)vr[k * m.stride];
     const int c = min(row + o, hi);
-    for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][c];
+    acc += a * (A)g(c);
   }
   for (int k = 1; k < m.kplus; ++k) {   // lower part from the stored transpose
     const int o = m.soffs[k];
@@ -224,51 +294,43 @@ __device__ __forceinline__ void sym_rows(const SlicedDev &m, const T *__restrict
     const int rc = min(max(rr, 0), hi);
     A a = (A)v[k * m.stride + rc];
     if (rr < 0) a = 0;
-    for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][rc];
+    acc += a * (A)g(rc);
   }
+  return acc;
 }
 
-template <int NX, typename T, typename A, typename X>
-__device__ __forceinline__ void slice_rows(const SlicedDev &m, const T *__restrict__ v, const X *const (&x)[NX],
-                                           int s, int lane, A (&acc)[NX]) {
-  if (m.sym) {
-    sym_rows<NX, T, A, X>(m, v, x, s, lane, acc);
-    return;
-  }
+template <typename T, typename A, typename G>
+__device__ __forceinline__ A slice_rows(const SlicedDev &m, const T *__restrict__ v, const G &g, int s, int lane) {
+  if (m.sym) return sym_rows<T, A, G>(m, v, g, s, lane);
   const int64_t b = m.base[s];
   const int meta = m.meta[s];
   const int K = meta & 0x0fffffff;
   const int row = s * kSlice + lane;
-  for (int j = 0; j < NX; ++j) acc[j] = 0;
+  A acc = 0;
   const T *vv = v + b + lane;
   if (!(meta >> 30)) {
     const int32_t *off = m.offs + m.aux[s];
     if (!((meta >> 29) & 1)) {
-      const uint32_t rb = (uint32_t)row * (uint32_t)sizeof(X);
+      const uint32_t rb = (uint32_t)row * (uint32_t)sizeof(typename G::elem);
       if (K == 27) {   // 27-point stencil slices: all loads of the slice in flight at once
 #pragma unroll
         for (int k = 0; k < 27; ++k) {
           const A a = (A)vv[k * kSlice];
-          const int o = off[k];
-          for (int j = 0; j < NX; ++j) acc[j] += a * (A)*(const X *)((const char *)(x[j] + o) + rb);
+          acc += a * (A)g.off(off[k], rb);
         }
-        return;
+        return acc;
       }
 #pragma unroll 9
       for (int k = 0; k < K; ++k) {
         const A a = (A)vv[k * kSlice];
-        const int o = off[k];
-        for (int j = 0; j < NX; ++j) {
-          const char *xk = (const char *)(x[j] + o);
-          acc[j] += a * (A)*(const X *)(xk + rb);
-        }
+        acc += a * (A)g.off(off[k], rb);
       }
     } else {
       const int hi = m.n - 1;
       for (int k = 0; k < K; ++k) {
         const A a = (A)vv[k * kSlice];
         const int c = min(max(row + off[k], 0), hi);
-        for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][c];
+        acc += a * (A)g(c);
       }
     }
   } else {
@@ -277,18 +339,16 @@ __device__ __forceinline__ void slice_rows(const SlicedDev &m, const T *__restri
     for (int k = 0; k < K; ++k) {
       const A a = (A)vv[k * kSlice];
       const int c = ec[k * kSlice];
-      for (int j = 0; j < NX; ++j) acc[j] += a * (A)x[j][c];
+      acc += a * (A)g(c);
     }
   }
+  return acc;
 }
 
 template <typename T, typename A>
 __device__ __forceinline__ A slice_row(const SlicedDev &m, const T *__restrict__ v, const T *__restrict__ x, int s,
                                        int lane) {
-  const T *const xs[1] = {x};
-  A acc[1];
-  slice_rows<1, T, A, T>(m, v, xs, s, lane, acc);
-  return acc[0];
+  return slice_rows<T, A>(m, v, VecRef<T>{x}, s, lane);
 }
 
 // ------------------------------------------------------------------ kernels ---
@@ -399,13 +459,25 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SlicedDev m, const T *__restr
   write_partials<2>(red, part);
 }
 
-// alpha ; x += alpha p ; r -= alpha q ; partials <r,z>, ||r||^2
+// alpha ; r -= alpha q ; partials <r,z>, ||r||^2 (x += alpha p is made by
+// k_dir, which reads p anyway: 32 + 48 bytes a row for the two kernels
+// instead of 56 + 32)
 template <typename T, typename A>
-__global__ __launch_bounds__(kThreads) void k_update(int32_t n, T *__restrict__ x, T *__restrict__ r,
-                                                     const T *__restrict__ p, const T *__restrict__ q,
+__global__ __launch_bounds__(kThreads) void k_update(int32_t n, T *__restrict__ r, const T *__restrict__ q,
                                                      const T *__restrict__ invd, CgState *S, int par,
                                                      const double *part_s, int gs, double *part) {
   if (S[par].done) return;
+  const int64_t i0 = (int64_t)blockIdx.x * kThreads + threadIdx.x, stride = (int64_t)gridDim.x * kThreads;
+  T rv[kPrefetch], qv[kPrefetch], dv[kPrefetch];
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k) {   // in flight while the partials are reduced
+    const int64_t i = i0 + k * stride;
+    if (i < n) {
+      rv[k] = r[i];
+      qv[k] = q[i];
+      dv[k] = invd ? invd[i] : (T)1;
+    }
+  }
   double red[2];
   reduce_partials<2>(part_s, gs, red);
   const double alpha = (red[0] != 0.0) ? S[par].rz / red[0] : 0.0;   // cg.f90:195-199
@@ -416,27 +488,43 @@ __global__ __launch_bounds__(kThreads) void k_update(int32_t n, T *__restrict__ 
   }
   const T a = (T)alpha;
   A rz = 0, rr = 0;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    x[i] = x[i] + a * p[i];
-    const T ri = r[i] + (-a) * q[i];
+  auto row = [&](int64_t i, T r0, T qi, T di) {
+    const T ri = r0 + (-a) * qi;
     r[i] = ri;
-    const T zi = invd ? invd[i] * ri : ri;
+    const T zi = invd ? di * ri : ri;
     rz += (A)ri * (A)zi;
     rr += (A)ri * (A)ri;
-  }
+  };
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k)
+    if (i0 + k * stride < n) row(i0 + k * stride, rv[k], qv[k], dv[k]);
+  for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) row(i, r[i], q[i], invd ? invd[i] : (T)1);
   double out[2] = {(double)rz, (double)rr};
   write_partials<2>(out, part);
 }
 
-// stopping test (cg.f90:323-407) ; beta ; p = z + beta p
+// stopping test (cg.f90:323-407) ; beta ; x += alpha p ; p = z + beta p
+// (the last iteration: x += alpha p only)
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_dir(int32_t n, T *__restrict__ p, const T *__restrict__ r,
-                                                  const T *__restrict__ invd, CgState *S, int par,
-                                                  const double *part_u, int gu, double *hist, CgResult *R) {
+__global__ __launch_bounds__(kThreads) void k_dir(int32_t n, T *__restrict__ x, T *__restrict__ p,
+                                                  const T *__restrict__ r, const T *__restrict__ invd, CgState *S,
+                                                  int par, const double *part_u, int gu, double *hist, CgResult *R) {
   const CgState &cur = S[par];
   if (cur.done) {   // keep `done` sticky in both parity slots
     if (blockIdx.x == 0 && threadIdx.x == 0) S[par ^ 1].done = 1;
     return;
+  }
+  const int64_t i0 = (int64_t)blockIdx.x * kThreads + threadIdx.x, stride = (int64_t)gridDim.x * kThreads;
+  T pv[kPrefetch], xv[kPrefetch], rv[kPrefetch], dv[kPrefetch];
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k) {   // in flight while the partials are reduced
+    const int64_t i = i0 + k * stride;
+    if (i < n) {
+      pv[k] = p[i];
+      xv[k] = x[i];
+      rv[k] = r[i];
+      dv[k] = invd ? invd[i] : (T)1;
+    }
   }
   double red[2];
   reduce_partials<2>(part_u, gu, red);
@@ -477,13 +565,25 @@ __global__ __launch_bounds__(kThreads) void k_dir(int32_t n, T *__restrict__ p, 
     R->converged = conv;
     R->done = done;
   }
-  if (done) return;
-  const T bt = (T)beta;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    const T zi = invd ? invd[i] * r[i] : r[i];
-    const T t = bt * p[i];
-    p[i] = t + zi;
+  const T a = (T)cur.alpha;
+  if (done) {
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k)
+      if (i0 + k * stride < n) x[i0 + k * stride] = xv[k] + a * pv[k];
+    for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) x[i] = x[i] + a * p[i];
+    return;
   }
+  const T bt = (T)beta;
+  auto row = [&](int64_t i, T pi, T xi, T ri, T di) {
+    x[i] = xi + a * pi;
+    const T zi = invd ? di * ri : ri;
+    const T t = bt * pi;
+    p[i] = t + zi;
+  };
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k)
+    if (i0 + k * stride < n) row(i0 + k * stride, pv[k], xv[k], rv[k], dv[k]);
+  for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) row(i, p[i], x[i], r[i], invd ? invd[i] : (T)1);
 }
 
 // ------------------------------------------------- FP32 with reliable updates ---
@@ -580,13 +680,12 @@ __global__ __launch_bounds__(kThreads) void k_ru_spmv(SlicedDev m, const float *
     acc[1] = pp;
   } else {
     for (int s = s0; s < s1; s += sstep) {
-      const double ax = slice_row<double, double>(m, v64, x, s, lane);
-      const float *const ds[1] = {d};
-      double ad[1];
-      slice_rows<1, double, double, float>(m, v64, ds, s, lane, ad);
+      // A (x + d) in one pass over the FP64 matrix, with x + d rounded as the
+      // update's x += d rounds it
+      const double ax = slice_rows<double, double>(m, v64, SumRef{x, d}, s, lane);
       const int row = slice_row_index(m, s, lane);
       if (row < m.n) {
-        const double ri = b[row] - ax - ad[0];
+        const double ri = b[row] - ax;
         const float rf = (float)ri;
         r32[row] = rf;
         const float zf = invd ? invd[row] * rf : rf;
@@ -599,7 +698,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_spmv(SlicedDev m, const float *
   write_partials<3>(acc, part);
 }
 
-// slot 2: alpha ; d += alpha p ; r -= alpha q (partials <r,z>, ||r||^2), or
+// slot 2: alpha ; r -= alpha q (partials <r,z>, ||r||^2), or
 // the update's second half: x += d ; d = 0 ; FP64 stopping test ; p = z + beta p
 __global__ __launch_bounds__(kThreads) void k_ru_update(int32_t n, double *__restrict__ x, float *__restrict__ d,
                                                         float *__restrict__ r, float *__restrict__ p,
@@ -610,6 +709,19 @@ __global__ __launch_bounds__(kThreads) void k_ru_update(int32_t n, double *__res
   // ru_*); they read done, ru, rz, iter, maxit and rhs_test.
   const CgState &cur = S[par];
   if (cur.done) return;
+  const int64_t i0 = (int64_t)blockIdx.x * kThreads + threadIdx.x, stride = (int64_t)gridDim.x * kThreads;
+  float rv[kPrefetch], qv[kPrefetch], dv[kPrefetch];
+  if (!cur.ru) {
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {   // in flight while the partials are reduced
+      const int64_t i = i0 + k * stride;
+      if (i < n) {
+        rv[k] = r[i];
+        qv[k] = q[i];
+        dv[k] = invd ? invd[i] : 1.0f;
+      }
+    }
+  }
   double red[3];
   reduce_partials<3>(part_s, gs, red);
   if (cur.ru) {
@@ -647,23 +759,27 @@ __global__ __launch_bounds__(kThreads) void k_ru_update(int32_t n, double *__res
   }
   const float a = (float)alpha;
   float rz = 0.f, rr = 0.f;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    d[i] = d[i] + a * p[i];
-    const float ri = r[i] + (-a) * q[i];
+  auto row = [&](int64_t i, float r0, float qi, float di) {
+    const float ri = r0 + (-a) * qi;
     r[i] = ri;
-    const float zi = invd ? invd[i] * ri : ri;
+    const float zi = invd ? di * ri : ri;
     rz += ri * zi;
     rr += ri * ri;
-  }
+  };
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k)
+    if (i0 + k * stride < n) row(i0 + k * stride, rv[k], qv[k], dv[k]);
+  for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) row(i, r[i], q[i], invd ? invd[i] : 1.0f);
   double out[2] = {(double)rz, (double)rr};
   write_partials<2>(out, part);
 }
 
-// slot 3: request a reliable update or take the CG direction step; after an
-// update, publish its results to the next parity slot
-__global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restrict__ p, const float *__restrict__ r,
-                                                     const float *__restrict__ invd, CgState *S, int par,
-                                                     const double *part_u, int gu, CgResult *R) {
+// slot 3: d += alpha p, then request a reliable update or take the CG
+// direction step; after an update, publish its results to the next parity slot
+__global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restrict__ d, float *__restrict__ p,
+                                                     const float *__restrict__ r, const float *__restrict__ invd,
+                                                     CgState *S, int par, const double *part_u, int gu,
+                                                     CgResult *R) {
   const CgState &cur = S[par];
   if (cur.done) {   // keep `done` sticky in both parity slots
     if (blockIdx.x == 0 && threadIdx.x == 0) S[par ^ 1].done = 1;
@@ -692,6 +808,18 @@ __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restric
     }
     return;
   }
+  const int64_t i0 = (int64_t)blockIdx.x * kThreads + threadIdx.x, stride = (int64_t)gridDim.x * kThreads;
+  float pv[kPrefetch], dv[kPrefetch], rv[kPrefetch], iv[kPrefetch];
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k) {   // in flight while the partials are reduced
+    const int64_t i = i0 + k * stride;
+    if (i < n) {
+      pv[k] = p[i];
+      dv[k] = d[i];
+      rv[k] = r[i];
+      iv[k] = invd ? invd[i] : 1.0f;
+    }
+  }
   double red[2];
   reduce_partials<2>(part_u, gu, red);
   const int it = cur.iter + 1;
@@ -712,13 +840,25 @@ __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restric
     }
     S[par ^ 1] = nx;
   }
-  if (ru) return;
-  const float bt = (float)beta;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    const float zi = invd ? invd[i] * r[i] : r[i];
-    const float t = bt * p[i];
-    p[i] = t + zi;
+  const float a = (float)cur.alpha;
+  if (ru) {   // the increment only; the next slot recomputes the residual
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k)
+      if (i0 + k * stride < n) d[i0 + k * stride] = dv[k] + a * pv[k];
+    for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) d[i] = d[i] + a * p[i];
+    return;
   }
+  const float bt = (float)beta;
+  auto row = [&](int64_t i, float pi, float di, float ri, float ii) {
+    d[i] = di + a * pi;
+    const float zi = invd ? ii * ri : ri;
+    const float t = bt * pi;
+    p[i] = t + zi;
+  };
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k)
+    if (i0 + k * stride < n) row(i0 + k * stride, pv[k], dv[k], rv[k], iv[k]);
+  for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) row(i, p[i], d[i], r[i], invd ? invd[i] : 1.0f);
 }
 
 // ------------------------------------------------------------------ helpers ---
@@ -868,6 +1008,13 @@ SlicedDev sliced_dev(femhip_operator &op) {
   m.pstride = op.sl.sym && !op.sl.h_soffs.empty() ? std::max(1, op.sl.h_soffs.back() / kSlice) : 0;
   static const char *ord = getenv("FEMHIP_SLICE_ORDER");
   m.order = ord ? atoi(ord) : 1;
+  // offsets 0, 1, then four triplets {d-1, d, d+1} (FEMHIP_TRIPLETS=0 disables)
+  static const char *tri = getenv("FEMHIP_TRIPLETS");
+  const std::vector<int32_t> &o = op.sl.h_soffs;
+  bool trip = op.sl.sym && op.sl.kplus == 14 && (int)o.size() == 14 && o[0] == 0 && o[1] == 1 &&
+              !(tri && atoi(tri) == 0);
+  for (int t = 1; trip && t <= 4; ++t) trip = o[3 * t - 1] + 1 == o[3 * t] && o[3 * t] + 1 == o[3 * t + 1];
+  m.trip = trip ? 1 : 0;
   return m;
 }
 
@@ -982,9 +1129,8 @@ void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = 
     const double *invd = jac ? s.invd.p : nullptr;
     L(k_spmv<double, double>, GM, B, m, op.sl.v64.p, s.p.p, s.q.p, s.st.p, par, s.part_s.p);
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
-    L(k_update<double, double>, GV, B, n, s.x.p, s.r.p, s.p.p, s.q.p, invd, s.st.p, par, s.part_s.p, s.g_mat,
-      s.part_u.p);
-    L(k_dir<double>, GV, B, n, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.hist.p, s.res.p);
+    L(k_update<double, double>, GV, B, n, s.r.p, s.q.p, invd, s.st.p, par, s.part_s.p, s.g_mat, s.part_u.p);
+    L(k_dir<double>, GV, B, n, s.x.p, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.hist.p, s.res.p);
   } else {
     const float *invd = jac ? s.invd32.p : nullptr;
     L(k_ru_spmv, GM, B, m, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.d32.p, s.r32.p, invd, s.st.p,
@@ -992,7 +1138,7 @@ void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = 
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
     L(k_ru_update, GV, B, n, s.x.p, s.d32.p, s.r32.p, s.p32.p, s.q32.p, invd, s.st.p, par, s.part_s.p, s.g_mat,
       s.part_u.p);
-    L(k_ru_dir, GV, B, n, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.res.p);
+    L(k_ru_dir, GV, B, n, s.d32.p, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.res.p);
   }
 }
 
