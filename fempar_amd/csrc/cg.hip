@@ -66,6 +66,8 @@ struct CgState {
 struct CgResult {
   double est, rhs_test, rnorm, bnorm;
   int iter, done, converged, updates;
+  double prog;                 // latest quantity the stopping test compares with rhs_test ...
+  int prog_it, pad;            // ... and its iteration (for the host's chunk planning)
 };
 
 struct SlicedDev {
@@ -434,6 +436,8 @@ __global__ __launch_bounds__(kThreads) void k_init_fin(const double *part, int g
   R->done = 0;
   R->converged = 0;
   R->updates = 0;
+  R->prog = 0.0;   // no estimate before the first iteration
+  R->prog_it = 0;
 }
 
 // q = A p ; partials <p,q>, ||p||^2
@@ -504,14 +508,20 @@ __global__ __launch_bounds__(kThreads) void k_update(int32_t n, T *__restrict__ 
 }
 
 // stopping test (cg.f90:323-407) ; beta ; x += alpha p ; p = z + beta p
-// (the last iteration: x += alpha p only)
+// (the last iteration: x += alpha p only).  Rh (the last iteration of a chunk
+// only): host-mapped copy of the result block, read by the host after the
+// chunk, so no copy kernel is needed.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_dir(int32_t n, T *__restrict__ x, T *__restrict__ p,
                                                   const T *__restrict__ r, const T *__restrict__ invd, CgState *S,
-                                                  int par, const double *part_u, int gu, double *hist, CgResult *R) {
+                                                  int par, const double *part_u, int gu, double *hist, CgResult *R,
+                                                  CgResult *Rh) {
   const CgState &cur = S[par];
   if (cur.done) {   // keep `done` sticky in both parity slots
-    if (blockIdx.x == 0 && threadIdx.x == 0) S[par ^ 1].done = 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S[par ^ 1].done = 1;
+      if (Rh) *Rh = *R;
+    }
     return;
   }
   const int64_t i0 = (int64_t)blockIdx.x * kThreads + threadIdx.x, stride = (int64_t)gridDim.x * kThreads;
@@ -564,6 +574,9 @@ __global__ __launch_bounds__(kThreads) void k_dir(int32_t n, T *__restrict__ x, 
     R->rnorm = rnorm;
     R->converged = conv;
     R->done = done;
+    R->prog = est;
+    R->prog_it = it;
+    if (Rh) *Rh = *R;
   }
   const T a = (T)cur.alpha;
   if (done) {
@@ -644,6 +657,8 @@ __global__ __launch_bounds__(kThreads) void k_ru_init_fin(const double *part, in
   R->done = s.done;
   R->converged = s.converged;
   R->updates = 0;
+  R->prog = rnorm;
+  R->prog_it = 0;
 }
 
 // One iteration slot of the FP32 path is either a CG iteration or, when the
@@ -779,10 +794,13 @@ __global__ __launch_bounds__(kThreads) void k_ru_update(int32_t n, double *__res
 __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restrict__ d, float *__restrict__ p,
                                                      const float *__restrict__ r, const float *__restrict__ invd,
                                                      CgState *S, int par, const double *part_u, int gu,
-                                                     CgResult *R) {
+                                                     CgResult *R, CgResult *Rh) {
   const CgState &cur = S[par];
   if (cur.done) {   // keep `done` sticky in both parity slots
-    if (blockIdx.x == 0 && threadIdx.x == 0) S[par ^ 1].done = 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S[par ^ 1].done = 1;
+      if (Rh) *Rh = *R;
+    }
     return;
   }
   if (cur.ru) {
@@ -805,6 +823,9 @@ __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restric
       R->converged = cur.ru_conv;
       R->done = cur.ru_done;
       R->updates = nx.updates;
+      R->prog = cur.ru_rnorm;
+      R->prog_it = cur.iter;
+      if (Rh) *Rh = *R;
     }
     return;
   }
@@ -839,6 +860,9 @@ __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restric
       nx.rmax = fmax(cur.rmax, rnorm);
     }
     S[par ^ 1] = nx;
+    R->prog = rnorm;   // the FP32 recurrence residual
+    R->prog_it = it;
+    if (Rh) *Rh = *R;
   }
   const float a = (float)cur.alpha;
   if (ru) {   // the increment only; the next slot recomputes the residual
@@ -970,10 +994,11 @@ struct femhip_solver {
   DevBuf<CgState> st;            // S[2]
   DevBuf<CgResult> res;
   DevBuf<int> flag;
-  CgResult *h_res[2] = {nullptr, nullptr};   // pinned polling slots
-  hipEvent_t ev[2] = {nullptr, nullptr};     // completion of the copy into each slot
-  hipGraphExec_t graph = nullptr;
-  int graph_kind = -1, graph_chunk = 0;
+  CgResult *h_res[2] = {nullptr, nullptr};     // pinned, device-mapped result slots
+  CgResult *d_res_map[2] = {nullptr, nullptr}; // their device addresses
+  hipEvent_t ev[2] = {nullptr, nullptr};       // completion of the chunk writing each slot
+  hipGraphExec_t graphs[2][2] = {};            // [chunk size][result slot]
+  int graph_kind = -1;
   bool prepared = false;
   bool have_solution = false;
   ~femhip_solver() {
@@ -981,7 +1006,9 @@ struct femhip_solver {
       if (h) (void)hipHostFree(h);
     for (auto e : ev)
       if (e) (void)hipEventDestroy(e);
-    if (graph) (void)hipGraphExecDestroy(graph);
+    for (auto &row : graphs)
+      for (auto g : row)
+        if (g) (void)hipGraphExecDestroy(g);
   }
 };
 
@@ -1055,7 +1082,11 @@ void prepare(femhip_solver &s) {
       s.r32.alloc(n); s.p32.alloc(n); s.q32.alloc(n); s.d32.alloc(n);
       if (jac) s.invd32.alloc(n);
     }
-    for (auto &h : s.h_res) FH_HIP(hipHostMalloc((void **)&h, sizeof(CgResult), hipHostMallocDefault));
+    for (int k = 0; k < 2; ++k) {   // written by the GPU, read by the host: mapped, coherent
+      FH_HIP(hipHostMalloc((void **)&s.h_res[k], sizeof(CgResult), hipHostMallocMapped | hipHostMallocCoherent));
+      std::memset(s.h_res[k], 0, sizeof(CgResult));
+      FH_HIP(hipHostGetDevicePointer((void **)&s.d_res_map[k], s.h_res[k], 0));
+    }
     for (auto &e : s.ev) FH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     s.prepared = true;
   }
@@ -1117,8 +1148,9 @@ struct Launcher {
 };
 
 // Enqueue one iteration of parity `par`: SpMV, update, direction.  With
-// `ev`, events are recorded on the launcher's stream around the SpMV.
-void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = nullptr) {
+// `ev`, events are recorded on the launcher's stream around the SpMV; with
+// `rh`, the direction kernel also writes the result block there.
+void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = nullptr, CgResult *rh = nullptr) {
   femhip_operator &op = *s.op;
   const SlicedDev m = sliced_dev(op);
   const int32_t n = s.n;
@@ -1130,7 +1162,7 @@ void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = 
     L(k_spmv<double, double>, GM, B, m, op.sl.v64.p, s.p.p, s.q.p, s.st.p, par, s.part_s.p);
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
     L(k_update<double, double>, GV, B, n, s.r.p, s.q.p, invd, s.st.p, par, s.part_s.p, s.g_mat, s.part_u.p);
-    L(k_dir<double>, GV, B, n, s.x.p, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.hist.p, s.res.p);
+    L(k_dir<double>, GV, B, n, s.x.p, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.hist.p, s.res.p, rh);
   } else {
     const float *invd = jac ? s.invd32.p : nullptr;
     L(k_ru_spmv, GM, B, m, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.d32.p, s.r32.p, invd, s.st.p,
@@ -1138,74 +1170,111 @@ void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = 
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
     L(k_ru_update, GV, B, n, s.x.p, s.d32.p, s.r32.p, s.p32.p, s.q32.p, invd, s.st.p, par, s.part_s.p, s.g_mat,
       s.part_u.p);
-    L(k_ru_dir, GV, B, n, s.d32.p, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.res.p);
+    L(k_ru_dir, GV, B, n, s.d32.p, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.res.p, rh);
   }
 }
 
-// Enqueue `chunk` iterations (chunk even: iteration parity restarts at 0).
-void enqueue_iterations(femhip_solver &s, int chunk, Launcher &L) {
-  for (int i = 0; i < chunk; ++i) enqueue_one(s, i & 1, L);
-}
+// Chunks of iterations (even sizes: iteration parity restarts at 0): kChunk
+// while convergence is far, kChunkTail near the end, so that few iterations
+// are queued past convergence (each still costs its three launches).
+constexpr int kChunk = 16, kChunkTail = 4;
+constexpr int kChunkSizes[2] = {kChunk, kChunkTail};
 
-constexpr int kChunk = 16;
+// Enqueue a chunk; its last direction kernel writes the result block to `rh`.
+void enqueue_iterations(femhip_solver &s, int chunk, Launcher &L, CgResult *rh) {
+  for (int i = 0; i < chunk; ++i) enqueue_one(s, i & 1, L, nullptr, i == chunk - 1 ? rh : nullptr);
+}
 
 bool use_graph() {
   static const bool no_graph = getenv("FEMHIP_NO_GRAPH") != nullptr;
   return !no_graph;
 }
 
-// build (once per precision) the chunk of kChunk iterations as a hipGraph
-void ensure_graph(femhip_solver &s) {
+// build (once per precision, chunk size and result slot) the chunk as a hipGraph
+void ensure_graph(femhip_solver &s, int size_idx, int slot) {
   const int kind = mixed(s) ? 1 : 0;
-  if (!use_graph() || (s.graph && s.graph_kind == kind)) return;
-  if (s.graph) FH_HIP(hipGraphExecDestroy(s.graph));
-  s.graph = nullptr;
+  if (!use_graph()) return;
+  if (s.graph_kind != kind) {
+    for (auto &row : s.graphs)
+      for (auto &g : row)
+        if (g) {
+          FH_HIP(hipGraphExecDestroy(g));
+          g = nullptr;
+        }
+    s.graph_kind = kind;
+  }
+  hipGraphExec_t &g = s.graphs[size_idx][slot];
+  if (g) return;
   Launcher L;
   FH_HIP(hipGraphCreate(&L.graph, 0));
   try {
-    enqueue_iterations(s, kChunk, L);
-    FH_HIP(hipGraphInstantiate(&s.graph, L.graph, nullptr, nullptr, 0));
+    enqueue_iterations(s, kChunkSizes[size_idx], L, s.d_res_map[slot]);
+    FH_HIP(hipGraphInstantiate(&g, L.graph, nullptr, nullptr, 0));
   } catch (...) {
     (void)hipGraphDestroy(L.graph);
     throw;
   }
   FH_HIP(hipGraphDestroy(L.graph));
-  s.graph_kind = kind;
 }
 
-void launch_chunk(femhip_solver &s) {
+// one chunk; its last kernel writes the result block into pinned slot `slot`
+// (mapped host memory), and the slot's event marks the chunk's completion
+void push_chunk(femhip_solver &s, int slot, int size_idx = 0) {
+  hipStream_t st = s.ctx->stream;
   if (!use_graph()) {
     Launcher L;
-    L.stream = s.ctx->stream;
-    enqueue_iterations(s, kChunk, L);
-    return;
+    L.stream = st;
+    enqueue_iterations(s, kChunkSizes[size_idx], L, s.d_res_map[slot]);
+  } else {
+    ensure_graph(s, size_idx, slot);
+    FH_HIP(hipGraphLaunch(s.graphs[size_idx][slot], st));
   }
-  ensure_graph(s);
-  FH_HIP(hipGraphLaunch(s.graph, s.ctx->stream));
-}
-
-// one chunk, then an asynchronous copy of the result block into pinned slot
-void push_chunk(femhip_solver &s, int slot) {
-  hipStream_t st = s.ctx->stream;
-  launch_chunk(s);
-  FH_HIP(hipMemcpyAsync(s.h_res[slot], s.res.p, sizeof(CgResult), hipMemcpyDeviceToHost, st));
   FH_HIP(hipEventRecord(s.ev[slot], st));
 }
 
-// launch chunks until the device state says done; one chunk of look-ahead
+// Launch chunks until the device state says done, keeping one chunk of
+// look-ahead.  Near the end the remaining iteration count is extrapolated from
+// the stopping quantity at the last two chunk ends (a geometric rate): chunks
+// shrink to kChunkTail, and none is added while the one in flight is expected
+// to converge.  The plan affects only how many early-exit launches follow
+// convergence, never the iterates.
 void iterate(femhip_solver &s, int maxit) {
-  int queued = 0, cur = 0;
-  push_chunk(s, 0);
-  queued += kChunk;
-  for (;;) {
-    const bool more = queued < maxit;
-    if (more) {
-      push_chunk(s, cur ^ 1);
-      queued += kChunk;
+  struct Out {
+    int slot, size;
+  } q[2];
+  int head = 0, nq = 0, next_slot = 0;
+  int64_t queued = 0;
+  auto push = [&](int size_idx) {
+    push_chunk(s, next_slot, size_idx);
+    q[(head + nq) & 1] = {next_slot, kChunkSizes[size_idx]};
+    ++nq;
+    next_slot ^= 1;
+    queued += kChunkSizes[size_idx];
+  };
+  push(0);
+  if (queued < maxit) push(0);
+  double prev = 0.0;
+  int prev_it = -1;
+  while (nq > 0) {
+    const Out c = q[head];
+    head ^= 1;
+    --nq;
+    FH_HIP(hipEventSynchronize(s.ev[c.slot]));
+    const CgResult r = *s.h_res[c.slot];
+    if (r.done) break;
+    if (queued >= maxit) continue;   // budget queued: wait for the rest
+    double need = 1e30;              // iterations still needed
+    if (prev_it >= 0 && r.prog_it > prev_it && r.prog > 0.0 && prev > r.prog && r.rhs_test > 0.0) {
+      const double rate = std::log(r.prog / prev) / (r.prog_it - prev_it);
+      need = r.prog <= r.rhs_test ? 1.0 : std::log(r.rhs_test / r.prog) / rate + 4.0;   // 4: margin
     }
-    FH_HIP(hipEventSynchronize(s.ev[cur]));
-    if (s.h_res[cur]->done || !more) break;
-    cur ^= 1;
+    if (r.prog > 0.0) {
+      prev = r.prog;
+      prev_it = r.prog_it;
+    }
+    const double rest = need - (double)(queued - r.prog_it);   // beyond what is queued
+    if (nq > 0 && rest <= 0.0) continue;
+    push(rest > kChunk ? 0 : 1);
   }
   FH_HIP(hipStreamSynchronize(s.ctx->stream));
 }
@@ -1330,7 +1399,7 @@ void solver_start(femhip_solver &s) {
   FH_HIP(hipMemcpyAsync(s.b.p, s.op->rhs.p, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s.n, s.x.p);
   begin(s, true);
-  ensure_graph(s);
+  for (int slot = 0; slot < 2; ++slot) ensure_graph(s, 0, slot);
 }
 
 void solver_push(femhip_solver &s, int slot) { push_chunk(s, slot); }

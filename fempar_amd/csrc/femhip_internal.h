@@ -232,8 +232,8 @@ void error_norms(femhip_operator &op, const double *d_x, int exact, double out[4
 // non-blocking CG driver (cg.hip) used by the request server: solver_start
 // prepares the solver format and enqueues the initial residual for b = the
 // operator's RHS and x0 = 0; solver_push enqueues one chunk of solver_chunk()
-// iterations followed by a copy of the result block into pinned slot 0 or 1;
-// solver_poll returns 0 while that copy is pending, else 2 once the device
+// iterations whose last kernel writes the result block into pinned slot 0 or 1;
+// solver_poll returns 0 while that chunk is running, else 2 once the device
 // state says done and 1 otherwise; solver_finish waits for the stream.
 int solver_chunk();
 void solver_start(femhip_solver &s);
