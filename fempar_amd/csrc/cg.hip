@@ -382,8 +382,10 @@ __global__ __launch_bounds__(kThreads) void k_init(SlicedDev m, const T *__restr
   write_partials<4>(acc, part);
 }
 
-// one block: init_convergence_data (cg.f90:279-321)
-__global__ __launch_bounds__(kThreads) void k_init_fin(const double *part, int g, CgState *S, CgResult *R) {
+// one block: init_convergence_data (cg.f90:279-321).  A raised zero-pivot flag
+// of the Jacobi setup ends the solve before its first iteration.
+__global__ __launch_bounds__(kThreads) void k_init_fin(const double *part, int g, CgState *S, CgResult *R,
+                                                       const int *zero_diag) {
   double v[4];
   reduce_partials<4>(part, g, v);
   if (threadIdx.x != 0) return;
@@ -421,7 +423,7 @@ __global__ __launch_bounds__(kThreads) void k_init_fin(const double *part, int g
   s.rhs_test = t;
   s.rhs_extra = extra;
   s.iter = 0;
-  s.done = 0;
+  s.done = (zero_diag && *zero_diag) ? 1 : 0;
   s.converged = 0;
   s.est = 0.0;
   s.ru = 0;
@@ -433,7 +435,7 @@ __global__ __launch_bounds__(kThreads) void k_init_fin(const double *part, int g
   R->rnorm = rnorm;
   R->bnorm = bnorm;
   R->iter = 0;
-  R->done = 0;
+  R->done = s.done;
   R->converged = 0;
   R->updates = 0;
   R->prog = 0.0;   // no estimate before the first iteration
@@ -631,7 +633,8 @@ __global__ __launch_bounds__(kThreads) void k_ru_init(SlicedDev m, const double 
   write_partials<4>(acc, part);
 }
 
-__global__ __launch_bounds__(kThreads) void k_ru_init_fin(const double *part, int g, CgState *S, CgResult *R) {
+__global__ __launch_bounds__(kThreads) void k_ru_init_fin(const double *part, int g, CgState *S, CgResult *R,
+                                                          const int *zero_diag) {
   double v[4];
   reduce_partials<4>(part, g, v);
   if (threadIdx.x != 0) return;
@@ -645,7 +648,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_init_fin(const double *part, in
   s.rmax = sqrt(v[3]);
   s.iter = 0;
   s.converged = rnorm <= s.rhs_test;
-  s.done = s.converged;
+  s.done = s.converged || (zero_diag && *zero_diag);
   s.ru = 0;
   s.updates = 0;
   S[1] = s;
@@ -938,6 +941,62 @@ __global__ void k_jacobi_sliced(SlicedDev m, const double *__restrict__ v, doubl
   if (invd32) invd32[i] = (float)inv;
 }
 
+// Symmetric-diagonal layout from the CSR: the entries of row r with col >= r
+// go to V[k][r] for col - r = soffs[k] (both ascending); missing diagonals and
+// rows in [n, stride) get 0.  One 64-thread block per 64 rows: the rows' CSR
+// run (at most 64 x maxlen entries, maxlen = 2 kplus - 1 distinct offsets) is
+// read coalesced into LDS, then each thread walks its row there and every
+// store of one diagonal is coalesced.
+constexpr int kFillRows = 64;
+__global__ __launch_bounds__(kFillRows) void k_fill_sym(int32_t n, const int32_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ col,
+                                                        const double *__restrict__ val,
+                                                        const int32_t *__restrict__ soffs, int kplus, int64_t stride,
+                                                        double *__restrict__ v64, float *__restrict__ v32) {
+  extern __shared__ double fill_lds[];
+  const int maxlen = 2 * kplus - 1;
+  double *sv = fill_lds;
+  int32_t *sc = (int32_t *)(fill_lds + kFillRows * maxlen);
+  const int64_t r0 = (int64_t)blockIdx.x * kFillRows;
+  const int lane = threadIdx.x;
+  const int64_t r = r0 + lane;
+  int32_t e0 = 0, cnt = 0;
+  if (r0 < n) {
+    e0 = rowptr[r0];
+    cnt = rowptr[min(r0 + kFillRows, (int64_t)n)] - e0;
+  }
+  for (int j = lane; j < cnt; j += kFillRows) {
+    sc[j] = col[e0 + j];
+    sv[j] = val[e0 + j];
+  }
+  __syncthreads();
+  int32_t j = 0, e = 0;
+  if (r < n) {
+    j = rowptr[r] - e0;
+    e = rowptr[r + 1] - e0;
+    while (j < e && sc[j] < r) ++j;
+  }
+  for (int k = 0; k < kplus; ++k) {
+    double a = 0.0;
+    if (j < e && sc[j] - (int32_t)r == soffs[k]) a = sv[j++];
+    if (v64) v64[(int64_t)k * stride + r] = a;
+    if (v32) v32[(int64_t)k * stride + r] = (float)a;
+  }
+}
+
+// Jacobi setup from the symmetric-diagonal copy: diagonal 0 is offset 0.
+// A zero pivot raises the flag (mapped host memory, read after the solve).
+__global__ void k_jacobi_sym(int32_t n, const double *__restrict__ v, double *__restrict__ invd,
+                             float *__restrict__ invd32, int *zero_diag) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = v[i];
+  if (d == 0.0) *zero_diag = 1;
+  const double inv = 1.0 / d;
+  invd[i] = inv;
+  if (invd32) invd32[i] = (float)inv;
+}
+
 // gather CSR values into the sliced layout (FP64 and/or FP32)
 __global__ void k_fill_sliced(int64_t E, const int32_t *__restrict__ map, const double *__restrict__ val,
                               double *__restrict__ v64, float *__restrict__ v32) {
@@ -993,7 +1052,7 @@ struct femhip_solver {
   DevBuf<float> r32, p32, q32, d32, invd32;
   DevBuf<CgState> st;            // S[2]
   DevBuf<CgResult> res;
-  DevBuf<int> flag;
+  int *h_flag = nullptr, *d_flag = nullptr;    // zero-pivot flag of the Jacobi setup (pinned, mapped)
   CgResult *h_res[2] = {nullptr, nullptr};     // pinned, device-mapped result slots
   CgResult *d_res_map[2] = {nullptr, nullptr}; // their device addresses
   hipEvent_t ev[2] = {nullptr, nullptr};       // completion of the chunk writing each slot
@@ -1004,6 +1063,7 @@ struct femhip_solver {
   ~femhip_solver() {
     for (auto *h : h_res)
       if (h) (void)hipHostFree(h);
+    if (h_flag) (void)hipHostFree(h_flag);
     for (auto e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto &row : graphs)
@@ -1076,7 +1136,6 @@ void prepare(femhip_solver &s) {
     s.hist.alloc((size_t)std::max(1, s.prm.max_num_iterations));
     s.st.alloc(2);
     s.res.alloc(1);
-    s.flag.alloc(1);
     if (jac) s.invd.alloc(n);
     if (mx) {
       s.r32.alloc(n); s.p32.alloc(n); s.q32.alloc(n); s.d32.alloc(n);
@@ -1087,6 +1146,9 @@ void prepare(femhip_solver &s) {
       std::memset(s.h_res[k], 0, sizeof(CgResult));
       FH_HIP(hipHostGetDevicePointer((void **)&s.d_res_map[k], s.h_res[k], 0));
     }
+    FH_HIP(hipHostMalloc((void **)&s.h_flag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    *s.h_flag = 0;
+    FH_HIP(hipHostGetDevicePointer((void **)&s.d_flag, s.h_flag, 0));
     for (auto &e : s.ev) FH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     s.prepared = true;
   }
@@ -1094,26 +1156,39 @@ void prepare(femhip_solver &s) {
   if (op.sl.filled_version != op.version || (mx && op.sl.filled_version32 != op.version)) {
     if (!op.sl.v64.p) op.sl.v64.alloc((size_t)std::max<int64_t>(1, op.sl.entries));
     if (mx && !op.sl.v32.p) op.sl.v32.alloc((size_t)std::max<int64_t>(1, op.sl.entries));
-    hipLaunchKernelGGL(k_fill_sliced, dim3(2048), dim3(256), 0, st, op.sl.entries, op.sl.map.p, op.val.p,
-                       op.sl.v64.p, mx ? op.sl.v32.p : nullptr);
+    if (op.sl.sym) {   // stride is a multiple of kFillRows; every row has at most 2 kplus - 1 entries
+      const size_t lds = (size_t)kFillRows * (2 * op.sl.kplus - 1) * (sizeof(double) + sizeof(int32_t));
+      hipLaunchKernelGGL(k_fill_sym, dim3((unsigned)(op.sl.stride / kFillRows)), dim3(kFillRows), lds, st, n,
+                         op.rowptr.p, op.col.p, op.val.p, op.sl.soffs.p, op.sl.kplus, op.sl.stride, op.sl.v64.p,
+                         mx ? op.sl.v32.p : nullptr);
+    } else
+      hipLaunchKernelGGL(k_fill_sliced, dim3(2048), dim3(256), 0, st, op.sl.entries, op.sl.map.p, op.val.p,
+                         op.sl.v64.p, mx ? op.sl.v32.p : nullptr);
     FH_HIP(hipGetLastError());
     op.sl.filled_version = op.version;
     if (mx) op.sl.filled_version32 = op.version;
   }
+  // Jacobi setup without a host round trip: a zero pivot raises s.h_flag,
+  // which ends the solve at its initial state and is reported by read_result
   if (jac && n > 0) {
-    int zero = 0;
-    FH_HIP(hipMemcpyAsync(s.flag.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
-    if (op.structured)   // no CSR: the diagonal comes from the sliced copy
-      hipLaunchKernelGGL(k_jacobi_sliced, dim3((n + 255) / 256), dim3(256), 0, st, sliced_dev(op), op.sl.v64.p,
-                         s.invd.p, mx ? s.invd32.p : nullptr, s.flag.p);
+    *s.h_flag = 0;
+    const dim3 G((n + 255) / 256), B(256);
+    if (op.sl.sym)
+      hipLaunchKernelGGL(k_jacobi_sym, G, B, 0, st, n, op.sl.v64.p, s.invd.p, mx ? s.invd32.p : nullptr, s.d_flag);
+    else if (op.structured)   // no CSR: the diagonal comes from the sliced copy
+      hipLaunchKernelGGL(k_jacobi_sliced, G, B, 0, st, sliced_dev(op), op.sl.v64.p, s.invd.p,
+                         mx ? s.invd32.p : nullptr, s.d_flag);
     else
-      hipLaunchKernelGGL(k_jacobi, dim3((n + 255) / 256), dim3(256), 0, st, n, op.rowptr.p, op.col.p, op.val.p,
-                         s.invd.p, mx ? s.invd32.p : nullptr, s.flag.p);
+      hipLaunchKernelGGL(k_jacobi, G, B, 0, st, n, op.rowptr.p, op.col.p, op.val.p, s.invd.p,
+                         mx ? s.invd32.p : nullptr, s.d_flag);
     FH_HIP(hipGetLastError());
-    FH_HIP(hipMemcpyAsync(&zero, s.flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipStreamSynchronize(st));
-    FH_CHECK(!zero, FEMHIP_ERR_ZERO_DIAGONAL, "op1 cannot be entrywise inverted; there exists a null diagonal entry");
   }
+}
+
+// after the stream finished: the Jacobi setup's zero-pivot report
+void check_pivots(const femhip_solver &s) {
+  FH_CHECK(!(s.prm.preconditioner == FEMHIP_PREC_JACOBI && s.h_flag && *s.h_flag), FEMHIP_ERR_ZERO_DIAGONAL,
+           "op1 cannot be entrywise inverted; there exists a null diagonal entry");
 }
 
 // Launches a kernel on a stream, or appends it as a kernel node after the
@@ -1304,7 +1379,7 @@ void begin(femhip_solver &s, bool x_zero) {
                        s.r.p, s.r32.p, s.p32.p, s.d32.p, jac ? s.invd32.p : nullptr, s.part_i.p);
   }
   hipLaunchKernelGGL(mixed(s) ? k_ru_init_fin : k_init_fin, dim3(1), dim3(kThreads), 0, st, s.part_i.p, s.g_mat,
-                     s.st.p, s.res.p);
+                     s.st.p, s.res.p, (const int *)(jac ? s.d_flag : nullptr));
   FH_HIP(hipGetLastError());
 }
 
@@ -1312,6 +1387,7 @@ void begin(femhip_solver &s, bool x_zero) {
 void read_result(femhip_solver &s, femhip_cg_result_t *out) {
   CgResult r;
   FH_HIP(hipMemcpy(&r, s.res.p, sizeof(r), hipMemcpyDeviceToHost));
+  check_pivots(s);
   out->num_iterations = r.iter;
   out->did_converge = r.converged;
   out->outer_iterations = mixed(s) ? r.updates : 0;
@@ -1425,7 +1501,7 @@ void solver_copy_solution(const femhip_solver &s, double *x) {
 }
 
 size_t solver_device_bytes(const femhip_solver &s) {
-  size_t b = s.st.bytes() + s.res.bytes() + s.flag.bytes();
+  size_t b = s.st.bytes() + s.res.bytes();
   for (auto *v : {&s.x, &s.b, &s.r, &s.p, &s.q, &s.invd, &s.part_i, &s.part_s, &s.part_u, &s.part_r, &s.hist})
     b += v->bytes();
   for (auto *v : {&s.r32, &s.p32, &s.q32, &s.d32, &s.invd32}) b += v->bytes();

@@ -154,7 +154,7 @@ struct Sliced {
   DevBuf<int32_t> aux;            // ns: offset into offs (diagonal) or ecol (ELL)
   DevBuf<int32_t> offs;           // diagonal offsets
   DevBuf<int32_t> ecol;           // ELL column indices
-  DevBuf<int32_t> map;            // entries: CSR position, -1 for padding
+  DevBuf<int32_t> map;            // entries: CSR position, -1 for padding (not used by the symmetric mode)
   DevBuf<double> v64;
   DevBuf<float> v32;
   int64_t filled_version = -1, filled_version32 = -1;

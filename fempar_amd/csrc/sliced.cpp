@@ -66,17 +66,7 @@ bool try_build_symmetric_dia(femhip_operator &op) {
   const int64_t E = (int64_t)up.size() * stride;
   const int64_t stored = ((int64_t)cl.size() + n) / 2;
   if (E > stored * 3 / 2 + 64 || E >= (int64_t)INT32_MAX) return false;
-  std::vector<int32_t> map((size_t)E, -1);
-  parallel_rows(n, [&](unsigned, int64_t lo, int64_t hi) {
-    for (int64_t r = lo; r < hi; ++r)
-      for (size_t k = 0; k < up.size(); ++k) {
-        const int64_t c = r + up[k];
-        if (c >= n) break;
-        auto b = cl.begin() + rp[(size_t)r], e = cl.begin() + rp[(size_t)r + 1];
-        auto it = std::lower_bound(b, e, (int32_t)c);
-        if (it != e && *it == c) map[(size_t)(k * stride + r)] = (int32_t)(it - cl.begin());
-      }
-  });
+  // values are filled from the CSR row by row on the device (no entry map)
   S.sym = true;
   S.kplus = (int32_t)up.size();
   S.stride = stride;
@@ -86,9 +76,8 @@ bool try_build_symmetric_dia(femhip_operator &op) {
   S.dia_slices = S.ns;
   S.soffs.alloc(up.size());
   FH_HIP(hipMemcpy(S.soffs.p, up.data(), sizeof(int32_t) * up.size(), hipMemcpyHostToDevice));
-  S.map.alloc((size_t)E);
-  FH_HIP(hipMemcpy(S.map.p, map.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
   // unused sliced arrays stay valid 1-element buffers
+  S.map.alloc(1);
   S.base.alloc(1);
   S.meta.alloc(1);
   S.aux.alloc(1);
