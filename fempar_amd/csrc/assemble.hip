@@ -24,6 +24,7 @@
 // most one thread per launch.  Both are free of atomics and deterministic.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "femhip_internal.h"
@@ -292,6 +293,26 @@ __global__ void k_gather_slots(VtxGrid g, int32_t n, const int32_t *__restrict__
   if (!ok) atomicOr(bad, 1);
 }
 
+// Every present neighbour `code` of every row must sit at column row + off[code].
+struct CodeOffsets {
+  int32_t off[27];
+};
+template <int DIM>
+__global__ void k_check_code_offsets(int32_t n, const uint8_t *__restrict__ slot, const int32_t *__restrict__ rowptr,
+                                     const int32_t *__restrict__ col, CodeOffsets co, int *__restrict__ bad) {
+  constexpr int K = DIM == 2 ? 9 : 27;
+  constexpr int KS = DIM == 2 ? 16 : 32;
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int32_t b = rowptr[r];
+  bool ok = true;
+  for (int k = 0; k < K; ++k) {
+    const uint8_t s = slot[(int64_t)r * KS + k];
+    if (s != 255) ok = ok && col[b + s] - r == co.off[k];
+  }
+  if (!ok) atomicOr(bad, 1);
+}
+
 // On a uniform mesh the d-th coordinate of quadrature point q of cell (i,j,k)
 // depends only on (i_d, the d-th 1-D Gauss index of q), so the trigonometric
 // factors of the coefficient and source are tabulated per direction:
@@ -333,6 +354,16 @@ __device__ __forceinline__ void coef_tab(const PhysDev &ph, const TrigTab &tt, c
   }
 }
 
+// The solver's symmetric-diagonal copy, written by the assembly as well when
+// v64 is set: V[kidx[code]][row] = the row's value for neighbour `code` (0 for
+// a fixed neighbour), for the codes of the stored (upper) diagonals.
+struct SymOut {
+  double *v64;
+  float *v32;
+  int64_t stride;
+  int8_t kidx[32];
+};
+
 // One thread per free row; a block owns RB consecutive rows, whose CSR values
 // are contiguous: they are staged in LDS and stored with coalesced writes.
 template <int DIM, int RB>
@@ -340,7 +371,7 @@ __global__ __launch_bounds__(RB) void k_assemble_gather(
     VtxGrid g, PhysDev ph, TrigTab tt, int32_t n, const int32_t *__restrict__ row_vtx,
     const int32_t *__restrict__ vtx_dof, const uint8_t *__restrict__ slot, const int32_t *__restrict__ rowptr,
     const double *__restrict__ tab, const double *__restrict__ fixed, double *__restrict__ val,
-    double *__restrict__ rhs) {
+    double *__restrict__ rhs, SymOut so) {
   constexpr int ND = 1 << DIM, NQ = ND, K = DIM == 2 ? 9 : 27, KS = DIM == 2 ? 16 : 32;
   __shared__ double sv[RB * K];
   const double *M = tab;                 // M[NQ][ND][ND]
@@ -397,6 +428,17 @@ __global__ __launch_bounds__(RB) void k_assemble_gather(
     }
   }
   rhs[r] = bacc;
+  if (so.v64) {   // stores of one diagonal are coalesced over the block's rows
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int d = so.kidx[k];
+      if (d >= 0) {
+        const double v = sr[k] != 255 ? acc[k] : 0.0;
+        so.v64[d * so.stride + r] = v;
+        if (so.v32) so.v32[d * so.stride + r] = (float)v;
+      }
+    }
+  }
   }
   __syncthreads();
   for (int32_t e = threadIdx.x; e < b1 - b0; e += RB) val[b0 + e] = sv[e];   // rows <= K entries each
@@ -531,18 +573,90 @@ void setup_gather(femhip_operator &op) {
 
 constexpr int kGatherRows = 128;
 
+}  // namespace
+
+// The owner-computes rows can fill the symmetric-diagonal solver copy too when
+// the column offset of each neighbour code is the same on every row (the
+// structured numberings for which that copy exists).  The offsets are read off
+// one row that has all its neighbours, mapped onto the stored diagonals (each
+// used by exactly one code) and checked on every row on the device.
+void setup_sym_gather(femhip_operator &op) {
+  if (op.sym_gather != 0) return;
+  op.sym_gather = -1;
+  if (op.gather != 1 || !op.sl.sym || op.n <= 0) return;
+  const int dim = op.space->fe.dim;
+  const int K = dim == 2 ? 9 : 27, KS = dim == 2 ? 16 : 32;
+  int32_t off[27] = {};
+  bool found = false;
+  for (const int64_t r : {(int64_t)op.n / 2, (int64_t)op.n / 3, 2 * (int64_t)op.n / 3}) {
+    uint8_t sl[32];
+    FH_HIP(hipMemcpy(sl, op.slot.p + r * KS, (size_t)KS, hipMemcpyDeviceToHost));
+    bool full = true;
+    for (int k = 0; k < K; ++k) full = full && sl[k] != 255;
+    if (!full) continue;
+    for (int k = 0; k < K; ++k) off[k] = op.h_col[(size_t)op.h_rowptr[(size_t)r] + sl[k]] - (int32_t)r;
+    found = true;
+    break;
+  }
+  if (!found) return;
+  const std::vector<int32_t> &so = op.sl.h_soffs;
+  uint64_t seen = 0;
+  for (int k = 0; k < 32; ++k) op.sym_kidx[k] = -1;
+  for (int k = 0; k < K; ++k) {
+    if (off[k] < 0) continue;
+    const auto it = std::find(so.begin(), so.end(), off[k]);
+    if (it == so.end()) return;
+    const int d = (int)(it - so.begin());
+    if (d >= 64 || ((seen >> d) & 1)) return;
+    seen |= 1ull << d;
+    op.sym_kidx[k] = (int8_t)d;
+  }
+  if (so.size() > 64 || seen != (so.size() == 64 ? ~0ull : (1ull << so.size()) - 1)) return;
+  CodeOffsets co{};
+  for (int k = 0; k < K; ++k) co.off[k] = off[k];
+  hipStream_t st = op.ctx->stream;
+  DevBuf<int> bad;
+  bad.alloc(1);
+  FH_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
+  const unsigned rb = (unsigned)((op.n + 255) / 256);
+  if (dim == 2)
+    hipLaunchKernelGGL(k_check_code_offsets<2>, dim3(rb), dim3(256), 0, st, op.n, op.slot.p, op.rowptr.p, op.col.p,
+                       co, bad.p);
+  else
+    hipLaunchKernelGGL(k_check_code_offsets<3>, dim3(rb), dim3(256), 0, st, op.n, op.slot.p, op.rowptr.p, op.col.p,
+                       co, bad.p);
+  FH_HIP(hipGetLastError());
+  int h_bad = 1;
+  FH_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  FH_HIP(hipStreamSynchronize(st));
+  if (!h_bad) op.sym_gather = 1;
+}
+
+namespace {
+
 void launch_gather(femhip_operator &op, const PhysDev &ph, const double *d_fixed) {
   femhip_space &s = *op.space;
   hipStream_t st = op.ctx->stream;
   const VtxGrid g = vtx_grid(s.mesh);
   const TrigTab tt{reinterpret_cast<const double4 *>(op.trig.p), {op.trig_off[0], op.trig_off[1], op.trig_off[2]}};
   const unsigned rb = (unsigned)((op.n + kGatherRows - 1) / kGatherRows);
+  // the solver copies that exist (FP64, and FP32 once a refined solver made
+  // it) are written too; their padding rows stay as the first fill left them
+  SymOut so{};
+  op.sym_written = 0;
+  if (op.sym_gather == 1 && op.sl.v64.p) {
+    so.v64 = op.sl.v64.p;
+    so.v32 = op.sl.v32.p;
+    so.stride = op.sl.stride;
+    for (int k = 0; k < 32; ++k) so.kidx[k] = op.sym_kidx[k];
+    op.sym_written = 1 | (so.v32 ? 2 : 0);
+  }
   if (s.fe.dim == 2)
     hipLaunchKernelGGL((k_assemble_gather<2, kGatherRows>), dim3(rb), dim3(kGatherRows), 0, st, g, ph, tt, op.n,
-                       op.row_vtx.p, op.vtx_dof.p, op.slot.p, op.rowptr.p, op.tab.p, d_fixed, op.val.p, op.rhs.p);
+                       op.row_vtx.p, op.vtx_dof.p, op.slot.p, op.rowptr.p, op.tab.p, d_fixed, op.val.p, op.rhs.p, so);
   else
     hipLaunchKernelGGL((k_assemble_gather<3, kGatherRows>), dim3(rb), dim3(kGatherRows), 0, st, g, ph, tt, op.n,
-                       op.row_vtx.p, op.vtx_dof.p, op.slot.p, op.rowptr.p, op.tab.p, d_fixed, op.val.p, op.rhs.p);
+                       op.row_vtx.p, op.vtx_dof.p, op.slot.p, op.rowptr.p, op.tab.p, d_fixed, op.val.p, op.rhs.p, so);
 }
 
 }  // namespace
@@ -630,6 +744,7 @@ void assemble(femhip_operator &op, const double *fixed_host, float *ms) {
   const bool poisson_q1 = op.tab_kind == 0 && fe.order == 1 && fe.ncomp == 1 && fe.nd == (1 << fe.dim);
   if (poisson_q1 && op.gather == 0) setup_gather(op);
   const bool gather = poisson_q1 && op.gather == 1;
+  op.sym_written = 0;
   hipEvent_t e0, e1;
   FH_HIP(hipEventCreate(&e0));
   FH_HIP(hipEventCreate(&e1));

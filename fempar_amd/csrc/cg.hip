@@ -1115,6 +1115,7 @@ void prepare(femhip_solver &s) {
   const bool jac = s.prm.preconditioner == FEMHIP_PREC_JACOBI;
   const bool mx = mixed(s);
   if (!op.sl.built) build_sliced(op);
+  if (op.sl.sym && op.gather == 1 && op.sym_gather == 0) setup_sym_gather(op);
   if (!s.prepared) {
     s.n = n;
     const int ns = op.sl.ns;

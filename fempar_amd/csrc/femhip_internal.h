@@ -196,6 +196,13 @@ struct femhip_operator {
   femhip::DevBuf<uint8_t> slot;
   femhip::DevBuf<double> trig;              // 4 values per (direction, layer, 1-D point)
   int trig_off[3] = {0, 0, 0};
+  // the owner-computes path also writes the solver's symmetric-diagonal copy
+  // when every neighbour code has one column offset over all rows: sym_kidx
+  // [code] = stored diagonal of that offset (-1: lower part).  sym_written:
+  // copies the last compute wrote (1 FP64, 2 FP32).
+  int sym_gather = 0;                       // 0 not checked, 1 in use, -1 not applicable
+  int8_t sym_kidx[32];
+  int sym_written = 0;
   // structured stencil operator (no CSR, no space): rows are component-major
   // blocks of `blk` rows (the `nodes` free nodes, padded to a multiple of 64),
   // values assembled in place into the sliced layout
@@ -213,6 +220,9 @@ void build_sliced(femhip_operator &op);
 
 // assembly (assemble.hip)
 void assemble(femhip_operator &op, const double *fixed_host, float *ms);
+// once the symmetric-diagonal solver copy exists: decide whether the
+// owner-computes assembly can write it directly (sets op.sym_gather)
+void setup_sym_gather(femhip_operator &op);
 void upload_physics(femhip_operator &op);
 void spmv_full(femhip_operator &op, const double *d_x, double *d_y);
 

@@ -187,6 +187,46 @@ def test_fp32_refined_reaches_fp64_accuracy(env):
     assert np.linalg.norm(x - x_ref) <= 1e-8 * np.linalg.norm(x_ref)
 
 
+def _solver(env, op, precision):
+    s = F.iterative_linear_solver_t().create(env)
+    s.set_type_from_string("CG")
+    s.set_parameters_from_pl({"ILS_RTOL": 1e-10, "ILS_MAX_NUM_ITERATIONS": 20000, "FEMHIP_PRECISION": precision})
+    s.set_operators(op, F.jacobi_preconditioner_t(op))
+    return s
+
+
+@pytest.mark.parametrize("name", ["2d_q1_64_sin", "3d_q1_cellk"])
+def test_recompute_writes_solver_copies(env, name):
+    """Once solvers exist, the owner-computes assembly of a scalar Q1 operator
+    also writes their symmetric-diagonal copies (FP64 and FP32).  A recompute
+    with new coefficients must then solve exactly as a fresh operator, whose
+    copies are filled from the CSR."""
+    print(f"[recompute] {name}", flush=True)
+    osp, oph, fes, op, di = _build(env, name)
+    op.compute()
+    solvers = {p: _solver(env, op, p) for p in ("FP64", "FP32_REFINED")}
+    for s in solvers.values():
+        s.solve_resident()
+    if di.k == "cell":
+        di2 = F.poisson_discrete_integration_t(k="cell", k_cell=di.k_cell * np.linspace(0.5, 2.0, di.k_cell.size),
+                                               f=di.f, f0=di.f0)
+    else:
+        di2 = F.poisson_discrete_integration_t(k="const", k0=3.5, f=di.f, f0=di.f0)
+    phys = di2.physics()
+    F._native.check(op._lib.femhip_operator_set_physics(op.h, F._native.C.byref(phys)))
+    op.compute()
+    op_ref = F.fe_affine_operator_t()
+    op_ref.create("CSR", [True], fes, di2)
+    op_ref.compute()
+    assert np.array_equal(op.get_matrix().val, op_ref.get_matrix().val)
+    for p, s in solvers.items():
+        r = s.solve_resident()
+        s_ref = _solver(env, op_ref, p)
+        r_ref = s_ref.solve_resident()
+        assert r["num_iterations"] == r_ref["num_iterations"], p
+        assert np.array_equal(s.get_solution(), s_ref.get_solution()), p
+
+
 def test_error_paths(env):
     osp, oph, fes, op, di = _build(env, "2d_q1_64_sin")
     op.compute()
