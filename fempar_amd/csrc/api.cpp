@@ -309,8 +309,7 @@ int femhip_operator_compute(femhip_operator_t *op, const double *fixed_values, d
   assemble(*op, fixed_values, &t);
   op->computed = true;
   ++op->version;
-  if (op->sym_written & 1) op->sl.filled_version = op->version;   // the assembly wrote the solver copies
-  if (op->sym_written & 2) op->sl.filled_version32 = op->version;
+  if (op->sym_written & 1) op->sl.filled_version = op->version;   // the assembly wrote the solver copy
   if (ms) *ms = t;
   return FEMHIP_OK;
   FH_CATCH

@@ -359,7 +359,6 @@ __device__ __forceinline__ void coef_tab(const PhysDev &ph, const TrigTab &tt, c
 // a fixed neighbour), for the codes of the stored (upper) diagonals.
 struct SymOut {
   double *v64;
-  float *v32;
   int64_t stride;
   int8_t kidx[32];
 };
@@ -433,9 +432,7 @@ __global__ __launch_bounds__(RB) void k_assemble_gather(
     for (int k = 0; k < K; ++k) {
       const int d = so.kidx[k];
       if (d >= 0) {
-        const double v = sr[k] != 255 ? acc[k] : 0.0;
-        so.v64[d * so.stride + r] = v;
-        if (so.v32) so.v32[d * so.stride + r] = (float)v;
+        so.v64[d * so.stride + r] = sr[k] != 255 ? acc[k] : 0.0;
       }
     }
   }
@@ -640,16 +637,15 @@ void launch_gather(femhip_operator &op, const PhysDev &ph, const double *d_fixed
   const VtxGrid g = vtx_grid(s.mesh);
   const TrigTab tt{reinterpret_cast<const double4 *>(op.trig.p), {op.trig_off[0], op.trig_off[1], op.trig_off[2]}};
   const unsigned rb = (unsigned)((op.n + kGatherRows - 1) / kGatherRows);
-  // the solver copies that exist (FP64, and FP32 once a refined solver made
-  // it) are written too; their padding rows stay as the first fill left them
+  // the solver's FP64 copy is written too (its padding rows stay as the first
+  // fill left them); the FP32 copy is derived from it by the solver
   SymOut so{};
   op.sym_written = 0;
   if (op.sym_gather == 1 && op.sl.v64.p) {
     so.v64 = op.sl.v64.p;
-    so.v32 = op.sl.v32.p;
     so.stride = op.sl.stride;
     for (int k = 0; k < 32; ++k) so.kidx[k] = op.sym_kidx[k];
-    op.sym_written = 1 | (so.v32 ? 2 : 0);
+    op.sym_written = 1;
   }
   if (s.fe.dim == 2)
     hipLaunchKernelGGL((k_assemble_gather<2, kGatherRows>), dim3(rb), dim3(kGatherRows), 0, st, g, ph, tt, op.n,

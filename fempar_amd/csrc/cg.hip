@@ -60,6 +60,7 @@ struct CgState {
   double rtol, atol;
   double rmax, delta;          // reliable updates
   double ru_rz, ru_rmax, ru_rnorm, ru_beta;
+  double rscale, ru_rscale;    // ||S r|| / ||r|| at the last update (scaled inner residual)
   int iter, done, converged, maxit, crit, ru, updates, ru_conv, ru_done, pad;
 };
 
@@ -81,6 +82,7 @@ struct SlicedDev {
   const int32_t *srow;         // rows of the lanes of permuted slices
   int32_t pstride;             // order 2: slices per step of a wave (about one plane)
   int32_t trip;                // symmetric 27-point offsets in column triplets (sym27_row)
+  int32_t unit;                // symmetric layout: the diagonal is 1 and is not read
 };
 
 // Row of lane `lane` of slice s (meta bit 28: the slice's rows are a sorted
@@ -185,6 +187,10 @@ __device__ __forceinline__ WaveRange wave_slices(int ns, int order, int pstride 
   return {(int)((int64_t)ns * wave / nw), (int)((int64_t)ns * (wave + 1) / nw), 1};
 }
 
+// The next CG direction p' = beta p + z, rounded once (fused multiply-add).
+__device__ __forceinline__ double dir_value(double beta, double p, double z) { return __builtin_fma(beta, p, z); }
+__device__ __forceinline__ float dir_value(float beta, float p, float z) { return __builtin_fmaf(beta, p, z); }
+
 // Right-hand vectors of the slice kernels.  operator()(c) is element c;
 // off(o, rb) is element row + o where rb = row * sizeof(elem): a scalar base
 // pointer per diagonal plus a per-lane 32-bit byte offset, so a gather needs no
@@ -196,15 +202,11 @@ struct VecRef {   // x
   __device__ __forceinline__ X operator()(int c) const { return x[c]; }
   __device__ __forceinline__ X off(int o, uint32_t rb) const { return *(const X *)((const char *)(x + o) + rb); }
 };
-struct SumRef {   // x + d: FP64 solution plus FP32 increment, the solution a reliable update makes
-  using elem = double;
-  const double *__restrict__ x;
-  const float *__restrict__ d;
-  __device__ __forceinline__ double operator()(int c) const { return x[c] + (double)d[c]; }
-  __device__ __forceinline__ double off(int o, uint32_t rb) const {
-    return *(const double *)((const char *)(x + o) + rb) + (double)*(const float *)((const char *)(d + o) + rb / 2);
-  }
-};
+// FP64 solution plus FP32 increment (scaled by sc when set): the solution a
+// reliable update makes.
+__device__ __forceinline__ double new_x(double x, float d, const double *sc, int64_t i) {
+  return sc ? __builtin_fma(sc[i], (double)d, x) : x + (double)d;
+}
 
 // Row `lane` of slice s (s wave-uniform): returns (A g)_row.  Slices whose
 // columns may leave [0, n) (flag bit 29) clamp the column.
@@ -241,7 +243,8 @@ __device__ __forceinline__ A sym27_row(const SlicedDev &m, const T *__restrict__
   A acc = 0;
   const T *vr = v + row;
   // upper part: k = 0 (column row), 1 (row + 1), k >= 2: triplet (k+1)/3, member (k+1)%3
-  acc += (A)vr[0] * (A)xv[0][1];
+  if (m.unit) acc += (A)xv[0][1];
+  else acc += (A)vr[0] * (A)xv[0][1];
   acc += (A)vr[m.stride] * (A)xv[0][2];
 #pragma unroll
   for (int k = 2; k < 14; ++k) acc += (A)vr[(int64_t)k * m.stride] * (A)xv[(k + 1) / 3][(k + 1) % 3];
@@ -269,7 +272,7 @@ __device__ __forceinline__ A sym_rows(const SlicedDev &m, const T *__restrict__ 
 #pragma unroll
     for (int k = 0; k < 14; ++k) {
       const int o = m.soffs[k];
-      const A a = (A)vr[(int64_t)k * m.stride];
+      const A a = (k == 0 && m.unit) ? (A)1 : (A)vr[(int64_t)k * m.stride];
       const int c = min(row + o, hi);
       acc += a * (A)g(c);
     }
@@ -286,7 +289,7 @@ __device__ __forceinline__ A sym_rows(const SlicedDev &m, const T *__restrict__ 
   }
   for (int k = 0; k < m.kplus; ++k) {   // diagonal and upper part
     const int o = m.soffs[k];
-    const A a = (A)vr[k * m.stride];
+    const A a = (k == 0 && m.unit) ? (A)1 : (A)vr[k * m.stride];
     const int c = min(row + o, hi);
     acc += a * (A)g(c);
   }
@@ -603,12 +606,20 @@ __global__ __launch_bounds__(kThreads) void k_dir(int32_t n, T *__restrict__ x, 
 
 // ------------------------------------------------- FP32 with reliable updates ---
 
-// r64 = b - A64 x64 ; r32 = r64 ; z = M r32 ; p = z ; d = 0 ; partials rz32, ||r64||^2, ||b||^2
+// With the Jacobi preconditioner the FP32 iterations run on the symmetrically
+// scaled system  (S A S) y = S r,  S = diag(A)^(-1/2)  (sc = S in FP64): CG on
+// S A S is Jacobi-PCG on A in other variables (same Krylov spaces), its matrix
+// has a unit diagonal that is not read, and z = r needs no preconditioner
+// pass.  The increment d lives in the scaled variables; a reliable update
+// folds x += S d into the FP64 solution and scales the FP64 residual b - A x
+// into the FP32 one.  Without a preconditioner sc is null and S = I.
+
+// r64 = b - A64 x64 ; r32 = S r64 ; p = r32 ; d = 0 ; partials <r32,r32>, ||r64||^2, ||b||^2, ||r32||^2
 __global__ __launch_bounds__(kThreads) void k_ru_init(SlicedDev m, const double *__restrict__ v64,
                                                       const double *__restrict__ b, const double *__restrict__ x,
                                                       int x_zero, double *__restrict__ r64, float *__restrict__ r32,
                                                       float *__restrict__ p32, float *__restrict__ d32,
-                                                      const float *__restrict__ invd32, double *part) {
+                                                      const double *__restrict__ sc, double *part) {
   const int lane = threadIdx.x & 63;
   const WaveRange wr = wave_slices(m.ns, m.order, m.pstride);
   const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
@@ -619,12 +630,11 @@ __global__ __launch_bounds__(kThreads) void k_ru_init(SlicedDev m, const double 
     if (row < m.n) {
       const double bi = b[row], ri = bi - ax;
       r64[row] = ri;
-      const float rf = (float)ri;
-      const float zf = invd32 ? invd32[row] * rf : rf;
+      const float rf = (float)(sc ? sc[row] * ri : ri);
       r32[row] = rf;
-      p32[row] = zf;
+      p32[row] = rf;
       d32[row] = 0.0f;
-      acc[0] += (double)(rf * zf);
+      acc[0] += (double)(rf * rf);
       acc[1] += ri * ri;
       acc[2] += bi * bi;
       acc[3] += (double)(rf * rf);
@@ -646,6 +656,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_init_fin(const double *part, in
   s.rnorm = rnorm;
   s.rhs_test = (s.crit == FEMHIP_RES_RHS ? bnorm : rnorm) * s.rtol + s.atol;
   s.rmax = sqrt(v[3]);
+  s.rscale = rnorm > 0.0 ? sqrt(v[3]) / rnorm : 1.0;
   s.iter = 0;
   s.converged = rnorm <= s.rhs_test;
   s.done = s.converged || (zero_diag && *zero_diag);
@@ -668,14 +679,19 @@ __global__ __launch_bounds__(kThreads) void k_ru_init_fin(const double *part, in
 // previous slot requested it (state.ru), a reliable update; the same three
 // kernels serve both, so an update costs no extra launches.
 
-// slot 1: q = A32 p (partials <p,q>, ||p||^2), or r = b - A64 (x + d) in FP64
-// rounded into r32 (partials ||r64||^2, <r32, M r32>, ||r32||^2)
-__global__ __launch_bounds__(kThreads) void k_ru_spmv(SlicedDev m, const float *__restrict__ v32,
+// slot 1: q = A32 p (partials <p,q>, ||p||^2), or r = b - A64 x in FP64 (the
+// slot before folded the increment into x), scaled and rounded into r32
+// (partials ||r64||^2, <r32,r32>, ||r32||^2).
+// m32: the FP32 matrix's view (unit diagonal when scaled).  Two blocks per CU
+// must stay resident for the FP32 iterations (4 waves per SIMD: at most 128
+// VGPRs; the second bound is waves per execution unit); the rare
+// FP64 update branch, which alone would take more, absorbs the limit.
+__global__ __launch_bounds__(kThreads, 4) void k_ru_spmv(SlicedDev m, SlicedDev m32, const float *__restrict__ v32,
                                                       const double *__restrict__ v64, const float *__restrict__ p,
                                                       float *__restrict__ q, const double *__restrict__ b,
-                                                      const double *__restrict__ x, const float *__restrict__ d,
-                                                      float *__restrict__ r32, const float *__restrict__ invd,
-                                                      const CgState *S, int par, double *part) {
+                                                      const double *__restrict__ x, float *__restrict__ r32,
+                                                      const double *__restrict__ sc, const CgState *S, int par,
+                                                      double *part) {
   const CgState &cur = S[par];
   if (cur.done) return;
   const int lane = threadIdx.x & 63;
@@ -685,7 +701,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_spmv(SlicedDev m, const float *
   if (!cur.ru) {
     float pq = 0.f, pp = 0.f;
     for (int s = s0; s < s1; s += sstep) {
-      const float a = slice_row<float, float>(m, v32, p, s, lane);
+      const float a = slice_row<float, float>(m32, v32, p, s, lane);
       const int row = slice_row_index(m, s, lane);
       if (row < m.n) {
         q[row] = a;
@@ -698,17 +714,14 @@ __global__ __launch_bounds__(kThreads) void k_ru_spmv(SlicedDev m, const float *
     acc[1] = pp;
   } else {
     for (int s = s0; s < s1; s += sstep) {
-      // A (x + d) in one pass over the FP64 matrix, with x + d rounded as the
-      // update's x += d rounds it
-      const double ax = slice_rows<double, double>(m, v64, SumRef{x, d}, s, lane);
+      const double ax = slice_row<double, double>(m, v64, x, s, lane);
       const int row = slice_row_index(m, s, lane);
       if (row < m.n) {
         const double ri = b[row] - ax;
-        const float rf = (float)ri;
+        const float rf = (float)(sc ? sc[row] * ri : ri);
         r32[row] = rf;
-        const float zf = invd ? invd[row] * rf : rf;
         acc[0] += ri * ri;
-        acc[1] += (double)(rf * zf);
+        acc[1] += (double)(rf * rf);
         acc[2] += (double)(rf * rf);
       }
     }
@@ -716,19 +729,17 @@ __global__ __launch_bounds__(kThreads) void k_ru_spmv(SlicedDev m, const float *
   write_partials<3>(acc, part);
 }
 
-// slot 2: alpha ; r -= alpha q (partials <r,z>, ||r||^2), or
-// the update's second half: x += d ; d = 0 ; FP64 stopping test ; p = z + beta p
-__global__ __launch_bounds__(kThreads) void k_ru_update(int32_t n, double *__restrict__ x, float *__restrict__ d,
-                                                        float *__restrict__ r, float *__restrict__ p,
-                                                        const float *__restrict__ q, const float *__restrict__ invd,
-                                                        CgState *S, int par, const double *part_s, int gs,
-                                                        double *part) {
+// slot 2: alpha ; r -= alpha q (partials <r,r> twice: <r,z> and ||r||^2), or
+// the update's second half: FP64 stopping test ; p = r + beta p
+__global__ __launch_bounds__(kThreads) void k_ru_update(int32_t n, float *__restrict__ r, float *__restrict__ p,
+                                                        const float *__restrict__ q, CgState *S, int par,
+                                                        const double *part_s, int gs, double *part) {
   // Block 0 writes only fields the other blocks do not read (alpha, pap, pp,
   // ru_*); they read done, ru, rz, iter, maxit and rhs_test.
   const CgState &cur = S[par];
   if (cur.done) return;
   const int64_t i0 = (int64_t)blockIdx.x * kThreads + threadIdx.x, stride = (int64_t)gridDim.x * kThreads;
-  float rv[kPrefetch], qv[kPrefetch], dv[kPrefetch];
+  float rv[kPrefetch], qv[kPrefetch];
   if (!cur.ru) {
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k) {   // in flight while the partials are reduced
@@ -736,7 +747,6 @@ __global__ __launch_bounds__(kThreads) void k_ru_update(int32_t n, double *__res
       if (i < n) {
         rv[k] = r[i];
         qv[k] = q[i];
-        dv[k] = invd ? invd[i] : 1.0f;
       }
     }
   }
@@ -749,15 +759,8 @@ __global__ __launch_bounds__(kThreads) void k_ru_update(int32_t n, double *__res
     double beta = 1.0 / cur.rz;
     beta = beta * red[1];
     const float bt = (float)beta;
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-      x[i] = x[i] + (double)d[i];
-      d[i] = 0.0f;
-      if (!done) {
-        const float zi = invd ? invd[i] * r[i] : r[i];
-        const float t = bt * p[i];
-        p[i] = t + zi;
-      }
-    }
+    if (!done)
+      for (int64_t i = i0; i < n; i += stride) p[i] = dir_value(bt, p[i], r[i]);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       CgState &w = S[par];
       w.ru_rz = red[1];
@@ -766,6 +769,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_update(int32_t n, double *__res
       w.ru_beta = beta;
       w.ru_conv = conv;
       w.ru_done = done;
+      w.ru_rscale = rnorm > 0.0 ? sqrt(red[2]) / rnorm : cur.rscale;
     }
     return;
   }
@@ -776,28 +780,28 @@ __global__ __launch_bounds__(kThreads) void k_ru_update(int32_t n, double *__res
     S[par].pp = red[1];
   }
   const float a = (float)alpha;
-  float rz = 0.f, rr = 0.f;
-  auto row = [&](int64_t i, float r0, float qi, float di) {
+  float rr = 0.f;
+  auto row = [&](int64_t i, float r0, float qi) {
     const float ri = r0 + (-a) * qi;
     r[i] = ri;
-    const float zi = invd ? di * ri : ri;
-    rz += ri * zi;
     rr += ri * ri;
   };
 #pragma unroll
   for (int k = 0; k < kPrefetch; ++k)
-    if (i0 + k * stride < n) row(i0 + k * stride, rv[k], qv[k], dv[k]);
-  for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) row(i, r[i], q[i], invd ? invd[i] : 1.0f);
-  double out[2] = {(double)rz, (double)rr};
+    if (i0 + k * stride < n) row(i0 + k * stride, rv[k], qv[k]);
+  for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) row(i, r[i], q[i]);
+  double out[2] = {(double)rr, (double)rr};
   write_partials<2>(out, part);
 }
 
-// slot 3: d += alpha p, then request a reliable update or take the CG
-// direction step; after an update, publish its results to the next parity slot
+// slot 3: d += alpha p, then take the CG direction step p = r + beta p, or
+// request a reliable update and fold the increment into the FP64 solution
+// (x += S d, d = 0); after an update, publish its results to the next parity
+// slot
 __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restrict__ d, float *__restrict__ p,
-                                                     const float *__restrict__ r, const float *__restrict__ invd,
-                                                     CgState *S, int par, const double *part_u, int gu,
-                                                     CgResult *R, CgResult *Rh) {
+                                                     const float *__restrict__ r, double *__restrict__ x,
+                                                     const double *__restrict__ sc, CgState *S, int par,
+                                                     const double *part_u, int gu, CgResult *R, CgResult *Rh) {
   const CgState &cur = S[par];
   if (cur.done) {   // keep `done` sticky in both parity slots
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -819,6 +823,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restric
       nx.converged = cur.ru_conv;
       nx.done = cur.ru_done;
       nx.updates = cur.updates + 1;
+      nx.rscale = cur.ru_rscale;
       S[par ^ 1] = nx;
       R->iter = cur.iter;
       R->est = cur.ru_rnorm;
@@ -833,7 +838,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restric
     return;
   }
   const int64_t i0 = (int64_t)blockIdx.x * kThreads + threadIdx.x, stride = (int64_t)gridDim.x * kThreads;
-  float pv[kPrefetch], dv[kPrefetch], rv[kPrefetch], iv[kPrefetch];
+  float pv[kPrefetch], dv[kPrefetch], rv[kPrefetch];
 #pragma unroll
   for (int k = 0; k < kPrefetch; ++k) {   // in flight while the partials are reduced
     const int64_t i = i0 + k * stride;
@@ -841,15 +846,15 @@ __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restric
       pv[k] = p[i];
       dv[k] = d[i];
       rv[k] = r[i];
-      iv[k] = invd ? invd[i] : 1.0f;
     }
   }
   double red[2];
   reduce_partials<2>(part_u, gu, red);
   const int it = cur.iter + 1;
-  const double rnorm = sqrt(red[1]);
-  // residual fell by delta since the last update, looks converged, or budget spent
-  const bool ru = rnorm < cur.delta * cur.rmax || rnorm <= cur.rhs_test || it >= cur.maxit;
+  const double rnorm = sqrt(red[1]);   // of the scaled FP32 residual
+  // residual fell by delta since the last update, looks converged (threshold
+  // carried into the scaled norm by the last update's ratio), or budget spent
+  const bool ru = rnorm < cur.delta * cur.rmax || rnorm <= cur.rhs_test * cur.rscale || it >= cur.maxit;
   double beta = 1.0 / cur.rz;
   beta = beta * red[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -863,37 +868,46 @@ __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restric
       nx.rmax = fmax(cur.rmax, rnorm);
     }
     S[par ^ 1] = nx;
-    R->prog = rnorm;   // the FP32 recurrence residual
+    R->prog = cur.rscale > 0.0 ? rnorm / cur.rscale : rnorm;   // estimate in the stopping test's norm
     R->prog_it = it;
     if (Rh) *Rh = *R;
   }
   const float a = (float)cur.alpha;
-  if (ru) {   // the increment only; the next slot recomputes the residual
+  if (ru) {   // the next slot recomputes the residual of the new x
+    auto fold = [&](int64_t i, float di, float pi) {
+      x[i] = new_x(x[i], di + a * pi, sc, i);
+      d[i] = 0.0f;
+    };
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k)
-      if (i0 + k * stride < n) d[i0 + k * stride] = dv[k] + a * pv[k];
-    for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) d[i] = d[i] + a * p[i];
+      if (i0 + k * stride < n) fold(i0 + k * stride, dv[k], pv[k]);
+    for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) fold(i, d[i], p[i]);
     return;
   }
   const float bt = (float)beta;
-  auto row = [&](int64_t i, float pi, float di, float ri, float ii) {
+  auto row = [&](int64_t i, float pi, float di, float ri) {
     d[i] = di + a * pi;
-    const float zi = invd ? ii * ri : ri;
-    const float t = bt * pi;
-    p[i] = t + zi;
+    p[i] = dir_value(bt, pi, ri);
   };
 #pragma unroll
   for (int k = 0; k < kPrefetch; ++k)
-    if (i0 + k * stride < n) row(i0 + k * stride, pv[k], dv[k], rv[k], iv[k]);
-  for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) row(i, p[i], d[i], r[i], invd ? invd[i] : 1.0f);
+    if (i0 + k * stride < n) row(i0 + k * stride, pv[k], dv[k], rv[k]);
+  for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) row(i, p[i], d[i], r[i]);
 }
 
 // ------------------------------------------------------------------ helpers ---
 
-// Jacobi setup: invD = 1 / diag(A) (csr_sparse_matrix.f90:4231-4256, serial_scalar_array.f90:581-588)
+// Jacobi setup: invD = 1 / diag(A) (csr_sparse_matrix.f90:4231-4256,
+// serial_scalar_array.f90:581-588), or with `root` the scaling S = diag(A)^(-1/2)
+// of the FP32 iterations, which needs a positive diagonal.  A zero pivot (or a
+// negative one under `root`) raises the flag: 1 zero, 2 negative.
+__device__ __forceinline__ void jacobi_store(double d, int root, double *out, int32_t i, int *flag) {
+  if (d == 0.0) *flag = 1;
+  else if (root && d < 0.0) *flag = 2;
+  out[i] = root ? 1.0 / sqrt(d) : 1.0 / d;
+}
 __global__ void k_jacobi(int32_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                         const double *__restrict__ val, double *__restrict__ invd, float *__restrict__ invd32,
-                         int *zero_diag) {
+                         const double *__restrict__ val, double *__restrict__ out, int root, int *flag) {
   int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int32_t lo = rowptr[i], hi = rowptr[i + 1] - 1;
@@ -908,17 +922,14 @@ __global__ void k_jacobi(int32_t n, const int32_t *__restrict__ rowptr, const in
     if (c < i) lo = mid + 1;
     else hi = mid - 1;
   }
-  if (d == 0.0) *zero_diag = 1;
-  double inv = 1.0 / d;
-  invd[i] = inv;
-  if (invd32) invd32[i] = (float)inv;
+  jacobi_store(d, root, out, i, flag);
 }
 
 // The same from the sliced copy, for operators held without a CSR: the entry
 // of offset 0 of a diagonal slice, or the first ELL entry whose column is the
 // row (real entries precede the padding).
-__global__ void k_jacobi_sliced(SlicedDev m, const double *__restrict__ v, double *__restrict__ invd,
-                                float *__restrict__ invd32, int *zero_diag) {
+__global__ void k_jacobi_sliced(SlicedDev m, const double *__restrict__ v, double *__restrict__ out, int root,
+                                int *flag) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.n) return;
   const int32_t s = i / kSlice, lane = i % kSlice;
@@ -935,10 +946,7 @@ __global__ void k_jacobi_sliced(SlicedDev m, const double *__restrict__ v, doubl
       break;
     }
   }
-  if (d == 0.0) *zero_diag = 1;
-  const double inv = 1.0 / d;
-  invd[i] = inv;
-  if (invd32) invd32[i] = (float)inv;
+  jacobi_store(d, root, out, i, flag);
 }
 
 // Symmetric-diagonal layout from the CSR: the entries of row r with col >= r
@@ -986,15 +994,47 @@ __global__ __launch_bounds__(kFillRows) void k_fill_sym(int32_t n, const int32_t
 
 // Jacobi setup from the symmetric-diagonal copy: diagonal 0 is offset 0.
 // A zero pivot raises the flag (mapped host memory, read after the solve).
-__global__ void k_jacobi_sym(int32_t n, const double *__restrict__ v, double *__restrict__ invd,
-                             float *__restrict__ invd32, int *zero_diag) {
+__global__ void k_jacobi_sym(int32_t n, const double *__restrict__ v, double *__restrict__ out, int root,
+                             int *flag) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double d = v[i];
-  if (d == 0.0) *zero_diag = 1;
-  const double inv = 1.0 / d;
-  invd[i] = inv;
-  if (invd32) invd32[i] = (float)inv;
+  jacobi_store(v[i], root, out, i, flag);
+}
+
+// FP32 copy of the solver matrix, of S A S when sc (= S) is set:
+// v32 = v64 * (sc[row] * sc[col]) (the product in that order keeps the copy
+// symmetric); out-of-range columns carry 0 and are clamped.  One thread per
+// (slice, lane).
+__global__ void k_scale32(SlicedDev m, const double *__restrict__ v64, const double *__restrict__ sc,
+                          float *__restrict__ v32) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = (int)(t / kSlice), lane = (int)(t % kSlice);
+  if (s >= m.ns) return;
+  const int hi = m.n - 1;
+  if (m.sym) {
+    const int row = s * kSlice + lane;
+    for (int k = 0; k < m.kplus; ++k) {
+      const int64_t e = (int64_t)k * m.stride + row;
+      double a = row < m.n ? v64[e] : 0.0;
+      if (sc && row < m.n) a *= sc[row] * sc[min(row + m.soffs[k], hi)];
+      v32[e] = (float)a;
+    }
+    return;
+  }
+  const int64_t b = m.base[s];
+  const int meta = m.meta[s];
+  const int K = meta & 0x0fffffff;
+  const bool ell = (meta >> 30) & 1;
+  const int row = slice_row_index(m, s, lane);
+  for (int k = 0; k < K; ++k) {
+    const int64_t e = b + (int64_t)k * kSlice + lane;
+    double a = v64[e];
+    if (sc && row < m.n) {
+      const int c = ell ? m.ecol[m.aux[s] + (int64_t)k * kSlice + lane] : row + m.offs[m.aux[s] + k];
+      a *= sc[row] * sc[min(max(c, 0), hi)];
+    }
+    v32[e] = (float)a;
+  }
 }
 
 // gather CSR values into the sliced layout (FP64 and/or FP32)
@@ -1049,7 +1089,7 @@ struct femhip_solver {
   int32_t n = 0;
   int g_mat = 8, g_vec = 8;      // grid sizes of the matrix and vector kernels
   DevBuf<double> x, b, r, p, q, invd, part_i, part_s, part_u, part_r, hist;
-  DevBuf<float> r32, p32, q32, d32, invd32;
+  DevBuf<float> r32, p32, q32, d32;
   DevBuf<CgState> st;            // S[2]
   DevBuf<CgResult> res;
   int *h_flag = nullptr, *d_flag = nullptr;    // zero-pivot flag of the Jacobi setup (pinned, mapped)
@@ -1102,10 +1142,18 @@ SlicedDev sliced_dev(femhip_operator &op) {
               !(tri && atoi(tri) == 0);
   for (int t = 1; trip && t <= 4; ++t) trip = o[3 * t - 1] + 1 == o[3 * t] && o[3 * t] + 1 == o[3 * t + 1];
   m.trip = trip ? 1 : 0;
+  m.unit = 0;
   return m;
 }
 
 bool mixed(const femhip_solver &s) { return s.prm.precision == FEMHIP_PRECISION_FP32_REFINED; }
+
+// The FP32 matrix as the kernels see it: with Jacobi it holds S A S, whose unit
+// diagonal the symmetric layout does not read.
+SlicedDev fp32_view(const femhip_solver &s, SlicedDev m) {
+  m.unit = (s.prm.preconditioner == FEMHIP_PREC_JACOBI && m.sym) ? 1 : 0;
+  return m;
+}
 
 void prepare(femhip_solver &s) {
   femhip_operator &op = *s.op;
@@ -1119,14 +1167,22 @@ void prepare(femhip_solver &s) {
   if (!s.prepared) {
     s.n = n;
     const int ns = op.sl.ns;
-    // about 4 slices per wavefront for the matrix kernels, multiple of 8 blocks
-    // one resident wave set: about 32 waves per CU, grids a multiple of the 8 XCDs
-    // measured at 128^3: 4 blocks of 512 per CU for FP64, 2 for the FP32
-    // iterations (whose SpMV and partial reductions are cheaper per block)
+    // Blocks of 512 per CU, grids a multiple of the 8 XCDs.  Matrix kernels:
+    // each XCD's waves sweep its eighth of the slices together, and the
+    // symmetric layout re-reads a stored diagonal one plane of rows later; the
+    // sweep is fastest when its front (resident waves x one slice of values)
+    // is about 2 MB, well inside the XCD's 4 MB L2: measured at 128^3, 1 block
+    // per CU for the FP64 values (SpMV 54 us against 67 us at 4) and 2 for the
+    // FP32 values; the sliced formats (no re-read) keep 4 and 2 (Q2 64^3).
+    // Vector kernels: 4 blocks per CU for FP64, 2 for FP32.  FEMHIP_CG_BPC
+    // sets both, FEMHIP_CG_BPC_MAT the matrix kernels alone.
     const char *bpc_env = getenv("FEMHIP_CG_BPC");
-    const int bpc = bpc_env ? std::max(1, atoi(bpc_env)) : std::max(1, (mx ? 1024 : 2048) / kThreads);
+    const char *mat_env = getenv("FEMHIP_CG_BPC_MAT");
+    const int bpc = bpc_env ? std::max(1, atoi(bpc_env)) : (mx ? 2 : 4);
+    const int bpc_mat = mat_env ? std::max(1, atoi(mat_env)) : bpc_env ? bpc : (mx ? 2 : op.sl.sym ? 1 : 4);
     const int cap = std::max(8, bpc * std::max(1, s.ctx->num_cus));
-    s.g_mat = std::max(8, std::min(cap, (ns + 2 * kWaves - 1) / (2 * kWaves)));
+    const int cap_mat = std::max(8, bpc_mat * std::max(1, s.ctx->num_cus));
+    s.g_mat = std::max(8, std::min(cap_mat, (ns + 2 * kWaves - 1) / (2 * kWaves)));
     s.g_mat = (s.g_mat + 7) / 8 * 8;
     s.g_vec = std::max(8, std::min(cap, (n + 2 * kThreads - 1) / (2 * kThreads)));
     s.x.alloc(n); s.b.alloc(n); s.r.alloc(n); s.p.alloc(n); s.q.alloc(n);
@@ -1137,10 +1193,9 @@ void prepare(femhip_solver &s) {
     s.hist.alloc((size_t)std::max(1, s.prm.max_num_iterations));
     s.st.alloc(2);
     s.res.alloc(1);
-    if (jac) s.invd.alloc(n);
+    if (jac) s.invd.alloc(n);   // invD (FP64) or S (FP32_REFINED)
     if (mx) {
       s.r32.alloc(n); s.p32.alloc(n); s.q32.alloc(n); s.d32.alloc(n);
-      if (jac) s.invd32.alloc(n);
     }
     for (int k = 0; k < 2; ++k) {   // written by the GPU, read by the host: mapped, coherent
       FH_HIP(hipHostMalloc((void **)&s.h_res[k], sizeof(CgResult), hipHostMallocMapped | hipHostMallocCoherent));
@@ -1154,42 +1209,53 @@ void prepare(femhip_solver &s) {
     s.prepared = true;
   }
   // matrix-dependent work, redone whenever the operator was recomputed
-  if (op.sl.filled_version != op.version || (mx && op.sl.filled_version32 != op.version)) {
+  if (op.sl.filled_version != op.version) {
     if (!op.sl.v64.p) op.sl.v64.alloc((size_t)std::max<int64_t>(1, op.sl.entries));
-    if (mx && !op.sl.v32.p) op.sl.v32.alloc((size_t)std::max<int64_t>(1, op.sl.entries));
     if (op.sl.sym) {   // stride is a multiple of kFillRows; every row has at most 2 kplus - 1 entries
       const size_t lds = (size_t)kFillRows * (2 * op.sl.kplus - 1) * (sizeof(double) + sizeof(int32_t));
       hipLaunchKernelGGL(k_fill_sym, dim3((unsigned)(op.sl.stride / kFillRows)), dim3(kFillRows), lds, st, n,
                          op.rowptr.p, op.col.p, op.val.p, op.sl.soffs.p, op.sl.kplus, op.sl.stride, op.sl.v64.p,
-                         mx ? op.sl.v32.p : nullptr);
-    } else
+                         nullptr);
+    } else {
       hipLaunchKernelGGL(k_fill_sliced, dim3(2048), dim3(256), 0, st, op.sl.entries, op.sl.map.p, op.val.p,
-                         op.sl.v64.p, mx ? op.sl.v32.p : nullptr);
+                         op.sl.v64.p, nullptr);
+    }
     FH_HIP(hipGetLastError());
     op.sl.filled_version = op.version;
-    if (mx) op.sl.filled_version32 = op.version;
   }
-  // Jacobi setup without a host round trip: a zero pivot raises s.h_flag,
+  // Jacobi setup without a host round trip: invD for FP64, the scaling
+  // S = diag^(-1/2) for the FP32 iterations; a bad pivot raises s.h_flag,
   // which ends the solve at its initial state and is reported by read_result
   if (jac && n > 0) {
     *s.h_flag = 0;
     const dim3 G((n + 255) / 256), B(256);
+    const int root = mx ? 1 : 0;
     if (op.sl.sym)
-      hipLaunchKernelGGL(k_jacobi_sym, G, B, 0, st, n, op.sl.v64.p, s.invd.p, mx ? s.invd32.p : nullptr, s.d_flag);
+      hipLaunchKernelGGL(k_jacobi_sym, G, B, 0, st, n, op.sl.v64.p, s.invd.p, root, s.d_flag);
     else if (op.structured)   // no CSR: the diagonal comes from the sliced copy
-      hipLaunchKernelGGL(k_jacobi_sliced, G, B, 0, st, sliced_dev(op), op.sl.v64.p, s.invd.p,
-                         mx ? s.invd32.p : nullptr, s.d_flag);
+      hipLaunchKernelGGL(k_jacobi_sliced, G, B, 0, st, sliced_dev(op), op.sl.v64.p, s.invd.p, root, s.d_flag);
     else
-      hipLaunchKernelGGL(k_jacobi, G, B, 0, st, n, op.rowptr.p, op.col.p, op.val.p, s.invd.p,
-                         mx ? s.invd32.p : nullptr, s.d_flag);
+      hipLaunchKernelGGL(k_jacobi, G, B, 0, st, n, op.rowptr.p, op.col.p, op.val.p, s.invd.p, root, s.d_flag);
     FH_HIP(hipGetLastError());
+  }
+  // FP32 copy: of S A S with Jacobi, of A without
+  if (mx && (op.sl.filled_version32 != op.version || op.sl.scaled32 != jac)) {
+    if (!op.sl.v32.p) op.sl.v32.alloc((size_t)std::max<int64_t>(1, op.sl.entries));
+    const int64_t threads = (int64_t)op.sl.ns * kSlice;
+    hipLaunchKernelGGL(k_scale32, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, sliced_dev(op),
+                       op.sl.v64.p, jac ? s.invd.p : nullptr, op.sl.v32.p);
+    FH_HIP(hipGetLastError());
+    op.sl.filled_version32 = op.version;
+    op.sl.scaled32 = jac;
   }
 }
 
 // after the stream finished: the Jacobi setup's zero-pivot report
 void check_pivots(const femhip_solver &s) {
-  FH_CHECK(!(s.prm.preconditioner == FEMHIP_PREC_JACOBI && s.h_flag && *s.h_flag), FEMHIP_ERR_ZERO_DIAGONAL,
-           "op1 cannot be entrywise inverted; there exists a null diagonal entry");
+  const int f = (s.prm.preconditioner == FEMHIP_PREC_JACOBI && s.h_flag) ? *s.h_flag : 0;
+  FH_CHECK(f != 1, FEMHIP_ERR_ZERO_DIAGONAL, "op1 cannot be entrywise inverted; there exists a null diagonal entry");
+  FH_CHECK(f != 2, FEMHIP_ERR_ZERO_DIAGONAL,
+           "FP32_REFINED with Jacobi scales by diag(A)^(-1/2): the diagonal must be positive");
 }
 
 // Launches a kernel on a stream, or appends it as a kernel node after the
@@ -1240,13 +1306,13 @@ void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = 
     L(k_update<double, double>, GV, B, n, s.r.p, s.q.p, invd, s.st.p, par, s.part_s.p, s.g_mat, s.part_u.p);
     L(k_dir<double>, GV, B, n, s.x.p, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.hist.p, s.res.p, rh);
   } else {
-    const float *invd = jac ? s.invd32.p : nullptr;
-    L(k_ru_spmv, GM, B, m, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.d32.p, s.r32.p, invd, s.st.p,
-      par, s.part_s.p);
+    const double *sc = jac ? s.invd.p : nullptr;   // S = diag^(-1/2)
+    const SlicedDev m32 = fp32_view(s, m);
+    L(k_ru_spmv, GM, B, m, m32, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.r32.p, sc, s.st.p, par,
+      s.part_s.p);
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
-    L(k_ru_update, GV, B, n, s.x.p, s.d32.p, s.r32.p, s.p32.p, s.q32.p, invd, s.st.p, par, s.part_s.p, s.g_mat,
-      s.part_u.p);
-    L(k_ru_dir, GV, B, n, s.d32.p, s.p32.p, s.r32.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.res.p, rh);
+    L(k_ru_update, GV, B, n, s.r32.p, s.p32.p, s.q32.p, s.st.p, par, s.part_s.p, s.g_mat, s.part_u.p);
+    L(k_ru_dir, GV, B, n, s.d32.p, s.p32.p, s.r32.p, s.x.p, sc, s.st.p, par, s.part_u.p, s.g_vec, s.res.p, rh);
   }
 }
 
@@ -1377,7 +1443,7 @@ void begin(femhip_solver &s, bool x_zero) {
                        x_zero ? 1 : 0, s.r.p, s.p.p, jac ? s.invd.p : nullptr, s.part_i.p);
   } else {
     hipLaunchKernelGGL(k_ru_init, dim3(s.g_mat), dim3(kThreads), 0, st, m, op.sl.v64.p, s.b.p, s.x.p, x_zero ? 1 : 0,
-                       s.r.p, s.r32.p, s.p32.p, s.d32.p, jac ? s.invd32.p : nullptr, s.part_i.p);
+                       s.r.p, s.r32.p, s.p32.p, s.d32.p, jac ? s.invd.p : nullptr, s.part_i.p);
   }
   hipLaunchKernelGGL(mixed(s) ? k_ru_init_fin : k_init_fin, dim3(1), dim3(kThreads), 0, st, s.part_i.p, s.g_mat,
                      s.st.p, s.res.p, (const int *)(jac ? s.d_flag : nullptr));
@@ -1505,7 +1571,7 @@ size_t solver_device_bytes(const femhip_solver &s) {
   size_t b = s.st.bytes() + s.res.bytes();
   for (auto *v : {&s.x, &s.b, &s.r, &s.p, &s.q, &s.invd, &s.part_i, &s.part_s, &s.part_u, &s.part_r, &s.hist})
     b += v->bytes();
-  for (auto *v : {&s.r32, &s.p32, &s.q32, &s.d32, &s.invd32}) b += v->bytes();
+  for (auto *v : {&s.r32, &s.p32, &s.q32, &s.d32}) b += v->bytes();
   return b;
 }
 
@@ -1597,9 +1663,10 @@ int femhip_solver_profile_spmv(femhip_solver_t *s, int32_t reps, double *ms_per_
     hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s->n, s->p.p);
     if (mx) hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, st, (int64_t)s->n, s->p.p, s->p32.p);
     const SlicedDev m = sliced_dev(op);
+    const SlicedDev m32 = fp32_view(*s, m);
     auto launch = [&] {
       if (mx)
-        hipLaunchKernelGGL((k_spmv<float, float>), dim3(s->g_mat), dim3(kThreads), 0, st, m, op.sl.v32.p, s->p32.p,
+        hipLaunchKernelGGL((k_spmv<float, float>), dim3(s->g_mat), dim3(kThreads), 0, st, m32, op.sl.v32.p, s->p32.p,
                            s->q32.p, s->st.p, 0, s->part_s.p);
       else
         hipLaunchKernelGGL((k_spmv<double, double>), dim3(s->g_mat), dim3(kThreads), 0, st, m, op.sl.v64.p, s->p.p,
@@ -1619,13 +1686,15 @@ int femhip_solver_profile_spmv(femhip_solver_t *s, int32_t reps, double *ms_per_
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (ms_per_launch) *ms_per_launch = ms / reps;
-    // algorithmic bytes of the sliced format: values + per-slice metadata +
+    // algorithmic bytes of the sliced format: values (without the unit
+    // diagonal the scaled FP32 copy does not read) + per-slice metadata +
     // diagonal offsets + ELL indices + row lists of sorted windows + one read
     // of p and one write of q
     const double vb = mx ? 4.0 : 8.0;
     const int64_t ell = (int64_t)(op.sl.ecol.n > 1 ? op.sl.ecol.n : 0);
+    const double unit = (mx && m32.unit) ? vb * (double)op.sl.stride : 0.0;
     if (bytes)
-      *bytes = op.sl.sym ? (double)op.sl.entries * vb + 2.0 * vb * op.n
+      *bytes = op.sl.sym ? (double)op.sl.entries * vb - unit + 2.0 * vb * op.n
                          : (double)op.sl.entries * vb + 16.0 * op.sl.ns + 4.0 * (double)op.sl.offs.n +
                                4.0 * (double)ell + 4.0 * kSlice * (double)op.sl.perm_slices + 2.0 * vb * op.n;
     return FEMHIP_OK;

@@ -158,6 +158,7 @@ struct Sliced {
   DevBuf<double> v64;
   DevBuf<float> v32;
   int64_t filled_version = -1, filled_version32 = -1;
+  bool scaled32 = false;          // v32 holds S A S (S = diag^(-1/2)) rather than A
   bool built = false;
 };
 }  // namespace femhip
@@ -199,7 +200,7 @@ struct femhip_operator {
   // the owner-computes path also writes the solver's symmetric-diagonal copy
   // when every neighbour code has one column offset over all rows: sym_kidx
   // [code] = stored diagonal of that offset (-1: lower part).  sym_written:
-  // copies the last compute wrote (1 FP64, 2 FP32).
+  // copies the last compute wrote (1: FP64; the FP32 copy is derived from it).
   int sym_gather = 0;                       // 0 not checked, 1 in use, -1 not applicable
   int8_t sym_kidx[32];
   int sym_written = 0;

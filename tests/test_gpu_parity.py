@@ -198,9 +198,9 @@ def _solver(env, op, precision):
 @pytest.mark.parametrize("name", ["2d_q1_64_sin", "3d_q1_cellk"])
 def test_recompute_writes_solver_copies(env, name):
     """Once solvers exist, the owner-computes assembly of a scalar Q1 operator
-    also writes their symmetric-diagonal copies (FP64 and FP32).  A recompute
-    with new coefficients must then solve exactly as a fresh operator, whose
-    copies are filled from the CSR."""
+    also writes their symmetric-diagonal FP64 copy (the FP32 copy is derived
+    from it).  A recompute with new coefficients must then solve exactly as a
+    fresh operator, whose copies are filled from the CSR."""
     print(f"[recompute] {name}", flush=True)
     osp, oph, fes, op, di = _build(env, name)
     op.compute()
