@@ -52,6 +52,10 @@ constexpr int kWaves = kThreads / 64;
 // block partials, so the two latencies overlap (the grids give about 4
 // elements per thread at 128^3).
 constexpr int kPrefetch = 4;
+// Reliable-update factor of the FP32 iterations: an update when the scaled
+// recurrence residual fell by this factor (measured at 128^3: 0.02 gives 263
+// inner + 7 updates, 0.1 gives 259 + 11, 0.005 gives 276 + 5).
+constexpr double kDefaultDelta = 0.02;
 
 struct CgState {
   double rz, rnrm_m, alpha, beta;
@@ -715,7 +719,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_ru_spmv(SlicedDev m, SlicedDev 
   if (!cur.ru) {
     float pq = 0.f, pp = 0.f;
     for (int s = s0; s < s1; s += sstep) {
-      const float a = slice_row<float, float>(m32, v32, p, s, lane);
+      // FP32 values and vectors, the row sum held in FP64 registers: 8 % fewer
+      // inner iterations at 128^3 (259 against 281), no extra memory traffic
+      const float a = (float)slice_row<float, double>(m32, v32, p, s, lane);
       const int row = slice_row_index(m, s, lane);
       if (row < m.n) {
         q[row] = a;
@@ -1341,15 +1347,24 @@ void enqueue_iterations(femhip_solver &s, int chunk, Launcher &L, CgResult *rh) 
   for (int i = 0; i < chunk; ++i) enqueue_one(s, i & 1, L, nullptr, i == chunk - 1 ? rh : nullptr);
 }
 
-bool use_graph() {
+// Chunks run as hipGraphs for small systems, whose iterations are shorter than
+// the host's launches; from about a million rows the GPU outpaces the host
+// anyway and plain stream launches were measured 1 % faster (128^3: CG 13.64
+// against 13.82 ms FP32-refined, 21.19 against 21.31 ms FP64).
+// FEMHIP_NO_GRAPH forces stream launches, FEMHIP_GRAPH=1 graphs.
+constexpr int32_t kGraphMaxRows = 1 << 20;
+bool use_graph(const femhip_solver &s) {
   static const bool no_graph = getenv("FEMHIP_NO_GRAPH") != nullptr;
-  return !no_graph;
+  static const char *force = getenv("FEMHIP_GRAPH");
+  if (no_graph) return false;
+  if (force && atoi(force) == 1) return true;
+  return s.n < kGraphMaxRows;
 }
 
 // build (once per precision, chunk size and result slot) the chunk as a hipGraph
 void ensure_graph(femhip_solver &s, int size_idx, int slot) {
   const int kind = mixed(s) ? 1 : 0;
-  if (!use_graph()) return;
+  if (!use_graph(s)) return;
   if (s.graph_kind != kind) {
     for (auto &row : s.graphs)
       for (auto &g : row)
@@ -1377,7 +1392,7 @@ void ensure_graph(femhip_solver &s, int size_idx, int slot) {
 // (mapped host memory), and the slot's event marks the chunk's completion
 void push_chunk(femhip_solver &s, int slot, int size_idx = 0) {
   hipStream_t st = s.ctx->stream;
-  if (!use_graph()) {
+  if (!use_graph(s)) {
     Launcher L;
     L.stream = st;
     enqueue_iterations(s, kChunkSizes[size_idx], L, s.d_res_map[slot]);
@@ -1428,6 +1443,8 @@ void iterate(femhip_solver &s, int maxit) {
       prev = r.prog;
       prev_it = r.prog_it;
     }
+    static const char *plan_env = getenv("FEMHIP_TAIL_CHUNKS");   // 0: always full chunks
+    if (plan_env && atoi(plan_env) == 0) need = 1e30;
     const double rest = need - (double)(queued - r.prog_it);   // beyond what is queued
     if (nq > 0 && rest <= 0.0) continue;
     push(rest > kChunk ? 0 : 1);
@@ -1441,7 +1458,7 @@ void upload_state(femhip_solver &s) {
   h.atol = s.prm.atol;
   h.maxit = s.prm.max_num_iterations;
   h.crit = s.prm.stopping_criterion;
-  h.delta = s.prm.inner_rtol > 0 ? s.prm.inner_rtol : 0.1;
+  h.delta = s.prm.inner_rtol > 0 ? s.prm.inner_rtol : kDefaultDelta;
   FH_HIP(hipMemcpyAsync(s.st.p, &h, sizeof(h), hipMemcpyHostToDevice, s.ctx->stream));
 }
 
