@@ -56,7 +56,11 @@ def test_serve_matches_oracle(env):
             x = _to_reference(sp, n, srv.get_solution(t))
             r = rhs - O.spmv(irp, ja, val, x, False)
             assert np.linalg.norm(r) <= 1.0001 * RTOL * np.linalg.norm(rhs), (n, np.linalg.norm(r))
-            assert abs(res["true_residual"] - np.linalg.norm(r)) <= 1e-6 * np.linalg.norm(rhs)
+            # the reported FP64 residual is that of the returned solution (the two
+            # sums differ only in rounding, far below the residual itself)
+            assert res["true_residual"] > 0
+            assert abs(res["true_residual"] - np.linalg.norm(r)) <= 1e-3 * np.linalg.norm(r) + 1e-13 * np.linalg.norm(rhs)
+            assert abs(res["rhs_norm"] - np.linalg.norm(rhs)) <= 1e-12 * np.linalg.norm(rhs)
             x_ref, info = O.cg(irp, ja, val, rhs, False, precond="jacobi", rtol=1e-12, maxit=50000)
             assert info["converged"]
             assert np.linalg.norm(x - x_ref) <= 1e-6 * np.linalg.norm(x_ref), n
