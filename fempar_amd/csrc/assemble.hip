@@ -363,6 +363,42 @@ struct SymOut {
   int8_t kidx[32];
 };
 
+// Outputs of one row from its neighbour values acc[code] and source sum bacc:
+// the CSR values staged in sv from `base` (the row's slots), fixed neighbours
+// lifted into the RHS, the RHS entry, and the solver's symmetric copy.
+template <int DIM>
+__device__ __forceinline__ void finish_row(const VtxGrid &g, int32_t r, const int (&a)[3],
+                                           const double (&acc)[DIM == 2 ? 9 : 27], double bacc,
+                                           const int32_t *__restrict__ vtx_dof, const uint8_t *__restrict__ slot,
+                                           int32_t base, double *sv, const double *__restrict__ fixed,
+                                           double *__restrict__ rhs, const SymOut &so) {
+  constexpr int K = DIM == 2 ? 9 : 27, KS = DIM == 2 ? 16 : 32;
+  const uint8_t *sr = slot + (int64_t)r * KS;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint8_t s = sr[k];
+    if (s != 255) {
+      sv[base + s] = acc[k];
+    } else if (fixed) {   // a fixed neighbour lifts its value into the RHS
+      const int d[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
+      bool in = true;
+      for (int e = 0; e < DIM; ++e) in = in && a[e] + d[e] >= 0 && a[e] + d[e] < g.nv[e];
+      if (in) {
+        const int32_t id = vtx_dof[(a[0] + d[0]) + g.nv[0] * ((a[1] + d[1]) + g.nv[1] * (DIM > 2 ? a[2] + d[2] : 0))];
+        if (id < 0) bacc -= acc[k] * fixed[-id - 1];
+      }
+    }
+  }
+  rhs[r] = bacc;
+  if (so.v64) {   // stores of one diagonal are coalesced over the block's rows
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int d = so.kidx[k];
+      if (d >= 0) so.v64[d * so.stride + r] = sr[k] != 255 ? acc[k] : 0.0;
+    }
+  }
+}
+
 // One thread per free row; a block owns RB consecutive rows, whose CSR values
 // are contiguous: they are staged in LDS and stored with coalesced writes.
 template <int DIM, int RB>
@@ -409,36 +445,108 @@ __global__ __launch_bounds__(RB) void k_assemble_gather(
       acc[code] += v;
     }
   }
-  const uint8_t *sr = slot + (int64_t)r * KS;
-  const int32_t base = rowptr[r] - b0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const uint8_t s = sr[k];
-    if (s != 255) {
-      sv[base + s] = acc[k];
-    } else if (fixed) {   // a fixed neighbour lifts its value into the RHS
-      const int d[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
-      bool in = true;
-      for (int e = 0; e < DIM; ++e) in = in && a[e] + d[e] >= 0 && a[e] + d[e] < g.nv[e];
-      if (in) {
-        const int32_t id = vtx_dof[(a[0] + d[0]) + g.nv[0] * ((a[1] + d[1]) + g.nv[1] * (DIM > 2 ? a[2] + d[2] : 0))];
-        if (id < 0) bacc -= acc[k] * fixed[-id - 1];
-      }
-    }
-  }
-  rhs[r] = bacc;
-  if (so.v64) {   // stores of one diagonal are coalesced over the block's rows
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int d = so.kidx[k];
-      if (d >= 0) {
-        so.v64[d * so.stride + r] = sr[k] != 255 ? acc[k] : 0.0;
-      }
-    }
-  }
+  finish_row<DIM>(g, r, a, acc, bacc, vtx_dof, slot, rowptr[r] - b0, sv, fixed, rhs, so);
   }
   __syncthreads();
   for (int32_t e = threadIdx.x; e < b1 - b0; e += RB) val[b0 + e] = sv[e];   // rows <= K entries each
+}
+
+// 3-D scalar Q1 with the row values formed on the matrix cores.  For a tile of
+// 16 rows, acc[row][code] = sum over (cell e around the vertex, point q) of
+// k_e(q) W[8 e + q][code] and the source sum b[row] = sum f_e(q) Nw[q][7 - e]
+// are two FP64 MFMA chains (v_mfma_f64_16x16x4: A = k or f of (row, e, q),
+// B = W or the Nw column), with W[8 e + q][code] = M[q][7 - e][j] for the
+// neighbour code of local vertex j seen from cell e and 0 otherwise.  K runs
+// over (e, q) with e ascending (ascending cell index) and every 4-wide step
+// inside one cell, so entries (r, c) and (c, r) see the same products in the
+// same order: the matrix stays bitwise symmetric.  A block holds 128 rows in
+// 2 waves of 4 tiles; the results pass through LDS to one thread per row.
+// Operand layouts of the instruction (checked on the device): A[m][k] in lane
+// m + 16 k, B[k][n] in lane n + 16 k, D[m][n] in lane n + 16 (m % 4), slot m / 4.
+constexpr int kMfmaRows = 128;
+typedef double v4d __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kMfmaRows) void k_assemble_gather_mfma(
+    VtxGrid g, PhysDev ph, TrigTab tt, int32_t n, const int32_t *__restrict__ row_vtx,
+    const int32_t *__restrict__ vtx_dof, const uint8_t *__restrict__ slot, const int32_t *__restrict__ rowptr,
+    const double *__restrict__ tab, const double *__restrict__ fixed, double *__restrict__ val,
+    double *__restrict__ rhs, SymOut so) {
+  constexpr int ND = 8, NQ = 8, K = 27, KK = 64, WC = 32;
+  __shared__ double lds[kMfmaRows * K];   // W (64 x 32), then the row values, then the staged CSR values
+  __shared__ double sb[kMfmaRows];
+  const double *M = tab;                  // M[NQ][ND][ND]
+  const double *Nw = tab + NQ * ND * ND;  // Nw[NQ][ND]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int32_t r0 = blockIdx.x * kMfmaRows;
+  for (int i = tid; i < KK * WC; i += kMfmaRows) {   // W; column 31 carries Nw for the source chain
+    const int kk = i / WC, code = i % WC, e = kk >> 3, q = kk & 7;
+    double w = 0.0;
+    if (code < K) {
+      const int jx = code % 3 - (e & 1), jy = (code / 3) % 3 - ((e >> 1) & 1), jz = code / 9 - (e >> 2);
+      if (jx >= 0 && jx <= 1 && jy >= 0 && jy <= 1 && jz >= 0 && jz <= 1)
+        w = M[(q * ND + (7 - e)) * ND + jx + 2 * jy + 4 * jz];
+    } else if (code == WC - 1) {
+      w = Nw[q * ND + (7 - e)];
+    }
+    lds[i] = w;
+  }
+  int ax[4][3];
+  bool ok[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {   // the lane's row in each of the wave's four tiles
+    const int32_t row = r0 + wv * 64 + mt * 16 + (lane & 15);
+    ok[mt] = row < n;
+    ax[mt][0] = ax[mt][1] = ax[mt][2] = 0;
+    if (ok[mt]) vtx_coords(g, row_vtx[row], ax[mt]);
+  }
+  __syncthreads();
+  v4d acc[4][2], accf[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    accf[mt] = v4d{0.0, 0.0, 0.0, 0.0};
+    acc[mt][0] = acc[mt][1] = accf[mt];
+  }
+  for (int ks = 0; ks < KK / 4; ++ks) {
+    const int kk = ks * 4 + (lane >> 4), e = kk >> 3, q = kk & 7;
+    const double w0 = lds[kk * WC + (lane & 15)], w1 = lds[kk * WC + 16 + (lane & 15)];
+    const double wf = (lane & 15) == 0 ? lds[kk * WC + WC - 1] : 0.0;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      double kq = 0.0, fq = 0.0;
+      const int ci = ax[mt][0] - 1 + (e & 1), cj = ax[mt][1] - 1 + ((e >> 1) & 1), ck = ax[mt][2] - 1 + (e >> 2);
+      if (ok[mt] && ci >= 0 && ci < g.n[0] && cj >= 0 && cj < g.n[1] && ck >= 0 && ck < g.n[2]) {
+        const int64_t cell = ci + (int64_t)g.n[0] * (cj + (int64_t)g.n[1] * ck);
+        const int ijk[3] = {ci, cj, ck};
+        coef_tab<3>(ph, tt, ijk, q, cell, kq, fq);
+      }
+      acc[mt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(kq, w0, acc[mt][0], 0, 0, 0);
+      acc[mt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(kq, w1, acc[mt][1], 0, 0, 0);
+      accf[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fq, wf, accf[mt], 0, 0, 0);
+    }
+  }
+  __syncthreads();   // W is no longer read: lds takes the row values
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int lr = wv * 64 + mt * 16 + (lane >> 4) + 4 * i;
+      const int c0 = lane & 15;
+      lds[lr * K + c0] = acc[mt][0][i];
+      if (16 + c0 < K) lds[lr * K + 16 + c0] = acc[mt][1][i];
+      if (c0 == 0) sb[lr] = accf[mt][i];
+    }
+  __syncthreads();
+  const int32_t r = r0 + tid;
+  double av[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) av[k] = lds[tid * K + k];
+  const double bacc = sb[tid];
+  int a[3] = {0, 0, 0};
+  if (r < n) vtx_coords(g, row_vtx[r], a);
+  const int32_t b0 = rowptr[r0], b1 = rowptr[min(n, r0 + kMfmaRows)];
+  __syncthreads();   // lds now stages the block's CSR values
+  if (r < n) finish_row<3>(g, r, a, av, bacc, vtx_dof, slot, rowptr[r] - b0, lds, fixed, rhs, so);
+  __syncthreads();
+  for (int32_t e = tid; e < b1 - b0; e += kMfmaRows) val[b0 + e] = lds[e];   // rows <= K entries each
 }
 
 __global__ void k_spmv_csr(int32_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
@@ -647,9 +755,19 @@ void launch_gather(femhip_operator &op, const PhysDev &ph, const double *d_fixed
     for (int k = 0; k < 32; ++k) so.kidx[k] = op.sym_kidx[k];
     op.sym_written = 1;
   }
+  // FEMHIP_ASSEMBLY_MFMA=1: 3-D rows on the matrix cores.  Not the default: on
+  // gfx950 FP64 MFMA and FP64 vector FMA have the same peak, W is 8/27 dense and
+  // the source chain uses 1 of 16 columns, so the MFMA kernel issues about 5x the
+  // useful flops and took 0.91 ms against 0.44 ms at 128^3.  Read per call.
+  const char *mf = getenv("FEMHIP_ASSEMBLY_MFMA");
+  const bool mfma = mf && atoi(mf) == 1;
   if (s.fe.dim == 2)
     hipLaunchKernelGGL((k_assemble_gather<2, kGatherRows>), dim3(rb), dim3(kGatherRows), 0, st, g, ph, tt, op.n,
                        op.row_vtx.p, op.vtx_dof.p, op.slot.p, op.rowptr.p, op.tab.p, d_fixed, op.val.p, op.rhs.p, so);
+  else if (mfma)
+    hipLaunchKernelGGL(k_assemble_gather_mfma, dim3((unsigned)((op.n + kMfmaRows - 1) / kMfmaRows)), dim3(kMfmaRows),
+                       0, st, g, ph, tt, op.n, op.row_vtx.p, op.vtx_dof.p, op.slot.p, op.rowptr.p, op.tab.p, d_fixed,
+                       op.val.p, op.rhs.p, so);
   else
     hipLaunchKernelGGL((k_assemble_gather<3, kGatherRows>), dim3(rb), dim3(kGatherRows), 0, st, g, ph, tt, op.n,
                        op.row_vtx.p, op.vtx_dof.p, op.slot.p, op.rowptr.p, op.tab.p, d_fixed, op.val.p, op.rhs.p, so);
