@@ -237,6 +237,9 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help="only the serving sweep (printed as JSON)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only launch the FP64 and FP32 SpMV kernels 50 times each (for rocprofv3 --pmc runs)")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="only assemble the 128^3 problem 10 times with the vector kernel, then 10 times with "
+                         "the FP64 matrix-core kernel (for rocprofv3 --pmc runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device-to-device 5 times (counter calibration for --pmc runs)")
     args = ap.parse_args()
@@ -283,6 +286,20 @@ def main():
 
     if args.configs_only:
         print(json.dumps(other_configs()))
+        return
+    if args.assembly_only:
+        tria, fes, di, op = make_problem(F, env, n)
+        op.compute()                       # builds the owner-computes maps once
+        res = {}
+        for variant, flag in (("vector", "0"), ("mfma", "1")):
+            os.environ["FEMHIP_ASSEMBLY_MFMA"] = flag
+            ms = []
+            for _ in range(10):
+                op.compute()
+                ms.append(op.last_compute_ms)
+            res[variant] = {"compute_ms_median": float(np.median(ms))}
+        os.environ.pop("FEMHIP_ASSEMBLY_MFMA")
+        print(json.dumps(res))
         return
     if args.spmv_only:
         tria, fes, di, op = make_problem(F, env, n, order=args.order)

@@ -109,7 +109,6 @@ int femhip_space_create_structured(femhip_ctx_t *ctx, const femhip_mesh_desc_t *
   try {
     init_space(*s, ctx, mesh);
     number_structured(*s);
-    upload_dofs(*s);
   } catch (...) {
     femhip_space_destroy(s);
     throw;
@@ -141,6 +140,7 @@ int femhip_space_create_from_dofs(femhip_ctx_t *ctx, const femhip_mesh_desc_t *m
     s->n_free = n_free;
     s->n_fixed = n_fixed;
     upload_dofs(*s);
+    representatives_from_dofs(*s);
   } catch (...) {
     femhip_space_destroy(s);
     throw;
@@ -166,6 +166,8 @@ int femhip_space_info(const femhip_space_t *s, int64_t *num_cells, int32_t *nd, 
 int femhip_space_get_dofs(const femhip_space_t *s, int32_t *out) {
   FH_TRY
   FH_CHECK(s && out, FEMHIP_ERR_INVALID, "null argument");
+  FH_HIP(hipSetDevice(s->ctx->device));
+  ensure_host_dofs(*s);
   std::memcpy(out, s->dofs.data(), s->dofs.size() * sizeof(int32_t));
   return FEMHIP_OK;
   FH_CATCH
@@ -174,37 +176,11 @@ int femhip_space_get_dofs(const femhip_space_t *s, int32_t *out) {
 int femhip_space_interpolate_dirichlet(const femhip_space_t *s, const femhip_physics_t *phys, double *out) {
   FH_TRY
   FH_CHECK(s && phys && (out || s->n_fixed == 0), FEMHIP_ERR_INVALID, "null argument");
-  const RefFE &fe = s->fe;
-  const int dim = s->mesh.dim, nd = fe.nd, nn = fe.nnode;
   for (int32_t i = 0; i < s->n_fixed; ++i) out[i] = 0.0;
   if (phys->g_kind == FEMHIP_G_ZERO) return FEMHIP_OK;
   FH_CHECK(phys->g_kind == FEMHIP_G_LINEAR, FEMHIP_ERR_INVALID, "unknown Dirichlet function");
-  int n[3] = {s->mesh.n[0], dim > 1 ? s->mesh.n[1] : 1, dim > 2 ? s->mesh.n[2] : 1};
-  for (int64_t c = 0; c < s->ncells; ++c) {
-    const int32_t *cd = s->dofs.data() + c * nd;
-    int64_t r = c;
-    int ijk[3] = {0, 0, 0};
-    for (int k = 0; k < dim; ++k) {
-      ijk[k] = (int)(r % n[k]);
-      r /= n[k];
-    }
-    for (int l = 0; l < nd; ++l) {
-      if (cd[l] >= 0) continue;
-      int a = l % nn, rem = a;
-      double v = 0.0;
-      for (int k = 0; k < dim; ++k) {
-        int m = rem % fe.np1;
-        rem /= fe.np1;
-        // physical node position: vertex coordinate i/n mapped, midpoints for Q2 nodes
-        const double lo = s->mesh.limits[2 * k], hi = s->mesh.limits[2 * k + 1];
-        double x0 = (hi - lo) * ((double)ijk[k] / (double)n[k]) + lo;
-        double x1 = (hi - lo) * ((double)(ijk[k] + 1) / (double)n[k]) + lo;
-        double t = (double)m / (double)fe.order;
-        v += x0 * (1.0 - t) + x1 * t;
-      }
-      out[-cd[l] - 1] = v;
-    }
-  }
+  FH_HIP(hipSetDevice(s->ctx->device));
+  interpolate_dirichlet_linear(*s, out);   // g(x) = sum of the coordinates at each fixed node
   return FEMHIP_OK;
   FH_CATCH
 }
@@ -230,18 +206,9 @@ int femhip_operator_create(femhip_ctx_t *ctx, femhip_space_t *space, int32_t sym
   o->space = space;
   try {
     o->symmetric_storage = symmetric_storage ? 1 : 0;
-    build_pattern(*o);
-    o->rowptr.alloc((size_t)o->n + 1);
-    o->col.alloc((size_t)o->nnz_full);
+    build_pattern(*o);   // rowptr, col on the device
     o->val.alloc((size_t)o->nnz_full);
     o->rhs.alloc((size_t)std::max(1, o->n));
-    FH_HIP(hipMemcpy(o->rowptr.p, o->h_rowptr.data(), sizeof(int32_t) * (o->n + 1), hipMemcpyHostToDevice));
-    FH_HIP(hipMemcpy(o->col.p, o->h_col.data(), sizeof(int32_t) * (size_t)o->nnz_full, hipMemcpyHostToDevice));
-    if (o->symmetric_storage) {
-      o->upper_map.alloc((size_t)o->nnz_upper);
-      FH_HIP(hipMemcpy(o->upper_map.p, o->h_upper_map.data(), sizeof(int64_t) * (size_t)o->nnz_upper,
-                       hipMemcpyHostToDevice));
-    }
   } catch (...) {
     delete o;
     femhip_space_destroy(space);
@@ -266,6 +233,8 @@ int femhip_operator_info(const femhip_operator_t *op, int32_t *n, int64_t *nnz, 
 int femhip_operator_get_pattern(const femhip_operator_t *op, int32_t *irp, int32_t *ja) {
   FH_TRY
   FH_CHECK(op && irp && ja, FEMHIP_ERR_INVALID, "null argument");
+  FH_HIP(hipSetDevice(op->ctx->device));
+  ensure_host_pattern(*op);
   if (!op->symmetric_storage) {
     for (int32_t r = 0; r <= op->n; ++r) irp[r] = op->h_rowptr[(size_t)r] + 1;
     for (int64_t k = 0; k < op->nnz_full; ++k) ja[k] = op->h_col[(size_t)k] + 1;
@@ -324,9 +293,15 @@ int femhip_operator_get_values(const femhip_operator_t *op, double *val, double 
   if (val) {
     std::vector<double> full((size_t)op->nnz_full);
     FH_HIP(hipMemcpy(full.data(), op->val.p, sizeof(double) * full.size(), hipMemcpyDeviceToHost));
-    if (!op->symmetric_storage) std::memcpy(val, full.data(), sizeof(double) * full.size());
-    else
-      for (int64_t u = 0; u < op->nnz_upper; ++u) val[u] = full[(size_t)op->h_upper_map[(size_t)u]];
+    if (!op->symmetric_storage) {
+      std::memcpy(val, full.data(), sizeof(double) * full.size());
+    } else {   // entries on or above the diagonal, row by row
+      ensure_host_pattern(*op);
+      int64_t u = 0;
+      for (int32_t r = 0; r < op->n; ++r)
+        for (int64_t k = op->h_rowptr[(size_t)r]; k < op->h_rowptr[(size_t)r + 1]; ++k)
+          if (op->h_col[(size_t)k] >= r) val[u++] = full[(size_t)k];
+    }
   }
   if (rhs) FH_HIP(hipMemcpy(rhs, op->rhs.p, sizeof(double) * (size_t)op->n, hipMemcpyDeviceToHost));
   return FEMHIP_OK;

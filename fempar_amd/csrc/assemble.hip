@@ -699,7 +699,10 @@ void setup_sym_gather(femhip_operator &op) {
     bool full = true;
     for (int k = 0; k < K; ++k) full = full && sl[k] != 255;
     if (!full) continue;
-    for (int k = 0; k < K; ++k) off[k] = op.h_col[(size_t)op.h_rowptr[(size_t)r] + sl[k]] - (int32_t)r;
+    int32_t b = 0, cols[32];
+    FH_HIP(hipMemcpy(&b, op.rowptr.p + r, sizeof(int32_t), hipMemcpyDeviceToHost));
+    FH_HIP(hipMemcpy(cols, op.col.p + b, sizeof(int32_t) * (size_t)K, hipMemcpyDeviceToHost));
+    for (int k = 0; k < K; ++k) off[k] = cols[sl[k]] - (int32_t)r;
     found = true;
     break;
   }

@@ -114,8 +114,9 @@ struct femhip_space {
   femhip::RefFE fe;
   int64_t ncells = 0;
   int32_t n_free = 0, n_fixed = 0;
-  std::vector<int32_t> dofs;          // host copy (cells x nd), signed 1-based
+  std::vector<int32_t> dofs;          // host copy (cells x nd), signed 1-based; made on request
   femhip::DevBuf<int32_t> d_dofs;     // device: free -> row (0-based), fixed -> -(fixed index)-1
+  femhip::DevBuf<int64_t> d_rep;      // device: free row -> cell * nd + local DOF of one occurrence
 };
 
 namespace femhip {
@@ -172,14 +173,12 @@ struct femhip_operator {
   int symmetric_storage = 0;
   int32_t n = 0;
   int64_t nnz_full = 0, nnz_upper = 0;
-  // full CSR (solver format), 0-based on device
-  std::vector<int32_t> h_rowptr, h_col;     // host copies for the pattern API
+  // full CSR (solver format), 0-based, built on the device; host copies are
+  // made on request (pattern API, host-built solver formats)
+  std::vector<int32_t> h_rowptr, h_col;
   femhip::DevBuf<int32_t> rowptr, col;
   femhip::DevBuf<double> val, rhs;
   femhip::DevBuf<float> val32;              // FP32 copy for the refined solver (lazy)
-  // upper-triangle view: index into the full arrays
-  std::vector<int64_t> h_upper_map;
-  femhip::DevBuf<int64_t> upper_map;
   // physics
   femhip_physics_t phys{};
   bool has_physics = false;
@@ -213,9 +212,19 @@ struct femhip_operator {
 
 namespace femhip {
 
-// host setup (space.cpp)
+// device set-up (setup.hip): first-touch numbering of a structured space
+// (d_dofs, d_rep, n_free, n_fixed); representatives of a caller's numbering,
+// checked to conform to the mesh; the CSR pattern on the device (rowptr, col,
+// n, nnz_full, nnz_upper); host copies on request; the symmetric-diagonal
+// test (every row's column offsets in the longest row's set D, structurally
+// symmetric); the linear Dirichlet function at the fixed DOFs
 void number_structured(femhip_space &s);
+void representatives_from_dofs(femhip_space &s);
 void build_pattern(femhip_operator &op);
+void ensure_host_pattern(const femhip_operator &op);
+void ensure_host_dofs(const femhip_space &s);
+bool symmetric_offsets(const femhip_operator &op, std::vector<int32_t> &D);
+void interpolate_dirichlet_linear(const femhip_space &s, double *out_host);
 // sliced solver format (sliced.cpp)
 void build_sliced(femhip_operator &op);
 
@@ -257,13 +266,13 @@ size_t solver_device_bytes(const femhip_solver &s);
 // device memory held by an operator (CSR, solver format, tables) and its space
 inline size_t operator_device_bytes(const femhip_operator &op) {
   size_t b = op.rowptr.bytes() + op.col.bytes() + op.val.bytes() + op.rhs.bytes() + op.val32.bytes() +
-             op.upper_map.bytes() + op.k_cell.bytes() + op.tab.bytes() + op.fixed.bytes() + op.vtx_dof.bytes() +
+             op.k_cell.bytes() + op.tab.bytes() + op.fixed.bytes() + op.vtx_dof.bytes() +
              op.row_vtx.bytes() + op.slot.bytes() + op.trig.bytes();
   const Sliced &s = op.sl;
   b += s.soffs.bytes() + s.base.bytes() + s.meta.bytes() + s.srow.bytes() + s.aux.bytes() + s.offs.bytes() +
        s.ecol.bytes() +
        s.map.bytes() + s.v64.bytes() + s.v32.bytes();
-  if (op.space) b += op.space->d_dofs.bytes();
+  if (op.space) b += op.space->d_dofs.bytes() + op.space->d_rep.bytes();
   return b;
 }
 

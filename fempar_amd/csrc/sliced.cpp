@@ -20,42 +20,15 @@ void parallel_rows(int64_t n, F &&f) {
   for (auto &x : th) x.join();
 }
 
-bool row_has(const std::vector<int32_t> &rp, const std::vector<int32_t> &cl, int32_t r, int32_t c) {
-  return std::binary_search(cl.begin() + rp[(size_t)r], cl.begin() + rp[(size_t)r + 1], c);
-}
-
 // Symmetric-diagonal layout if the pattern is structurally symmetric and all
-// rows draw their offsets from one small symmetric set.
+// rows draw their offsets from one small symmetric set (checked on the device
+// against the longest row's offsets).
 bool try_build_symmetric_dia(femhip_operator &op) {
   Sliced &S = op.sl;
   const int32_t n = op.n;
-  const std::vector<int32_t> &rp = op.h_rowptr;
-  const std::vector<int32_t> &cl = op.h_col;
   if (n == 0) return false;
-  const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-  std::vector<std::vector<int32_t>> part(nth);
-  std::vector<char> bad(nth, 0);
-  parallel_rows(n, [&](unsigned t, int64_t lo, int64_t hi) {
-    std::vector<int32_t> &u = part[t];
-    for (int64_t r = lo; r < hi && !bad[t]; ++r) {
-      for (int32_t k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) {
-        const int32_t o = cl[(size_t)k] - (int32_t)r;
-        if (std::find(u.begin(), u.end(), o) == u.end()) {
-          u.push_back(o);
-          if (u.size() > 64) { bad[t] = 1; break; }
-        }
-        if (!row_has(rp, cl, cl[(size_t)k], (int32_t)r)) { bad[t] = 1; break; }
-      }
-    }
-  });
   std::vector<int32_t> D;
-  for (unsigned t = 0; t < nth; ++t) {
-    if (bad[t]) return false;
-    D.insert(D.end(), part[t].begin(), part[t].end());
-  }
-  std::sort(D.begin(), D.end());
-  D.erase(std::unique(D.begin(), D.end()), D.end());
-  if (D.size() > 64) return false;
+  if (!symmetric_offsets(op, D)) return false;
   for (int32_t o : D)
     if (!std::binary_search(D.begin(), D.end(), -o)) return false;
   std::vector<int32_t> up;
@@ -64,7 +37,7 @@ bool try_build_symmetric_dia(femhip_operator &op) {
   if (up.empty() || up[0] != 0) return false;
   const int64_t stride = ((int64_t)n + kSlice - 1) / kSlice * kSlice;
   const int64_t E = (int64_t)up.size() * stride;
-  const int64_t stored = ((int64_t)cl.size() + n) / 2;
+  const int64_t stored = (op.nnz_full + n) / 2;
   if (E > stored * 3 / 2 + 64 || E >= (int64_t)INT32_MAX) return false;
   // values are filled from the CSR row by row on the device (no entry map)
   S.sym = true;
@@ -101,6 +74,7 @@ void build_sliced(femhip_operator &op) {
   S.perm_slices = 0;
   const int32_t n = op.n;
   const int32_t ns = (n + kSlice - 1) / kSlice;
+  ensure_host_pattern(op);
   const std::vector<int32_t> &rp = op.h_rowptr;
   const std::vector<int32_t> &cl = op.h_col;
 

@@ -139,12 +139,14 @@ const char *femhip_last_error(void);
  * (sbm_serial_triangulation.i90:61-118, uniform_hex_mesh_generator.f90:145-647)
  * + serial_fe_space_t%create: strong Dirichlet numbering
  * (sbm_serial_fe_space.i90:525-715) and first-touch free numbering
- * (sbm_serial_fe_space.i90:2623-2922), and uploads the cell DOF lists. */
+ * (sbm_serial_fe_space.i90:2623-2922), computed on the device (the cell DOF
+ * lists stay in HBM; femhip_space_get_dofs copies them out). */
 int femhip_space_create_structured(femhip_ctx_t *ctx, const femhip_mesh_desc_t *mesh,
                                    femhip_space_t **space);
 /* Drop-in form: the caller (FEMPAR's fe_space) already numbered the DOFs and
  * passes lst_dofs_gids (num_cells x nd, cell-major, reference local order,
- * fe_space.f90:636-637). */
+ * fe_space.f90:636-637).  Every free id must sit on one vef of the mesh at one
+ * position of its own-DOF list (FEMHIP_ERR_INVALID otherwise). */
 int femhip_space_create_from_dofs(femhip_ctx_t *ctx, const femhip_mesh_desc_t *mesh,
                                   const int32_t *lst_dofs_gids, int32_t n_free, int32_t n_fixed,
                                   femhip_space_t **space);
@@ -159,8 +161,8 @@ int femhip_space_destroy(femhip_space_t *space);
 
 /* ---- operator: fe_affine_operator_t + sparse_assembler_t + CSR sparse_matrix_t ----
  * femhip_operator_create replaces fe_affine_operator_t%create with
- * sparse_matrix_storage 'CSR' (sbm_fe_operator.i90:28-71, 162-308): it builds
- * the CSR pattern that sort_and_compress would produce
+ * sparse_matrix_storage 'CSR' (sbm_fe_operator.i90:28-71, 162-308): it builds,
+ * on the device, the CSR pattern that sort_and_compress would produce
  * (base_sparse_matrix.f90:4121-4829, csr_sparse_matrix.f90:416-479).
  * symmetric_storage = diagonal_blocks_symmetric_storage (upper triangle kept). */
 int femhip_operator_create(femhip_ctx_t *ctx, femhip_space_t *space, int32_t symmetric_storage,
