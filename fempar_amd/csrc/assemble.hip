@@ -1,5 +1,5 @@
-// assemble.hip -- cell-wise Galerkin integration + deterministic coloured
-// scatter-add into the CSR values and RHS on gfx950.
+// assemble.hip -- cell-wise Galerkin integration + deterministic owner-computes
+// assembly into the CSR values and RHS on gfx950.
 //
 // Mirrors the integrate_galerkin plugins
 //   Poisson:    Sources/Tests/Serial/test_poisson/modules/poisson_discrete_integration.f90:62-141
@@ -16,15 +16,15 @@
 // matrix factors as  elmat_c = sum_q k_c(x_q) * M_q  with M_q = detJ w_q G_q^T G_q
 // fixed per space (nq small nd x nd matrices), and elvec_c = sum_q f_c(x_q) Nw_q.
 //
-// Scalar Q1 (the CG workloads) is assembled owner-computes: one thread per
-// free row gathers the element entries of the cells around its vertex and
-// writes each value of its row once.  Other spaces use a coloured scatter:
-// cells are processed colour by colour (2^dim colours by cell-index parity),
-// cells of one colour share no DOF, so every (row, col) slot is updated by at
-// most one thread per launch.  Both are free of atomics and deterministic.
+// Every row is assembled by its owner, which sums the element entries of the
+// cells around its DOF in ascending cell index and writes each value of its
+// row once: scalar Q1 (the CG workloads) with one thread per row and per-row
+// slot maps, other spaces (Q2, vector Q1) with one wavefront per row.  Both
+// are free of atomics and deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 #include "femhip_internal.h"
@@ -35,190 +35,11 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-struct MeshDev {
-  int dim;
-  int n[3];
-  double lim[6];
-};
-
 struct PhysDev {
   int kind, k_kind, f_kind;
   double k0, f0;
   const double *k_cell;
 };
-
-__device__ __forceinline__ double vcoord(const MeshDev &m, int d, int i) {
-  // coordinates i/n mapped to the domain limits (uniform_hex_mesh_generator.f90:608-612, 633-635)
-  double c = (double)i / (double)m.n[d];
-  return (m.lim[2 * d + 1] - m.lim[2 * d]) * c + m.lim[2 * d];
-}
-
-__device__ double k_at(const PhysDev &ph, int dim, const double x[3], int64_t cell) {
-  if (ph.k_kind == FEMHIP_K_SIN3) {
-    double v = sin(2 * kPi * x[0]) * sin(2 * kPi * x[1]);
-    if (dim == 3) v *= sin(2 * kPi * x[2]);
-    return 1.0 + 0.5 * v;
-  }
-  if (ph.k_kind == FEMHIP_K_CELL) return ph.k_cell[cell];
-  return ph.k0;
-}
-
-__device__ double f_at(const PhysDev &ph, int dim, const double x[3], int64_t cell) {
-  if (ph.f_kind == FEMHIP_F_MANUFACTURED) {
-    double su[3] = {1, 1, 1}, cu[3] = {0, 0, 0};
-    for (int k = 0; k < dim; ++k) {
-      su[k] = sin(kPi * x[k]);
-      cu[k] = cos(kPi * x[k]);
-    }
-    double u = su[0] * su[1] * su[2];
-    double kk = k_at(ph, dim, x, cell);
-    double f = kk * dim * kPi * kPi * u;
-    if (ph.k_kind == FEMHIP_K_SIN3) {
-      double s2[3] = {1, 1, 1}, c2[3] = {0, 0, 0};
-      for (int k = 0; k < dim; ++k) {
-        s2[k] = sin(2 * kPi * x[k]);
-        c2[k] = cos(2 * kPi * x[k]);
-      }
-      double dot = kPi * c2[0] * s2[1] * s2[2] * kPi * cu[0] * su[1] * su[2] +
-                   kPi * s2[0] * c2[1] * s2[2] * kPi * su[0] * cu[1] * su[2];
-      if (dim == 3) dot += kPi * s2[0] * s2[1] * c2[2] * kPi * su[0] * su[1] * cu[2];
-      f -= dot;
-    }
-    return f;
-  }
-  if (ph.f_kind == FEMHIP_F_CELL_SIN) {
-    double u = 1.0;
-    for (int k = 0; k < dim; ++k) u *= sin(kPi * x[k]);
-    return ph.f0 * k_at(ph, dim, x, cell) * dim * kPi * kPi * u;
-  }
-  return ph.f0;
-}
-
-__global__ void k_fill_zero(double *__restrict__ a, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) a[i] = 0.0;
-}
-
-// position of column c in the sorted row segment col[b, e); -1 if absent
-__device__ __forceinline__ int64_t find_col(const int32_t *__restrict__ col, int64_t b, int64_t e, int32_t c) {
-  int64_t lo = b, hi = e - 1;
-  while (lo <= hi) {
-    int64_t mid = (lo + hi) >> 1;
-    int32_t v = col[mid];
-    if (v == c) return mid;
-    if (v < c) lo = mid + 1;
-    else hi = mid - 1;
-  }
-  return -1;
-}
-
-// Cell (i,j,k) of colour `color` with index t inside the colour.
-__device__ __forceinline__ int64_t color_cell(const MeshDev &m, int color, int64_t t, int ijk[3]) {
-  int64_t rem = t;
-  int64_t cell = 0, mul = 1;
-  for (int d = 0; d < m.dim; ++d) {
-    int b = (color >> d) & 1;
-    int cnt = (m.n[d] - b + 1) / 2;
-    int a = (int)(rem % cnt);
-    rem /= cnt;
-    ijk[d] = 2 * a + b;
-    cell += mul * ijk[d];
-    mul *= m.n[d];
-  }
-  for (int d = m.dim; d < 3; ++d) ijk[d] = 0;
-  return cell;
-}
-
-// Poisson: ND local DOFs, NQ quadrature points, CPB cells per block, one
-// thread per (cell, local row).
-template <int DIM, int ND, int NQ, int CPB>
-__global__ __launch_bounds__(CPB *ND) void k_assemble_poisson(
-    MeshDev m, PhysDev ph, int color, int64_t ncolor_cells, const int32_t *__restrict__ dofs,
-    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ tab,
-    const double *__restrict__ fixed, double *__restrict__ val, double *__restrict__ rhs) {
-  // tab layout: M[NQ][ND][ND] | Nw[NQ][ND] | G[NQ][2^DIM]
-  const double *M = tab;
-  const double *Nw = tab + NQ * ND * ND;
-  const double *G = Nw + NQ * ND;
-  __shared__ double s_k[CPB][NQ];
-  __shared__ double s_f[CPB][NQ];
-  __shared__ int64_t s_cell[CPB];
-  const int64_t base = (int64_t)blockIdx.x * CPB;
-
-  for (int t = threadIdx.x; t < CPB * NQ; t += blockDim.x) {
-    int lc = t / NQ, q = t % NQ;
-    int64_t tc = base + lc;
-    if (tc >= ncolor_cells) continue;
-    int ijk[3];
-    int64_t cell = color_cell(m, color, tc, ijk);
-    if (q == 0) s_cell[lc] = cell;
-    // x_q = sum_v N_v(q) x_v (sbm_cell_map.i90:673-687)
-    double x[3] = {0.0, 0.0, 0.0};
-    for (int v = 0; v < (1 << DIM); ++v) {
-      double gv = G[q * (1 << DIM) + v];
-      for (int d = 0; d < DIM; ++d) x[d] += vcoord(m, d, ijk[d] + ((v >> d) & 1)) * gv;
-    }
-    s_k[lc][q] = k_at(ph, DIM, x, cell);
-    s_f[lc][q] = f_at(ph, DIM, x, cell);
-  }
-  __syncthreads();
-
-  const int lc = threadIdx.x / ND, i = threadIdx.x % ND;
-  if (base + lc >= ncolor_cells) return;
-  const int64_t cell = s_cell[lc];
-  const int32_t *cd = dofs + cell * ND;
-  const int32_t gi = cd[i];
-  if (gi < 0) return;   // fixed row: dropped (coo append skips ia < 1)
-  const int64_t rb = rowptr[gi], re = rowptr[gi + 1];
-  double bacc = 0.0;
-  for (int q = 0; q < NQ; ++q) bacc += s_f[lc][q] * Nw[q * ND + i];
-#pragma unroll 4
-  for (int j = 0; j < ND; ++j) {
-    double e = 0.0;
-    for (int q = 0; q < NQ; ++q) e += s_k[lc][q] * M[(q * ND + i) * ND + j];
-    const int32_t gj = cd[j];
-    if (gj >= 0) {
-      int64_t pos = find_col(col, rb, re, gj);
-      if (pos >= 0) val[pos] += e;
-    } else if (fixed) {
-      bacc -= e * fixed[-gj - 1];
-    }
-  }
-  rhs[gi] += bacc;
-}
-
-// Elasticity with constant (lambda, mu) and body force: the element matrix and
-// vector are the same for every cell; only the scatter runs per cell.
-template <int ND, int CPB>
-__global__ __launch_bounds__(CPB *ND) void k_assemble_const(
-    MeshDev m, int color, int64_t ncolor_cells, const int32_t *__restrict__ dofs,
-    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ tab,
-    const double *__restrict__ fixed, double *__restrict__ val, double *__restrict__ rhs) {
-  const double *E = tab;          // elmat, row-major [ND][ND]
-  const double *F = tab + ND * ND;
-  const int64_t tc = (int64_t)blockIdx.x * CPB + threadIdx.x / ND;
-  const int i = threadIdx.x % ND;
-  if (tc >= ncolor_cells) return;
-  int ijk[3];
-  const int64_t cell = color_cell(m, color, tc, ijk);
-  const int32_t *cd = dofs + cell * ND;
-  const int32_t gi = cd[i];
-  if (gi < 0) return;
-  const int64_t rb = rowptr[gi], re = rowptr[gi + 1];
-  double bacc = F[i];
-  for (int j = 0; j < ND; ++j) {
-    const double e = E[i * ND + j];
-    const int32_t gj = cd[j];
-    if (gj >= 0) {
-      int64_t pos = find_col(col, rb, re, gj);
-      if (pos >= 0) val[pos] += e;
-    } else if (fixed) {
-      bacc -= e * fixed[-gj - 1];
-    }
-  }
-  rhs[gi] += bacc;
-}
 
 // ---------------------------------------------- owner-computes scalar Q1 ---
 // Every free DOF of a scalar Q1 space sits on one vertex, and its CSR row holds
@@ -316,21 +137,26 @@ __global__ void k_check_code_offsets(int32_t n, const uint8_t *__restrict__ slot
 // On a uniform mesh the d-th coordinate of quadrature point q of cell (i,j,k)
 // depends only on (i_d, the d-th 1-D Gauss index of q), so the trigonometric
 // factors of the coefficient and source are tabulated per direction:
-// trig[off[d] + i_d * 2 + a] = {sin(pi x), cos(pi x), sin(2 pi x), cos(2 pi x)}.
+// trig[off[d] + i_d * nq1 + a] = {sin(pi x), cos(pi x), sin(2 pi x), cos(2 pi x)}.
 struct TrigTab {
   const double4 *t;
   int off[3];
 };
 
-// k and f at a quadrature point from the tables: the same expressions as
-// k_at and f_at (the coefficient variant of poisson_discrete_integration.f90)
-template <int DIM>
+// k and f at quadrature point q (NQ1 Gauss points per direction, first index
+// fastest) from the tables: k = 1 + 0.5 sin 2pi x sin 2pi y sin 2pi z (or the
+// cell's value, or constant), f = -div(k grad u) for u = sin pi x sin pi y
+// sin pi z (the coefficient variant of poisson_discrete_integration.f90)
+template <int DIM, int NQ1 = 2>
 __device__ __forceinline__ void coef_tab(const PhysDev &ph, const TrigTab &tt, const int ijk[3], int q, int64_t cell,
                                          double &k, double &f) {
   double su[3] = {1, 1, 1}, cu[3] = {0, 0, 0}, s2[3] = {1, 1, 1}, c2[3] = {0, 0, 0};
+  int qr = q;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    const double4 v = tt.t[tt.off[d] + ijk[d] * 2 + ((q >> d) & 1)];
+    const int qd = qr % NQ1;
+    qr /= NQ1;
+    const double4 v = tt.t[tt.off[d] + ijk[d] * NQ1 + qd];
     su[d] = v.x;
     cu[d] = v.y;
     s2[d] = v.z;
@@ -351,6 +177,312 @@ __device__ __forceinline__ void coef_tab(const PhysDev &ph, const TrigTab &tt, c
     f = ph.f0 * k * DIM * kPi * kPi * (su[0] * su[1] * su[2]);
   } else {
     f = ph.f0;
+  }
+}
+
+// ----------------------------------------- owner-computes, any Lagrangian ---
+// One wavefront per free row (Q2 scalar, vector Q1, and scalar Q1 where the
+// vertex maps do not apply).  The row's DOF sits on the vef of its
+// representative occurrence (cell, local DOF); the cells around that vef are
+// visited in ascending cell index (the order in which the reference's COO
+// append + stable sort sums duplicates, base_sparse_matrix.f90:3469-3513,
+// 4121-4829), and each adds its element row for the row's local DOF: lane j
+// forms entry (l, j) and adds it to the row's value at the CSR slot of column
+// dofs[cell, j] (found by binary search in the row's sorted columns, kept in
+// LDS with the values); a fixed column lifts e g_j into the RHS in column
+// order, as impose_strong_dirichlet_bcs does (sbm_fe_cell_iterator.i90:
+// 1610-1637).  The row is written once, coalesced: no clearing pass, no
+// colours, no read-modify-write in HBM.
+// MODE 0: Poisson, entry = sum_q k(x_q) M_q[l][j], RHS sum_q f(x_q) Nw_q[l],
+// k and f from the per-direction sin/cos tables at the NQ1 1-D Gauss points;
+// MODE 1: one element matrix and vector for every cell (elasticity).
+constexpr int kOwnMaxRow = 128;
+template <int DIM, int ND, int NQ1, int MODE>
+__global__ __launch_bounds__(64) void k_assemble_owner(PhysDev ph, TrigTab tt, FaceTab t, int32_t n,
+                                                       const int64_t *__restrict__ rep,
+                                                       const int32_t *__restrict__ dofs,
+                                                       const int32_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ col, const double *__restrict__ tab,
+                                                       const double *__restrict__ fixed, double *__restrict__ val,
+                                                       double *__restrict__ rhs) {
+  constexpr int NQ = MODE == 0 ? (DIM == 2 ? NQ1 * NQ1 : NQ1 * NQ1 * NQ1) : 1;
+  __shared__ int32_t scol[kOwnMaxRow];
+  __shared__ double sacc[kOwnMaxRow];
+  __shared__ double sk[NQ], sf[NQ];
+  __shared__ double sfix[ND];
+  const double *M = tab;                         // MODE 0: M[NQ][ND][ND] | Nw[NQ][ND]
+  const double *Nw = tab + NQ * ND * ND;
+  const double *E = tab, *F = tab + ND * ND;     // MODE 1: element matrix (row-major), vector
+  const int lane = threadIdx.x;
+  const int32_t r = blockIdx.x;
+  const int32_t b0 = rowptr[r], len = rowptr[r + 1] - b0;
+  for (int i = lane; i < kOwnMaxRow; i += 64) {
+    scol[i] = i < len ? col[b0 + i] : INT_MAX;
+    sacc[i] = 0.0;
+  }
+  const int64_t c0 = rep[r];
+  const int64_t cell0 = c0 / ND;
+  const int l0 = (int)(c0 % ND), f = t.face_of[l0], pos = t.pos_of[l0];
+  int ijk[3];
+  cell_ijk(t, cell0, ijk);
+  int ax[3] = {0, 0, 0}, na = 0;
+  for (int k = 0; k < DIM; ++k)
+    if (!((t.type[f] >> k) & 1)) ax[na++] = k;
+  double bacc = 0.0;
+  __syncthreads();
+  for (int b = 0; b < (1 << na); ++b) {   // ascending cell index
+    int cj[3] = {ijk[0], ijk[1], ijk[2]}, anc = 0;
+    bool ok = true;
+    for (int e = 0; e < na; ++e) {
+      const int k = ax[e], w = ijk[k] + ((t.anchor[f] >> k) & 1);
+      cj[k] = w - 1 + ((b >> e) & 1);
+      ok = ok && cj[k] >= 0 && cj[k] < t.n[k];
+      anc |= (w - cj[k]) << k;
+    }
+    if (!ok) continue;   // the same for the whole block (one row)
+    const int lc = t.lst[t.ptr[t.face_by[t.type[f]][anc]] + pos];   // the row's DOF in this cell
+    const int64_t cell = cell_index(t, cj);
+    if (MODE == 0) {
+      if (lane < NQ) coef_tab<DIM, NQ1>(ph, tt, cj, lane, cell, sk[lane], sf[lane]);
+      __syncthreads();
+    }
+    if (lane < ND) {
+      double e;
+      if (MODE == 0) {
+        e = 0.0;
+        for (int q = 0; q < NQ; ++q) e += sk[q] * M[(q * ND + lc) * ND + lane];
+      } else {
+        e = E[lc * ND + lane];
+      }
+      const int32_t gj = dofs[cell * ND + lane];
+      double fx = 0.0;
+      if (gj >= 0) {
+        int lo = 0, hi = len - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (scol[mid] < gj) lo = mid + 1;
+          else hi = mid;
+        }
+        sacc[lo] += e;   // the cell's columns are distinct: one lane per slot
+      } else if (fixed) {
+        fx = e * fixed[-gj - 1];
+      }
+      sfix[lane] = fx;
+    }
+    __syncthreads();
+    if (lane == 0) {
+      double be;
+      if (MODE == 0) {
+        be = 0.0;
+        for (int q = 0; q < NQ; ++q) be += sf[q] * Nw[q * ND + lc];
+      } else {
+        be = F[lc];
+      }
+      for (int j = 0; j < ND; ++j) be -= sfix[j];
+      bacc += be;
+    }
+    __syncthreads();
+  }
+  for (int i = lane; i < len; i += 64) val[b0 + i] = sacc[i];
+  if (lane == 0) rhs[r] = bacc;
+}
+
+// ----------------------------------------------- 3-D Q2: element matrices ---
+// Phase 1 forms every cell's element matrix as one FP64 GEMM on the matrix
+// cores: elmat[cell][l j] = sum_q k_cell(q) M_q[l][j], i.e. (cells x 28) k
+// times (28 x 736) M (27 points and 729 entries, zero-padded), in
+// v_mfma_f64_16x16x4f64 tiles of 16 cells x 16 entries; a wave holds the k of
+// its 16 cells (7 K-steps) in registers and sweeps the 46 column tiles.  The
+// element vectors elvec[cell][l] = sum_q f_cell(q) Nw_q[l] are summed in q
+// order on the vector ALUs.  Operand layouts of the instruction: A[m][k] in
+// lane m + 16 k, B[k][n] in lane n + 16 k, D[m][n] in lane n + 16 (m % 4),
+// slot m / 4.
+constexpr int kQ2Nd = 27, kQ2Nq = 27, kQ2Ent = kQ2Nd * kQ2Nd, kQ2Tiles = (kQ2Ent + 15) / 16, kQ2Steps = 7;
+__global__ __launch_bounds__(64) void k_elmat_q2(PhysDev ph, TrigTab tt, FaceTab t, int64_t ncells,
+                                                 const double *__restrict__ tab, double *__restrict__ elmat,
+                                                 double *__restrict__ elvec) {
+  __shared__ double sf[16][kQ2Nq];
+  const double *M = tab;                          // M[q][l * 27 + j]
+  const double *Nw = tab + kQ2Nq * kQ2Ent;        // Nw[q][l]
+  const int lane = threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * 16;
+  const int m = lane & 15, kq = lane >> 4;
+  double a[kQ2Steps];
+  {
+    const int64_t cell = c0 + m;
+    int ijk[3] = {0, 0, 0};
+    if (cell < ncells) cell_ijk(t, cell, ijk);
+#pragma unroll
+    for (int st = 0; st < kQ2Steps; ++st) {
+      const int q = 4 * st + kq;
+      double kv = 0.0, fv = 0.0;
+      if (cell < ncells && q < kQ2Nq) coef_tab<3, 3>(ph, tt, ijk, q, cell, kv, fv);
+      a[st] = kv;
+    }
+  }
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  for (int tile = 0; tile < kQ2Tiles; ++tile) {
+    const int colb = tile * 16 + m;             // this lane's B column
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int st = 0; st < kQ2Steps; ++st) {
+      const int q = 4 * st + kq;
+      const double b = (q < kQ2Nq && colb < kQ2Ent) ? M[q * kQ2Ent + colb] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b, acc, 0, 0, 0);
+    }
+    const int col = tile * 16 + (lane & 15);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t cell = c0 + 4 * i + (lane >> 4);
+      if (cell < ncells && col < kQ2Ent) elmat[cell * kQ2Ent + col] = acc[i];
+    }
+  }
+  // element vectors: f at the points of the 16 cells, then sum_q f Nw in q order
+  for (int p = lane; p < 16 * kQ2Nq; p += 64) {
+    const int cm = p / kQ2Nq, q = p % kQ2Nq;
+    const int64_t cell = c0 + cm;
+    double kv = 0.0, fv = 0.0;
+    if (cell < ncells) {
+      int ijk[3];
+      cell_ijk(t, cell, ijk);
+      coef_tab<3, 3>(ph, tt, ijk, q, cell, kv, fv);
+    }
+    sf[cm][q] = fv;
+  }
+  __syncthreads();
+  for (int p = lane; p < 16 * kQ2Nd; p += 64) {
+    const int cm = p / kQ2Nd, l = p % kQ2Nd;
+    const int64_t cell = c0 + cm;
+    if (cell >= ncells) continue;
+    double be = 0.0;
+    for (int q = 0; q < kQ2Nq; ++q) be += sf[cm][q] * Nw[q * kQ2Nd + l];
+    elvec[cell * kQ2Nd + l] = be;
+  }
+}
+
+// Phase 2, one wavefront per free row: the cells around the row's vef in
+// ascending index (at most 8) and the row's local DOF in each; every (cell, j)
+// pair at once over the lanes: entry elmat[cell][l j], its column's CSR slot
+// (binary search in the row's sorted columns in LDS), or its Dirichlet lift
+// e g_j; then each slot sums its contributions in ascending cell order (the
+// reference's duplicate-summation order) and the row is written once; the RHS
+// is sum over cells of (elvec - the lifts in column order).
+constexpr int kQ2Rows = 1;   // rows (wavefronts) per block (4 measured slower: 6.0 against 5.0 ms)
+__global__ __launch_bounds__(64 * kQ2Rows) void k_gather_q2(FaceTab t, int32_t n, const int64_t *__restrict__ rep,
+                                                            const int32_t *__restrict__ dofs,
+                                                            const int32_t *__restrict__ rowptr,
+                                                            const int32_t *__restrict__ col,
+                                                            const double *__restrict__ elmat,
+                                                            const double *__restrict__ elvec,
+                                                            const double *__restrict__ fixed, double *__restrict__ val,
+                                                            double *__restrict__ rhs) {
+  constexpr int kCells = 8, kPairs = kCells * kQ2Nd;
+  __shared__ int32_t scol_[kQ2Rows][kOwnMaxRow];
+  __shared__ int16_t slist_[kQ2Rows][kOwnMaxRow][kCells];   // pair feeding (slot, cell), -1 if none
+  __shared__ double contrib_[kQ2Rows][kPairs];
+  __shared__ double lift_[kQ2Rows][kPairs];
+  __shared__ int64_t scell_[kQ2Rows][kCells];
+  __shared__ int slc_[kQ2Rows][kCells];
+  __shared__ double sbe_[kQ2Rows][kCells];
+  __shared__ int snc_[kQ2Rows];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int32_t *scol = scol_[w];
+  int16_t(*slist)[kCells] = slist_[w];
+  double *contrib = contrib_[w], *lift = lift_[w], *sbe = sbe_[w];
+  int64_t *scell = scell_[w];
+  int *slc = slc_[w];
+  const int32_t r = blockIdx.x * kQ2Rows + w;
+  const bool live = r < n;   // every wave takes part in the block's barriers
+  const int32_t b0 = live ? rowptr[r] : 0, len = live ? rowptr[r + 1] - b0 : 0;
+  for (int i = lane; i < kOwnMaxRow; i += 64) {
+    scol[i] = i < len ? col[b0 + i] : INT_MAX;
+    for (int c = 0; c < kCells; ++c) slist[i][c] = -1;
+  }
+  if (lane == 0) {   // the cells around the row's vef, ascending, and the row's DOF in each
+    int nc = 0;
+    if (live) {
+      const int64_t c0 = rep[r];
+      const int l0 = (int)(c0 % kQ2Nd), f = t.face_of[l0], pos = t.pos_of[l0];
+      int ijk[3];
+      cell_ijk(t, c0 / kQ2Nd, ijk);
+      int ax[3] = {0, 0, 0}, na = 0;
+      for (int k = 0; k < 3; ++k)
+        if (!((t.type[f] >> k) & 1)) ax[na++] = k;
+      for (int b = 0; b < (1 << na); ++b) {
+        int cj[3] = {ijk[0], ijk[1], ijk[2]}, anc = 0;
+        bool ok = true;
+        for (int e = 0; e < na; ++e) {
+          const int k = ax[e], wk = ijk[k] + ((t.anchor[f] >> k) & 1);
+          cj[k] = wk - 1 + ((b >> e) & 1);
+          ok = ok && cj[k] >= 0 && cj[k] < t.n[k];
+          anc |= (wk - cj[k]) << k;
+        }
+        if (!ok) continue;
+        scell[nc] = cell_index(t, cj);
+        slc[nc] = t.lst[t.ptr[t.face_by[t.type[f]][anc]] + pos];
+        ++nc;
+      }
+    }
+    snc_[w] = nc;
+  }
+  __syncthreads();
+  const int nc = snc_[w];
+  constexpr int kRounds = (kPairs + 63) / 64;
+  double ev[kRounds];
+  int32_t gv[kRounds];
+#pragma unroll
+  for (int k = 0; k < kRounds; ++k) {   // every load of the lane in flight before the searches
+    const int p = lane + 64 * k;
+    ev[k] = 0.0;
+    gv[k] = -1;
+    if (p < nc * kQ2Nd) {
+      const int ci = p / kQ2Nd, j = p % kQ2Nd;
+      const int64_t cell = scell[ci];
+      ev[k] = elmat[cell * kQ2Ent + slc[ci] * kQ2Nd + j];
+      gv[k] = dofs[cell * kQ2Nd + j];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kRounds; ++k) {
+    const int p = lane + 64 * k;
+    if (p >= nc * kQ2Nd) continue;
+    const int ci = p / kQ2Nd;
+    const double e = ev[k];
+    const int32_t gj = gv[k];
+    double lf = 0.0;
+    if (gj >= 0) {
+      int lo = 0, hi = len - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (scol[mid] < gj) lo = mid + 1;
+        else hi = mid;
+      }
+      slist[lo][ci] = (int16_t)p;   // a cell's columns are distinct: one pair per (slot, cell)
+      contrib[p] = e;
+    } else if (fixed) {
+      lf = e * fixed[-gj - 1];
+    }
+    lift[p] = lf;
+  }
+  __syncthreads();
+  for (int s = lane; s < len; s += 64) {
+    double v = 0.0;
+    for (int c = 0; c < nc; ++c) {
+      const int p = slist[s][c];
+      if (p >= 0) v += contrib[p];
+    }
+    val[b0 + s] = v;
+  }
+  if (lane < nc) {
+    double be = elvec[scell[lane] * kQ2Nd + slc[lane]];
+    for (int j = 0; j < kQ2Nd; ++j) be -= lift[lane * kQ2Nd + j];
+    sbe[lane] = be;
+  }
+  __syncthreads();
+  if (lane == 0 && live) {
+    double b = 0.0;
+    for (int c = 0; c < nc; ++c) b += sbe[c];
+    rhs[r] = b;
   }
 }
 
@@ -558,39 +690,41 @@ __global__ void k_spmv_csr(int32_t n, const int32_t *__restrict__ rowptr, const 
   y[r] = acc;
 }
 
-MeshDev mesh_dev(const femhip_mesh_desc_t &d) {
-  MeshDev m{};
-  m.dim = d.dim;
-  for (int k = 0; k < 3; ++k) m.n[k] = k < d.dim ? d.n[k] : 1;
-  for (int k = 0; k < 6; ++k) m.lim[k] = d.limits[k];
-  return m;
+
+void build_trig(femhip_operator &op);
+
+void launch_q2(femhip_operator &op, const PhysDev &ph, const double *d_fixed) {
+  femhip_space &s = *op.space;
+  if (op.n <= 0) return;
+  FH_CHECK(s.fe.order == 2 && s.fe.ncomp == 1 && s.fe.dim == 3, FEMHIP_ERR_INVALID, "not a scalar 3-D Q2 space");
+  if (op.trig.n == 0) build_trig(op);
+  const int64_t nc = s.ncells;   // rows of 5^3 = 125 <= kOwnMaxRow columns
+  if (op.elmat.n != (size_t)(nc * kQ2Ent)) {
+    op.elmat.alloc((size_t)(nc * kQ2Ent));
+    op.elvec.alloc((size_t)(nc * kQ2Nd));
+  }
+  const TrigTab tt{reinterpret_cast<const double4 *>(op.trig.p), {op.trig_off[0], op.trig_off[1], op.trig_off[2]}};
+  const FaceTab t = face_tab(s);
+  hipLaunchKernelGGL(k_elmat_q2, dim3((unsigned)((nc + 15) / 16)), dim3(64), 0, op.ctx->stream, ph, tt, t, nc, op.tab.p,
+                     op.elmat.p, op.elvec.p);
+  hipLaunchKernelGGL(k_gather_q2, dim3((unsigned)((op.n + kQ2Rows - 1) / kQ2Rows)), dim3(64 * kQ2Rows), 0,
+                     op.ctx->stream, t, op.n, s.d_rep.p, s.d_dofs.p, op.rowptr.p, op.col.p, op.elmat.p, op.elvec.p,
+                     d_fixed, op.val.p, op.rhs.p);
 }
 
-template <int DIM, int ND, int NQ, int CPB>
-void launch_poisson(femhip_operator &op, const MeshDev &m, const PhysDev &ph, const double *d_fixed) {
+template <int DIM, int ND, int NQ1, int MODE>
+void launch_owner(femhip_operator &op, const PhysDev &ph, const double *d_fixed) {
   femhip_space &s = *op.space;
-  for (int color = 0; color < (1 << DIM); ++color) {
-    int64_t cnt = 1;
-    for (int d = 0; d < DIM; ++d) cnt *= (m.n[d] - ((color >> d) & 1) + 1) / 2;
-    if (cnt == 0) continue;
-    int64_t blocks = (cnt + CPB - 1) / CPB;
-    hipLaunchKernelGGL((k_assemble_poisson<DIM, ND, NQ, CPB>), dim3((unsigned)blocks), dim3(CPB * ND), 0,
-                       op.ctx->stream, m, ph, color, cnt, s.d_dofs.p, op.rowptr.p, op.col.p, op.tab.p, d_fixed,
-                       op.val.p, op.rhs.p);
-  }
-}
-
-template <int ND, int CPB>
-void launch_const(femhip_operator &op, const MeshDev &m, const double *d_fixed) {
-  femhip_space &s = *op.space;
-  for (int color = 0; color < (1 << m.dim); ++color) {
-    int64_t cnt = 1;
-    for (int d = 0; d < m.dim; ++d) cnt *= (m.n[d] - ((color >> d) & 1) + 1) / 2;
-    if (cnt == 0) continue;
-    int64_t blocks = (cnt + CPB - 1) / CPB;
-    hipLaunchKernelGGL((k_assemble_const<ND, CPB>), dim3((unsigned)blocks), dim3(CPB * ND), 0, op.ctx->stream, m,
-                       color, cnt, s.d_dofs.p, op.rowptr.p, op.col.p, op.tab.p, d_fixed, op.val.p, op.rhs.p);
-  }
+  if (op.n <= 0) return;
+  // a row couples to the DOFs of the (2 p + 1)^d nodes around its own, per component
+  int row_max = s.fe.ncomp;
+  for (int d = 0; d < DIM; ++d) row_max *= 2 * s.fe.order + 1;
+  FH_CHECK(row_max <= kOwnMaxRow, FEMHIP_ERR_INVALID, "row longer than the owner-computes kernel holds");
+  if (MODE == 0 && op.trig.n == 0) build_trig(op);
+  const TrigTab tt{reinterpret_cast<const double4 *>(op.trig.p), {op.trig_off[0], op.trig_off[1], op.trig_off[2]}};
+  hipLaunchKernelGGL((k_assemble_owner<DIM, ND, NQ1, MODE>), dim3((unsigned)op.n), dim3(64), 0, op.ctx->stream, ph,
+                     tt, face_tab(s), op.n, s.d_rep.p, s.d_dofs.p, op.rowptr.p, op.col.p, op.tab.p, d_fixed, op.val.p,
+                     op.rhs.p);
 }
 
 VtxGrid vtx_grid(const femhip_mesh_desc_t &d) {
@@ -603,6 +737,38 @@ VtxGrid vtx_grid(const femhip_mesh_desc_t &d) {
   return g;
 }
 
+// Trig factors at the 1-D quadrature points of every cell layer: the d-th
+// coordinate of a quadrature point is x_i (1 - xi) / 2 + x_{i+1} (1 + xi) / 2
+// (the geometric Q1 map of sbm_cell_map.i90:673-687 along one direction).
+// Layout: per direction, [layer][1-D point] of (sin pi x, cos pi x, sin 2 pi x,
+// cos 2 pi x).
+void build_trig(femhip_operator &op) {
+  femhip_space &s = *op.space;
+  const RefFE &fe = s.fe;
+  std::vector<double> h;
+  int stride = 1;   // index step of the quadrature points along d (first index fastest)
+  for (int d = 0; d < fe.dim; ++d) {
+    op.trig_off[d] = (int)(h.size() / 4);
+    const int nd = s.mesh.n[d];
+    const double lo = s.mesh.limits[2 * d], hi = s.mesh.limits[2 * d + 1];
+    for (int i = 0; i < nd; ++i)
+      for (int a = 0; a < fe.nq1; ++a) {
+        const double xi = fe.qp[d][a * stride];   // quadrature point with 1-D index a along d, 0 elsewhere
+        const double x0 = (hi - lo) * ((double)i / (double)nd) + lo;
+        const double x1 = (hi - lo) * ((double)(i + 1) / (double)nd) + lo;
+        const double x = x0 * (0.5 * (1.0 - xi)) + x1 * (0.5 * (1.0 + xi));
+        h.push_back(std::sin(kPi * x));
+        h.push_back(std::cos(kPi * x));
+        h.push_back(std::sin(2 * kPi * x));
+        h.push_back(std::cos(2 * kPi * x));
+      }
+    stride *= fe.nq1;
+  }
+  op.trig.alloc(h.size());
+  FH_HIP(hipMemcpyAsync(op.trig.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, op.ctx->stream));
+  FH_HIP(hipStreamSynchronize(op.ctx->stream));
+}
+
 // One-time set-up of the owner-computes path (scalar Q1 Poisson only); the
 // maps are validated against the CSR pattern, and any mismatch keeps the
 // operator on the coloured scatter.
@@ -610,7 +776,7 @@ void setup_gather(femhip_operator &op) {
   femhip_space &s = *op.space;
   const RefFE &fe = s.fe;
   const int dim = fe.dim;
-  static const bool off = getenv("FEMHIP_NO_GATHER") != nullptr;
+  const bool off = getenv("FEMHIP_NO_GATHER") != nullptr;   // read per operator: tests route Q1 to k_assemble_owner
   if (off || op.tab_kind != 0 || fe.order != 1 || fe.ncomp != 1 || fe.nd != (1 << dim) || op.n <= 0) {
     op.gather = -1;
     return;
@@ -650,29 +816,7 @@ void setup_gather(femhip_operator &op) {
     op.gather = -1;
     return;
   }
-  // trig factors at the 1-D quadrature points of every cell layer: the d-th
-  // coordinate of a quadrature point is x_i (1 - xi) / 2 + x_{i+1} (1 + xi) / 2
-  // (the geometric Q1 map of sbm_cell_map.i90:673-687 along one direction)
-  std::vector<double> h;
-  for (int d = 0; d < dim; ++d) {
-    op.trig_off[d] = (int)(h.size() / 4);
-    const int nd = s.mesh.n[d];
-    const double lo = s.mesh.limits[2 * d], hi = s.mesh.limits[2 * d + 1];
-    for (int i = 0; i < nd; ++i)
-      for (int a = 0; a < 2; ++a) {
-        const double xi = fe.qp[d][a << d];   // quadrature point with 1-D index a along d, 0 elsewhere
-        const double x0 = (hi - lo) * ((double)i / (double)nd) + lo;
-        const double x1 = (hi - lo) * ((double)(i + 1) / (double)nd) + lo;
-        const double x = x0 * (0.5 * (1.0 - xi)) + x1 * (0.5 * (1.0 + xi));
-        h.push_back(std::sin(kPi * x));
-        h.push_back(std::cos(kPi * x));
-        h.push_back(std::sin(2 * kPi * x));
-        h.push_back(std::cos(2 * kPi * x));
-      }
-  }
-  op.trig.alloc(h.size());
-  FH_HIP(hipMemcpyAsync(op.trig.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, st));
-  FH_HIP(hipStreamSynchronize(st));
+  build_trig(op);
   op.gather = 1;
 }
 
@@ -817,7 +961,7 @@ void upload_physics(femhip_operator &op) {
   std::vector<double> t;
   if (ph.kind == FEMHIP_PHYS_POISSON) {
     FH_CHECK(fe.ncomp == 1, FEMHIP_ERR_INVALID, "Poisson plugin needs a scalar FE space");
-    t.assign((size_t)(nq * nd * nd + nq * nd + nq * (1 << dim)), 0.0);
+    t.assign((size_t)(nq * nd * nd + nq * nd), 0.0);
     for (int q = 0; q < nq; ++q) {
       double factor = fe.detj[q] * fe.qw[q];
       for (int i = 0; i < nd; ++i)
@@ -827,7 +971,6 @@ void upload_physics(femhip_operator &op) {
           t[(size_t)((q * nd + i) * nd + j)] = factor * dot;
         }
       for (int i = 0; i < nd; ++i) t[(size_t)(nq * nd * nd + q * nd + i)] = factor * fe.shp[i][q];
-      for (int v = 0; v < (1 << dim); ++v) t[(size_t)(nq * nd * nd + nq * nd + q * (1 << dim) + v)] = fe.gshp[v][q];
     }
     op.tab_kind = 0;
     if (ph.k_kind == FEMHIP_K_CELL) {
@@ -866,26 +1009,18 @@ void assemble(femhip_operator &op, const double *fixed_host, float *ms) {
   FH_HIP(hipEventCreate(&e0));
   FH_HIP(hipEventCreate(&e1));
   FH_HIP(hipEventRecord(e0, st));
-  MeshDev m = mesh_dev(s.mesh);
   PhysDev ph{op.phys.kind, op.phys.k_kind, op.phys.f_kind, op.phys.k0, op.phys.f0, op.k_cell.p};
-  if (gather) {   // every value and RHS entry is written once: no clearing pass
-    launch_gather(op, ph, d_fixed);
-  } else {
-    int zb = (int)std::min<int64_t>(4096, (op.nnz_full + 255) / 256 + 1);
-    hipLaunchKernelGGL(k_fill_zero, dim3(zb), dim3(256), 0, st, op.val.p, op.nnz_full);
-    hipLaunchKernelGGL(k_fill_zero, dim3(std::max(1, std::min(1024, (op.n + 255) / 256))), dim3(256), 0, st,
-                       op.rhs.p, (int64_t)op.n);
-  }
+  // every value and RHS entry is written once by either path: no clearing pass
   if (gather) {
-    // done above
+    launch_gather(op, ph, d_fixed);
   } else if (op.tab_kind == 0) {
-    if (fe.dim == 2 && fe.nd == 4) launch_poisson<2, 4, 4, 64>(op, m, ph, d_fixed);
-    else if (fe.dim == 2 && fe.nd == 9) launch_poisson<2, 9, 9, 28>(op, m, ph, d_fixed);
-    else if (fe.dim == 3 && fe.nd == 8) launch_poisson<3, 8, 8, 32>(op, m, ph, d_fixed);
-    else if (fe.dim == 3 && fe.nd == 27) launch_poisson<3, 27, 27, 8>(op, m, ph, d_fixed);
+    if (fe.dim == 2 && fe.nd == 4) launch_owner<2, 4, 2, 0>(op, ph, d_fixed);
+    else if (fe.dim == 2 && fe.nd == 9) launch_owner<2, 9, 3, 0>(op, ph, d_fixed);
+    else if (fe.dim == 3 && fe.nd == 8) launch_owner<3, 8, 2, 0>(op, ph, d_fixed);
+    else if (fe.dim == 3 && fe.nd == 27) launch_q2(op, ph, d_fixed);
     else throw Error{FEMHIP_ERR_INVALID, "unsupported Poisson element"};
   } else {
-    if (fe.nd == 24) launch_const<24, 10>(op, m, d_fixed);
+    if (fe.dim == 3 && fe.nd == 24) launch_owner<3, 24, 1, 1>(op, ph, d_fixed);
     else throw Error{FEMHIP_ERR_INVALID, "unsupported elasticity element"};
   }
   FH_HIP(hipGetLastError());

@@ -195,6 +195,9 @@ struct femhip_operator {
   femhip::DevBuf<int32_t> vtx_dof, row_vtx;
   femhip::DevBuf<uint8_t> slot;
   femhip::DevBuf<double> trig;              // 4 values per (direction, layer, 1-D point)
+  // 3-D Q2: every cell's element matrix (27 x 27, row-major) and vector,
+  // formed on the matrix cores, then gathered by the rows' owners
+  femhip::DevBuf<double> elmat, elvec;
   int trig_off[3] = {0, 0, 0};
   // the owner-computes path also writes the solver's symmetric-diagonal copy
   // when every neighbour code has one column offset over all rows: sym_kidx
@@ -211,6 +214,34 @@ struct femhip_operator {
 };
 
 namespace femhip {
+
+// n-face tables of a space's reference FE and its mesh, passed to kernels by
+// value: per n-face its type (tangent axes), anchor, own DOFs; per local DOF
+// its n-face and position in that face's own list; (type, anchor) -> n-face
+struct FaceTab {
+  int32_t dim, nd, nnf, fc, bc, np1, nnode, order;
+  int32_t n[3];
+  int32_t type[kMaxNf], anchor[kMaxNf], cnt[kMaxNf], ptr[kMaxNf + 1];
+  int8_t lst[kMaxNd];
+  int8_t face_of[kMaxNd];
+  int8_t pos_of[kMaxNd];
+  int8_t face_by[8][8];
+  double lo[3], hi[3];
+};
+FaceTab face_tab(const femhip_space &s);
+
+// (cell counts are below 2^31: 32-bit division)
+__device__ __forceinline__ void cell_ijk(const FaceTab &t, int64_t c, int (&ijk)[3]) {
+  int v = (int)c;
+  ijk[0] = v % t.n[0];
+  v /= t.n[0];
+  ijk[1] = v % t.n[1];
+  ijk[2] = v / t.n[1];
+}
+
+__device__ __forceinline__ int64_t cell_index(const FaceTab &t, const int (&ijk)[3]) {
+  return ijk[0] + (int64_t)t.n[0] * (ijk[1] + (int64_t)t.n[1] * ijk[2]);
+}
 
 // device set-up (setup.hip): first-touch numbering of a structured space
 // (d_dofs, d_rep, n_free, n_fixed); representatives of a caller's numbering,
@@ -267,7 +298,7 @@ size_t solver_device_bytes(const femhip_solver &s);
 inline size_t operator_device_bytes(const femhip_operator &op) {
   size_t b = op.rowptr.bytes() + op.col.bytes() + op.val.bytes() + op.rhs.bytes() + op.val32.bytes() +
              op.k_cell.bytes() + op.tab.bytes() + op.fixed.bytes() + op.vtx_dof.bytes() +
-             op.row_vtx.bytes() + op.slot.bytes() + op.trig.bytes();
+             op.row_vtx.bytes() + op.slot.bytes() + op.trig.bytes() + op.elmat.bytes() + op.elvec.bytes();
   const Sliced &s = op.sl;
   b += s.soffs.bytes() + s.base.bytes() + s.meta.bytes() + s.srow.bytes() + s.aux.bytes() + s.offs.bytes() +
        s.ecol.bytes() +

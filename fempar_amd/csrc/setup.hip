@@ -21,19 +21,6 @@
 
 namespace femhip {
 
-namespace {
-
-struct FaceTab {
-  int32_t dim, nd, nnf, fc, bc, np1, nnode, order;
-  int32_t n[3];
-  int32_t type[kMaxNf], anchor[kMaxNf], cnt[kMaxNf], ptr[kMaxNf + 1];
-  int8_t lst[kMaxNd];          // own DOFs of the n-faces (own_ptr ranges)
-  int8_t face_of[kMaxNd];      // n-face owning local DOF l ...
-  int8_t pos_of[kMaxNd];       // ... and l's position in its own list
-  int8_t face_by[8][8];        // (type, anchor) -> n-face, -1 if none
-  double lo[3], hi[3];
-};
-
 FaceTab face_tab(const femhip_space &s) {
   const RefFE &fe = s.fe;
   FaceTab t{};
@@ -68,16 +55,7 @@ FaceTab face_tab(const femhip_space &s) {
   return t;
 }
 
-__device__ __forceinline__ void cell_ijk(const FaceTab &t, int64_t c, int (&ijk)[3]) {
-  ijk[0] = (int)(c % t.n[0]);
-  c /= t.n[0];
-  ijk[1] = (int)(c % t.n[1]);
-  ijk[2] = (int)(c / t.n[1]);
-}
-
-__device__ __forceinline__ int64_t cell_index(const FaceTab &t, const int (&ijk)[3]) {
-  return ijk[0] + (int64_t)t.n[0] * (ijk[1] + (int64_t)t.n[1] * ijk[2]);
-}
+namespace {
 
 // the vef of n-face f of cell ijk carries the strong Dirichlet condition
 // (test_poisson_driver.f90:193-204: set_id 1 on the boundary; or the x = x_min plane)
