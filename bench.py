@@ -238,8 +238,9 @@ def main():
     ap.add_argument("--spmv-only", action="store_true",
                     help="only launch the FP64 and FP32 SpMV kernels 50 times each (for rocprofv3 --pmc runs)")
     ap.add_argument("--assembly-only", action="store_true",
-                    help="only assemble the 128^3 problem 10 times with the vector kernel, then 10 times with "
-                         "the FP64 matrix-core kernel (for rocprofv3 --pmc runs)")
+                    help="only assemble the --n^3 problem of --order 10 times (Q1: with the vector kernel, then "
+                         "10 times with the FP64 matrix-core kernel; Q2: element matrices on the matrix cores) "
+                         "(for rocprofv3 --pmc runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device-to-device 5 times (counter calibration for --pmc runs)")
     args = ap.parse_args()
@@ -288,10 +289,10 @@ def main():
         print(json.dumps(other_configs()))
         return
     if args.assembly_only:
-        tria, fes, di, op = make_problem(F, env, n)
+        tria, fes, di, op = make_problem(F, env, n, order=args.order)
         op.compute()                       # builds the owner-computes maps once
         res = {}
-        for variant, flag in (("vector", "0"), ("mfma", "1")):
+        for variant, flag in ((("vector", "0"), ("mfma", "1")) if args.order == 1 else (("mfma", "0"),)):
             os.environ["FEMHIP_ASSEMBLY_MFMA"] = flag
             ms = []
             for _ in range(10):

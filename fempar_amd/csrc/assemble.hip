@@ -367,7 +367,17 @@ __global__ __launch_bounds__(64) void k_elmat_q2(PhysDev ph, TrigTab tt, FaceTab
 // e g_j; then each slot sums its contributions in ascending cell order (the
 // reference's duplicate-summation order) and the row is written once; the RHS
 // is sum over cells of (elvec - the lifts in column order).
-constexpr int kQ2Rows = 1;   // rows (wavefronts) per block (4 measured slower: 6.0 against 5.0 ms)
+// Each wavefront works on its own row and LDS region; it orders its lanes'
+// LDS traffic with wavefront barriers only, so the block's waves never wait
+// for each other.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// rows (wavefronts) per block: the 64^3 Q2 assembly took 6.1-6.3 ms with 1,
+// 7.2 ms with 4 (block barriers or wavefront barriers alike)
+constexpr int kQ2Rows = 1;
 __global__ __launch_bounds__(64 * kQ2Rows) void k_gather_q2(FaceTab t, int32_t n, const int64_t *__restrict__ rep,
                                                             const int32_t *__restrict__ dofs,
                                                             const int32_t *__restrict__ rowptr,
@@ -392,15 +402,15 @@ __global__ __launch_bounds__(64 * kQ2Rows) void k_gather_q2(FaceTab t, int32_t n
   int64_t *scell = scell_[w];
   int *slc = slc_[w];
   const int32_t r = blockIdx.x * kQ2Rows + w;
-  const bool live = r < n;   // every wave takes part in the block's barriers
-  const int32_t b0 = live ? rowptr[r] : 0, len = live ? rowptr[r + 1] - b0 : 0;
+  if (r >= n) return;   // no block-wide barrier below
+  const int32_t b0 = rowptr[r], len = rowptr[r + 1] - b0;
   for (int i = lane; i < kOwnMaxRow; i += 64) {
     scol[i] = i < len ? col[b0 + i] : INT_MAX;
     for (int c = 0; c < kCells; ++c) slist[i][c] = -1;
   }
   if (lane == 0) {   // the cells around the row's vef, ascending, and the row's DOF in each
     int nc = 0;
-    if (live) {
+    {
       const int64_t c0 = rep[r];
       const int l0 = (int)(c0 % kQ2Nd), f = t.face_of[l0], pos = t.pos_of[l0];
       int ijk[3];
@@ -425,7 +435,7 @@ __global__ __launch_bounds__(64 * kQ2Rows) void k_gather_q2(FaceTab t, int32_t n
     }
     snc_[w] = nc;
   }
-  __syncthreads();
+  wave_sync();
   const int nc = snc_[w];
   constexpr int kRounds = (kPairs + 63) / 64;
   double ev[kRounds];
@@ -464,7 +474,7 @@ __global__ __launch_bounds__(64 * kQ2Rows) void k_gather_q2(FaceTab t, int32_t n
     }
     lift[p] = lf;
   }
-  __syncthreads();
+  wave_sync();
   for (int s = lane; s < len; s += 64) {
     double v = 0.0;
     for (int c = 0; c < nc; ++c) {
@@ -478,8 +488,8 @@ __global__ __launch_bounds__(64 * kQ2Rows) void k_gather_q2(FaceTab t, int32_t n
     for (int j = 0; j < kQ2Nd; ++j) be -= lift[lane * kQ2Nd + j];
     sbe[lane] = be;
   }
-  __syncthreads();
-  if (lane == 0 && live) {
+  wave_sync();
+  if (lane == 0) {
     double b = 0.0;
     for (int c = 0; c < nc; ++c) b += sbe[c];
     rhs[r] = b;
