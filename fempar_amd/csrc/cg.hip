@@ -948,11 +948,15 @@ __global__ void k_jacobi(int32_t n, const int32_t *__restrict__ rowptr, const in
 // The same from the sliced copy, for operators held without a CSR: the entry
 // of offset 0 of a diagonal slice, or the first ELL entry whose column is the
 // row (real entries precede the padding).
+// One thread per (slice, lane); a lane's row comes from the slice's row list
+// when the slice holds a length-sorted window.
 __global__ void k_jacobi_sliced(SlicedDev m, const double *__restrict__ v, double *__restrict__ out, int root,
                                 int *flag) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)m.ns * kSlice) return;
+  const int32_t s = (int32_t)(t / kSlice), lane = (int32_t)(t % kSlice);
+  const int32_t i = slice_row_index(m, s, lane);
   if (i >= m.n) return;
-  const int32_t s = i / kSlice, lane = i % kSlice;
   const int32_t meta = m.meta[s];
   const int32_t K = meta & ((1 << 28) - 1);
   const bool ell = (meta >> 30) & 1;
@@ -1057,12 +1061,25 @@ __global__ void k_scale32(SlicedDev m, const double *__restrict__ v64, const dou
   }
 }
 
+// operator order <-> solver order of a permuted layout: to[perm[i]] = from[i]
+// (padding rows of `to` cleared first), or to[i] = from[perm[i]]
+__global__ void k_permute_in(int32_t n, const int32_t *__restrict__ perm, const double *__restrict__ from,
+                             double *__restrict__ to) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    to[perm[i]] = from[i];
+}
+__global__ void k_permute_out(int32_t n, const int32_t *__restrict__ perm, const double *__restrict__ from,
+                              double *__restrict__ to) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    to[i] = from[perm[i]];
+}
+
 // gather CSR values into the sliced layout (FP64 and/or FP32)
 __global__ void k_fill_sliced(int64_t E, const int32_t *__restrict__ map, const double *__restrict__ val,
                               double *__restrict__ v64, float *__restrict__ v32) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
     const int32_t k = map[e];
-    const double a = k >= 0 ? val[k] : 0.0;
+    const double a = k >= 0 ? val[k] : (k == -2 ? 1.0 : 0.0);   // -2: identity row of a padded layout
     if (v64) v64[e] = a;
     if (v32) v32[e] = (float)a;
   }
@@ -1138,7 +1155,7 @@ namespace {
 SlicedDev sliced_dev(femhip_operator &op) {
   SlicedDev m;
   m.ns = op.sl.ns;
-  m.n = op.n;
+  m.n = op.sl.rows > 0 ? op.sl.rows : op.n;   // solver rows
   m.base = op.sl.base.p;
   m.meta = op.sl.meta.p;
   m.aux = op.sl.aux.p;
@@ -1179,10 +1196,10 @@ void prepare(femhip_solver &s) {
   femhip_operator &op = *s.op;
   FH_CHECK(op.computed, FEMHIP_ERR_STATE, "solve before fe_affine_operator%compute");
   hipStream_t st = s.ctx->stream;
-  const int32_t n = op.n;
   const bool jac = s.prm.preconditioner == FEMHIP_PREC_JACOBI;
   const bool mx = mixed(s);
   if (!op.sl.built) build_sliced(op);
+  const int32_t n = op.sl.rows > 0 ? op.sl.rows : op.n;   // solver rows (padded when permuted)
   if (op.sl.sym && op.gather == 1 && op.sym_gather == 0) setup_sym_gather(op);
   if (!s.prepared) {
     s.n = n;
@@ -1233,7 +1250,7 @@ void prepare(femhip_solver &s) {
     if (!op.sl.v64.p) op.sl.v64.alloc((size_t)std::max<int64_t>(1, op.sl.entries));
     if (op.sl.sym) {   // stride is a multiple of kFillRows; every row has at most 2 kplus - 1 entries
       const size_t lds = (size_t)kFillRows * (2 * op.sl.kplus - 1) * (sizeof(double) + sizeof(int32_t));
-      hipLaunchKernelGGL(k_fill_sym, dim3((unsigned)(op.sl.stride / kFillRows)), dim3(kFillRows), lds, st, n,
+      hipLaunchKernelGGL(k_fill_sym, dim3((unsigned)(op.sl.stride / kFillRows)), dim3(kFillRows), lds, st, op.n,
                          op.rowptr.p, op.col.p, op.val.p, op.sl.soffs.p, op.sl.kplus, op.sl.stride, op.sl.v64.p,
                          nullptr);
     } else {
@@ -1252,8 +1269,9 @@ void prepare(femhip_solver &s) {
     const int root = mx ? 1 : 0;
     if (op.sl.sym)
       hipLaunchKernelGGL(k_jacobi_sym, G, B, 0, st, n, op.sl.v64.p, s.invd.p, root, s.d_flag);
-    else if (op.structured)   // no CSR: the diagonal comes from the sliced copy
-      hipLaunchKernelGGL(k_jacobi_sliced, G, B, 0, st, sliced_dev(op), op.sl.v64.p, s.invd.p, root, s.d_flag);
+    else if (op.structured || op.sl.permuted)   // no CSR in the solver's order: the diagonal of the sliced copy
+      hipLaunchKernelGGL(k_jacobi_sliced, dim3((unsigned)(((int64_t)op.sl.ns * kSlice + 255) / 256)), B, 0, st,
+                         sliced_dev(op), op.sl.v64.p, s.invd.p, root, s.d_flag);
     else
       hipLaunchKernelGGL(k_jacobi, G, B, 0, st, n, op.rowptr.p, op.col.p, op.val.p, s.invd.p, root, s.d_flag);
     FH_HIP(hipGetLastError());
@@ -1463,6 +1481,39 @@ void upload_state(femhip_solver &s) {
 }
 
 // initial residual, direction and device state (x and b already in place)
+// A vector in operator order (the RHS or x0; device or host) into solver
+// vector dst: copied, or for a permuted layout staged in q (free before a
+// solve) and scattered to the solver rows, padding rows cleared.
+void load_vector(femhip_solver &s, double *dst, const double *src, hipMemcpyKind kind) {
+  hipStream_t st = s.ctx->stream;
+  femhip_operator &op = *s.op;
+  if (!op.sl.permuted) {
+    FH_HIP(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)s.n, kind, st));
+    return;
+  }
+  FH_HIP(hipMemcpyAsync(s.q.p, src, sizeof(double) * (size_t)op.n, kind, st));
+  hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s.n, dst);
+  hipLaunchKernelGGL(k_permute_in, dim3(1024), dim3(256), 0, st, op.n, op.sl.perm.p, s.q.p, dst);
+  FH_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+// The solution in operator order on the device: x itself, or for a permuted
+// layout gathered into q (valid until the next solve).  Synchronizes.
+const double *solution_operator_order(const femhip_solver &s) {
+  hipStream_t st = s.ctx->stream;
+  const femhip_operator &op = *s.op;
+  if (op.sl.permuted && op.n > 0) {
+    hipLaunchKernelGGL(k_permute_out, dim3(1024), dim3(256), 0, st, op.n, op.sl.perm.p, s.x.p, s.q.p);
+    FH_HIP(hipGetLastError());
+  }
+  FH_HIP(hipStreamSynchronize(st));
+  return op.sl.permuted ? s.q.p : s.x.p;
+}
+
+namespace {
+
 void begin(femhip_solver &s, bool x_zero) {
   hipStream_t st = s.ctx->stream;
   femhip_operator &op = *s.op;
@@ -1505,7 +1556,7 @@ void solve(femhip_solver &s, bool x_zero, femhip_cg_result_t *out) {
 // the iterations that ran (launches queued after convergence return at entry).
 void profiled_solve(femhip_solver &s, double *spmv_ms, int32_t *iterations) {
   hipStream_t st = s.ctx->stream;
-  FH_HIP(hipMemcpyAsync(s.b.p, s.op->rhs.p, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, st));
+  load_vector(s, s.b.p, s.op->rhs.p, hipMemcpyDeviceToDevice);
   hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s.n, s.x.p);
   begin(s, true);
   Launcher L;
@@ -1570,7 +1621,7 @@ void solver_start(femhip_solver &s) {
   prepare(s);
   FH_CHECK(s.n > 0, FEMHIP_ERR_STATE, "operator without free DOFs");
   hipStream_t st = s.ctx->stream;
-  FH_HIP(hipMemcpyAsync(s.b.p, s.op->rhs.p, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, st));
+  load_vector(s, s.b.p, s.op->rhs.p, hipMemcpyDeviceToDevice);
   hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s.n, s.x.p);
   begin(s, true);
   for (int slot = 0; slot < 2; ++slot) ensure_graph(s, 0, slot);
@@ -1595,7 +1646,7 @@ void solver_finish(femhip_solver &s, femhip_cg_result_t *out) {
 }
 
 void solver_copy_solution(const femhip_solver &s, double *x) {
-  FH_HIP(hipMemcpy(x, s.x.p, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToHost));
+  FH_HIP(hipMemcpy(x, solution_operator_order(s), sizeof(double) * (size_t)s.op->n, hipMemcpyDeviceToHost));
 }
 
 size_t solver_device_bytes(const femhip_solver &s) {
@@ -1652,12 +1703,12 @@ int femhip_solver_solve(femhip_solver_t *s, const double *b, double *x, femhip_c
     FH_HIP(hipEventCreate(&e1));
     FH_HIP(hipEventRecord(e0, st));
     prepare(*s);
-    const size_t bytes = sizeof(double) * (size_t)s->n;
+    const size_t bytes = sizeof(double) * (size_t)s->op->n;   // host vectors in operator order
     const bool x_zero = (x == nullptr);
     if (s->n > 0) {
-      if (b) FH_HIP(hipMemcpyAsync(s->b.p, b, bytes, hipMemcpyHostToDevice, st));
-      else FH_HIP(hipMemcpyAsync(s->b.p, s->op->rhs.p, bytes, hipMemcpyDeviceToDevice, st));
-      if (x) FH_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, st));
+      if (b) load_vector(*s, s->b.p, b, hipMemcpyHostToDevice);
+      else load_vector(*s, s->b.p, s->op->rhs.p, hipMemcpyDeviceToDevice);
+      if (x) load_vector(*s, s->x.p, x, hipMemcpyHostToDevice);
       else hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, (int64_t)s->n, s->x.p);
       solve(*s, x_zero, res);
     } else {
@@ -1673,7 +1724,7 @@ int femhip_solver_solve(femhip_solver_t *s, const double *b, double *x, femhip_c
     if (s->n > 0) res->true_residual = true_residual(*s);
     res->status = res->did_converge ? FEMHIP_OK : FEMHIP_ERR_NOT_CONVERGED;
     s->have_solution = true;
-    if (x && s->n > 0) FH_HIP(hipMemcpy(x, s->x.p, bytes, hipMemcpyDeviceToHost));
+    if (x && s->n > 0) FH_HIP(hipMemcpy(x, solution_operator_order(*s), bytes, hipMemcpyDeviceToHost));
     return FEMHIP_OK;
   } catch (const Error &e) {
     set_error(e.msg);
@@ -1759,7 +1810,8 @@ int femhip_solver_get_solution(const femhip_solver_t *s, double *x) {
     FH_CHECK(s && x, FEMHIP_ERR_INVALID, "null argument");
     FH_CHECK(s->have_solution, FEMHIP_ERR_STATE, "no solution yet");
     FH_HIP(hipSetDevice(s->ctx->device));
-    if (s->n > 0) FH_HIP(hipMemcpy(x, s->x.p, sizeof(double) * (size_t)s->n, hipMemcpyDeviceToHost));
+    if (s->n > 0)
+      FH_HIP(hipMemcpy(x, solution_operator_order(*s), sizeof(double) * (size_t)s->op->n, hipMemcpyDeviceToHost));
     return FEMHIP_OK;
   } catch (const Error &e) {
     set_error(e.msg);
@@ -1773,7 +1825,7 @@ int femhip_solver_error_norms(femhip_solver_t *s, int32_t exact, femhip_error_no
     FH_CHECK(s->have_solution, FEMHIP_ERR_STATE, "no solution yet");
     FH_HIP(hipSetDevice(s->ctx->device));
     double out[4] = {0, 0, 0, 0};
-    error_norms(*s->op, s->x.p, exact, out);
+    error_norms(*s->op, solution_operator_order(*s), exact, out);
     norms->l2_norm = out[0];
     norms->h1_seminorm = out[1];
     norms->h1_norm = out[2];

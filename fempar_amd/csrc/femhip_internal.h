@@ -161,6 +161,12 @@ struct Sliced {
   int64_t filled_version = -1, filled_version32 = -1;
   bool scaled32 = false;          // v32 holds S A S (S = diag^(-1/2)) rather than A
   bool built = false;
+  // Solver rows: the operator's rows, or (permuted) the class-blocked order of
+  // a structured scalar space of order >= 2, padded with identity rows; perm
+  // maps an operator row to its solver row.
+  int32_t rows = 0;
+  bool permuted = false;
+  DevBuf<int32_t> perm;
 };
 }  // namespace femhip
 
@@ -300,7 +306,7 @@ inline size_t operator_device_bytes(const femhip_operator &op) {
              op.k_cell.bytes() + op.tab.bytes() + op.fixed.bytes() + op.vtx_dof.bytes() +
              op.row_vtx.bytes() + op.slot.bytes() + op.trig.bytes() + op.elmat.bytes() + op.elvec.bytes();
   const Sliced &s = op.sl;
-  b += s.soffs.bytes() + s.base.bytes() + s.meta.bytes() + s.srow.bytes() + s.aux.bytes() + s.offs.bytes() +
+  b += s.perm.bytes() + s.soffs.bytes() + s.base.bytes() + s.meta.bytes() + s.srow.bytes() + s.aux.bytes() + s.offs.bytes() +
        s.ecol.bytes() +
        s.map.bytes() + s.v64.bytes() + s.v32.bytes();
   if (op.space) b += op.space->d_dofs.bytes() + op.space->d_rep.bytes();

@@ -334,3 +334,35 @@ def test_q2_assembly_symmetric_and_repeatable(env):
     n = len(A1.irp) - 1
     M = sps.csr_matrix((A1.val, A1.ja - 1, A1.irp - 1), shape=(n, n))
     assert (M != M.T).nnz == 0
+
+
+@pytest.mark.parametrize("name", ["3d_q2_8_sin3", "2d_q2_12", "2d_q2_linear_g"])
+def test_q2_class_layout_solves(env, name, monkeypatch):
+    """Q2: the solver's class-blocked diagonal layout (rows renumbered class by
+    class, identity rows in the padding) gives the oracle's FP64 solution within
+    1e-10 in FP64 and within 1e-8 in FP32-refined, with the iteration count of
+    the first-touch ELL layout (FEMHIP_CLASS_DIA=0) within 2."""
+    print(f"[q2 layout] {name}", flush=True)
+    sols, its = {}, {}
+    for layout in ("1", "0"):
+        monkeypatch.setenv("FEMHIP_CLASS_DIA", layout)
+        osp, oph, fes, op, di = _build(env, name)
+        g = osp.fixed_values(oph)
+        op.set_fixed_dof_values(g if osp.n_fixed else None)
+        op.compute()
+        for prec in ("FP64", "FP32_REFINED"):
+            solver = F.iterative_linear_solver_t().create(env)
+            solver.set_type_from_string("CG")
+            solver.set_parameters_from_pl({"ILS_RTOL": 1e-10, "ILS_MAX_NUM_ITERATIONS": 20000,
+                                           "FEMHIP_PRECISION": prec})
+            solver.set_operators(op.get_tangent(), F.jacobi_preconditioner_t(op))
+            sols[layout, prec] = solver.solve(op.get_translation())
+            its[layout, prec] = solver.result["num_iterations"]
+            assert solver.result["did_converge"], (layout, prec)
+    monkeypatch.delenv("FEMHIP_CLASS_DIA")
+    irp, ja, val, rhs = osp.assemble(oph, True, fixed_values=g)
+    x_ref, info = O.cg(irp, ja, val, rhs, True, precond="jacobi", rtol=1e-12, maxit=20000)
+    for layout in ("1", "0"):
+        assert np.linalg.norm(sols[layout, "FP64"] - x_ref) <= SOL_RTOL * np.linalg.norm(x_ref), layout
+        assert np.linalg.norm(sols[layout, "FP32_REFINED"] - x_ref) <= 1e-8 * np.linalg.norm(x_ref), layout
+    assert abs(its["1", "FP64"] - its["0", "FP64"]) <= 2
