@@ -322,7 +322,7 @@ __device__ __forceinline__ A sym_rows(const SlicedDev &m, const T *__restrict__ 
   return acc;
 }
 
-template <typename T, typename A, typename G>
+template <typename T, typename A, typename G, int U = 9>
 __device__ __forceinline__ A slice_rows(const SlicedDev &m, const T *__restrict__ v, const G &g, int s, int lane) {
   if (m.sym) return sym_rows<T, A, G>(m, v, g, s, lane);
   const int64_t b = m.base[s];
@@ -343,7 +343,10 @@ __device__ __forceinline__ A slice_rows(const SlicedDev &m, const T *__restrict_
         }
         return acc;
       }
-#pragma unroll 9
+      // U loads of values and of x in flight: 25 in the CG SpMV (64^3 Q2 class
+      // slices of 125, 75, 45 and 27 offsets: 386 us against 416 us with 9; 32
+      // and more more than doubled the time); 9 in kernels held to 128 VGPRs
+#pragma unroll U
       for (int k = 0; k < K; ++k) {
         const A a = (A)vv[k * kSlice];
         acc += a * (A)g.off(off[k], rb);
@@ -368,10 +371,10 @@ __device__ __forceinline__ A slice_rows(const SlicedDev &m, const T *__restrict_
   return acc;
 }
 
-template <typename T, typename A>
+template <typename T, typename A, int U = 9>
 __device__ __forceinline__ A slice_row(const SlicedDev &m, const T *__restrict__ v, const T *__restrict__ x, int s,
                                        int lane) {
-  return slice_rows<T, A>(m, v, VecRef<T>{x}, s, lane);
+  return slice_rows<T, A, VecRef<T>, U>(m, v, VecRef<T>{x}, s, lane);
 }
 
 // ------------------------------------------------------------------ kernels ---
@@ -473,7 +476,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SlicedDev m, const T *__restr
   const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
   A pq = 0, pp = 0;
   for (int s = s0; s < s1; s += sstep) {
-    const A acc = slice_row<T, A>(m, v, p, s, lane);
+    const A acc = slice_row<T, A, 25>(m, v, p, s, lane);
     const int row = slice_row_index(m, s, lane);
     if (row < m.n) {
       q[row] = (T)acc;
