@@ -28,6 +28,7 @@ CASES = {
     "3d_q1_linear_g": (3, (6, 6, 6), 1, 1, "boundary", dict(k="const", f="zero", g="linear")),
     "2d_q2_linear_g": (2, (5, 5), 2, 1, "boundary", dict(k="const", f="zero", g="linear")),
     "elasticity_8": (3, (8, 8, 8), 1, 3, "x0", dict(elasticity=True, E=50.0, nu=0.3)),
+    "3d_q2_x0_7x5x6": (3, (7, 5, 6), 2, 1, "x0", dict(k="sin3", f="manufactured")),
 }
 
 
@@ -336,7 +337,7 @@ def test_q2_assembly_symmetric_and_repeatable(env):
     assert (M != M.T).nnz == 0
 
 
-@pytest.mark.parametrize("name", ["3d_q2_8_sin3", "2d_q2_12", "2d_q2_linear_g"])
+@pytest.mark.parametrize("name", ["3d_q2_8_sin3", "2d_q2_12", "2d_q2_linear_g", "3d_q2_x0_7x5x6"])
 def test_q2_class_layout_solves(env, name, monkeypatch):
     """Q2: the solver's class-blocked diagonal layout (rows renumbered class by
     class, identity rows in the padding) gives the oracle's FP64 solution within
