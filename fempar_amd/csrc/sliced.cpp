@@ -3,6 +3,7 @@
 // `map` after every assembly).
 #include <algorithm>
 #include <climits>
+#include <cstdint>
 #include <cstdlib>
 #include <thread>
 
@@ -69,6 +70,106 @@ bool try_build_symmetric_dia(femhip_operator &op) {
 void build_slices(femhip_operator &op, int32_t n, const std::vector<int32_t> &rp, const std::vector<int32_t> &cl,
                   const std::vector<int32_t> *pos);
 
+constexpr int kClassMaxOffsets = 128;
+
+// every row r draws its column offsets perm[c] - perm[r] from its class's set
+__global__ void k_class_check(int32_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                              const int32_t *__restrict__ perm, const int32_t *__restrict__ cls,
+                              const int32_t *__restrict__ offs, const int32_t *__restrict__ optr,
+                              int32_t *__restrict__ bad) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int c = cls[r];
+  const int32_t o0 = optr[c], o1 = optr[c + 1], pr = perm[r];
+  bool ok = true;
+  for (int32_t e = rowptr[r]; e < rowptr[r + 1] && ok; ++e) {
+    const int32_t o = perm[col[e]] - pr;
+    int32_t lo = o0, hi = o1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (offs[mid] < o) lo = mid + 1;
+      else hi = mid;
+    }
+    ok = lo < o1 && offs[lo] == o;
+  }
+  if (!ok) *bad = 1;
+}
+
+// per slice (64 solver rows, one class): which of the class's offsets its rows
+// use, as a bit mask over the class's sorted offsets (padding rows use 0)
+__global__ __launch_bounds__(kSlice) void k_class_mask(int32_t rows, const int32_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ col,
+                                                       const int32_t *__restrict__ perm,
+                                                       const int32_t *__restrict__ iperm,
+                                                       const int32_t *__restrict__ slice_cls,
+                                                       const int32_t *__restrict__ offs,
+                                                       const int32_t *__restrict__ optr, uint64_t *__restrict__ mask) {
+  __shared__ uint64_t sm[2][kSlice];
+  const int32_t sl = blockIdx.x, lane = threadIdx.x, i = sl * kSlice + lane;
+  const int c = slice_cls[sl];
+  const int32_t o0 = optr[c], o1 = optr[c + 1];
+  uint64_t m[2] = {0, 0};
+  auto set = [&](int32_t o) {
+    int32_t lo = o0, hi = o1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (offs[mid] < o) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < o1 && offs[lo] == o) m[(lo - o0) >> 6] |= 1ull << ((lo - o0) & 63);
+  };
+  const int32_t r = i < rows ? iperm[i] : -1;
+  if (r >= 0) {
+    for (int32_t e = rowptr[r]; e < rowptr[r + 1]; ++e) set(perm[col[e]] - i);
+  } else if (i < rows) {
+    set(0);
+  }
+  sm[0][lane] = m[0];
+  sm[1][lane] = m[1];
+  __syncthreads();
+  if (lane == 0) {
+    uint64_t a = 0, b = 0;
+    for (int t = 0; t < kSlice; ++t) {
+      a |= sm[0][t];
+      b |= sm[1][t];
+    }
+    mask[2 * (int64_t)sl] = a;
+    mask[2 * (int64_t)sl + 1] = b;
+  }
+}
+
+// map of the class-blocked slices: entry (slice, k, lane) of solver row i holds
+// the CSR position of (iperm[i], iperm[i + offset k]), -2 for the unit
+// diagonal of a padding row, -1 where there is no entry
+__global__ void k_class_map(int32_t rows, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                            const int32_t *__restrict__ iperm, const int64_t *__restrict__ base,
+                            const int32_t *__restrict__ meta, const int32_t *__restrict__ aux,
+                            const int32_t *__restrict__ offs, int32_t *__restrict__ map) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const int32_t sl = i / kSlice, lane = i % kSlice;
+  const int32_t K = meta[sl] & 0x0fffffff, a = aux[sl];
+  const int32_t r = iperm[i];
+  const int32_t b = r >= 0 ? rowptr[r] : 0, e = r >= 0 ? rowptr[r + 1] : 0;
+  for (int32_t k = 0; k < K; ++k) {
+    const int32_t o = offs[a + k], j = i + o;
+    int32_t m = -1;
+    if (r < 0) {
+      m = o == 0 ? -2 : -1;
+    } else if (j >= 0 && j < rows && iperm[j] >= 0) {
+      const int32_t c = iperm[j];
+      int32_t lo = b, hi = e;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (col[mid] < c) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < e && col[lo] == c) m = lo;
+    }
+    map[base[sl] + (int64_t)k * kSlice + lane] = m;
+  }
+}
+
 // Class-blocked diagonal layout of a structured scalar space of order p >= 2.
 // The free DOFs sit on the nodes of the fine grid (p n_k + 1 nodes along k);
 // a node's class is its coordinate modulo p in every direction (p^d classes:
@@ -80,7 +181,9 @@ void build_slices(femhip_operator &op, int32_t n, const std::vector<int32_t> &rp
 // one set of diagonals and needs no column indices (64^3 Q2: 1.11 GB a
 // product instead of 1.59 GB with the first-touch rows in ELL slices).
 // Box positions without a free DOF become identity rows.  The operator's
-// numbering, CSR and values are untouched: the solver permutes b and x.
+// numbering, CSR and values are untouched: the solver permutes b and x.  The
+// permutation is computed on the host from the rows' representatives; the
+// offsets check and the slices' value map are built on the device.
 bool try_build_class_dia(femhip_operator &op) {
   const char *env = getenv("FEMHIP_CLASS_DIA");   // read per operator: tests compare both layouts
   if (env && atoi(env) == 0) return false;
@@ -144,39 +247,128 @@ bool try_build_class_dia(femhip_operator &op) {
     FH_CHECK(iperm[(size_t)perm[(size_t)r]] < 0, FEMHIP_ERR_INVALID, "two free DOFs on one node");
     iperm[(size_t)perm[(size_t)r]] = r;
   }
-  // the permuted pattern (columns ascending) and each entry's CSR position
-  ensure_host_pattern(op);
-  const std::vector<int32_t> &rp = op.h_rowptr, &cl = op.h_col;
-  std::vector<int32_t> rp2((size_t)rows + 1, 0);
-  for (int64_t i = 0; i < rows; ++i) {
-    const int32_t r = iperm[(size_t)i];
-    rp2[(size_t)i + 1] = rp2[(size_t)i] + (r < 0 ? 1 : rp[(size_t)r + 1] - rp[(size_t)r]);
+  // each class's offsets, read off its longest row; every row of the class is
+  // checked on the device to draw its offsets from that set
+  std::vector<int32_t> rowlen((size_t)n + 1);
+  FH_HIP(hipMemcpy(rowlen.data(), op.rowptr.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost));
+  std::vector<int32_t> best((size_t)ncls, -1);
+  for (int32_t r = 0; r < n; ++r) {
+    int32_t &b = best[(size_t)cls[(size_t)r]];
+    if (b < 0 || rowlen[(size_t)r + 1] - rowlen[(size_t)r] > rowlen[(size_t)b + 1] - rowlen[(size_t)b]) b = r;
   }
-  std::vector<int32_t> cl2((size_t)rp2[(size_t)rows]), pos2((size_t)rp2[(size_t)rows]);
-  parallel_rows(rows, [&](unsigned, int64_t i0, int64_t i1) {
-    std::vector<std::pair<int32_t, int32_t>> buf;
-    for (int64_t i = i0; i < i1; ++i) {
-      const int32_t r = iperm[(size_t)i];
-      const int32_t o = rp2[(size_t)i];
-      if (r < 0) {
-        cl2[(size_t)o] = (int32_t)i;
-        pos2[(size_t)o] = -2;
-        continue;
-      }
-      buf.clear();
-      for (int32_t k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) buf.push_back({perm[(size_t)cl[(size_t)k]], k});
-      std::sort(buf.begin(), buf.end());
-      for (size_t t = 0; t < buf.size(); ++t) {
-        cl2[(size_t)o + t] = buf[t].first;
-        pos2[(size_t)o + t] = buf[t].second;
-      }
+  std::vector<int32_t> offs, off_ptr((size_t)ncls + 1, 0);
+  for (int c = 0; c < ncls; ++c) {
+    const int32_t r = best[(size_t)c];
+    if (r >= 0) {
+      const int32_t len = rowlen[(size_t)r + 1] - rowlen[(size_t)r];
+      if (len > kClassMaxOffsets) return false;
+      std::vector<int32_t> cols((size_t)len);
+      FH_HIP(hipMemcpy(cols.data(), op.col.p + rowlen[(size_t)r], sizeof(int32_t) * (size_t)len,
+                       hipMemcpyDeviceToHost));
+      std::vector<int32_t> o;
+      for (int32_t col : cols) o.push_back(perm[(size_t)col] - perm[(size_t)r]);
+      std::sort(o.begin(), o.end());
+      offs.insert(offs.end(), o.begin(), o.end());
     }
-  });
-  build_slices(op, (int32_t)rows, rp2, cl2, &pos2);
+    off_ptr[(size_t)c + 1] = (int32_t)offs.size();
+  }
+  rowlen.clear();
+  rowlen.shrink_to_fit();
   Sliced &S = op.sl;
-  S.permuted = true;
+  hipStream_t st = op.ctx->stream;
+  DevBuf<int32_t> d_cls, d_iperm, d_offs, d_optr, bad;
   S.perm.alloc((size_t)n);
+  d_cls.alloc((size_t)n);
+  d_iperm.alloc((size_t)rows);
+  d_offs.alloc(std::max<size_t>(1, offs.size()));
+  d_optr.alloc((size_t)ncls + 1);
+  bad.alloc(1);
   FH_HIP(hipMemcpy(S.perm.p, perm.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+  FH_HIP(hipMemcpy(d_cls.p, cls.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+  FH_HIP(hipMemcpy(d_iperm.p, iperm.data(), sizeof(int32_t) * (size_t)rows, hipMemcpyHostToDevice));
+  if (!offs.empty()) FH_HIP(hipMemcpy(d_offs.p, offs.data(), sizeof(int32_t) * offs.size(), hipMemcpyHostToDevice));
+  FH_HIP(hipMemcpy(d_optr.p, off_ptr.data(), sizeof(int32_t) * off_ptr.size(), hipMemcpyHostToDevice));
+  FH_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), st));
+  hipLaunchKernelGGL(k_class_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, op.rowptr.p, op.col.p,
+                     S.perm.p, d_cls.p, d_offs.p, d_optr.p, bad.p);
+  FH_HIP(hipGetLastError());
+  int32_t h_bad = 1;
+  FH_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  FH_HIP(hipStreamSynchronize(st));
+  if (h_bad) {
+    S.perm.release();
+    return false;
+  }
+  // slices: every slice lies in one class and keeps the class offsets its
+  // rows use (diagonal slices, no column indices: boundary slices take fewer
+  // offsets); padding rows keep their unit diagonal
+  const int32_t ns = (int32_t)(rows / kSlice);
+  std::vector<int32_t> scls((size_t)ns);
+  for (int32_t sl = 0, c = 0; sl < ns; ++sl) {
+    while (base[(size_t)c + 1] <= (int64_t)sl * kSlice) ++c;
+    scls[(size_t)sl] = c;
+  }
+  std::vector<uint64_t> hmask((size_t)ns * 2);
+  {
+    DevBuf<int32_t> d_scls;
+    DevBuf<uint64_t> d_mask;
+    d_scls.alloc((size_t)ns);
+    d_mask.alloc((size_t)ns * 2);
+    FH_HIP(hipMemcpy(d_scls.p, scls.data(), sizeof(int32_t) * (size_t)ns, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_class_mask, dim3((unsigned)ns), dim3(kSlice), 0, st, (int32_t)rows, op.rowptr.p, op.col.p,
+                       S.perm.p, d_iperm.p, d_scls.p, d_offs.p, d_optr.p, d_mask.p);
+    FH_HIP(hipGetLastError());
+    FH_HIP(hipMemcpyAsync(hmask.data(), d_mask.p, sizeof(uint64_t) * hmask.size(), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+  }
+  std::vector<int64_t> sbase((size_t)ns + 1, 0);
+  std::vector<int32_t> meta((size_t)ns), aux((size_t)ns), soffs;
+  for (int32_t sl = 0; sl < ns; ++sl) {
+    const int c = scls[(size_t)sl];
+    aux[(size_t)sl] = (int32_t)soffs.size();
+    for (int32_t t = off_ptr[(size_t)c]; t < off_ptr[(size_t)c + 1]; ++t) {
+      const int bit = t - off_ptr[(size_t)c];
+      if ((hmask[(size_t)sl * 2 + (bit >> 6)] >> (bit & 63)) & 1) soffs.push_back(offs[(size_t)t]);
+    }
+    const int32_t K = (int32_t)soffs.size() - aux[(size_t)sl];
+    sbase[(size_t)sl + 1] = sbase[(size_t)sl] + (int64_t)K * kSlice;
+    meta[(size_t)sl] = K;
+    if (K > 0) {   // bit 29: some lane's column row + offset leaves [0, rows)
+      const int64_t lo = (int64_t)sl * kSlice + soffs[(size_t)aux[(size_t)sl]];
+      const int64_t hi = (int64_t)sl * kSlice + kSlice - 1 + soffs.back();
+      if (lo < 0 || hi >= rows) meta[(size_t)sl] |= 1 << 29;
+    }
+  }
+  offs.swap(soffs);
+  const int64_t E = sbase[(size_t)ns];
+  if (E >= (int64_t)INT32_MAX) {
+    S.perm.release();
+    return false;
+  }
+  S.sym = false;
+  S.perm_slices = 0;
+  S.ns = ns;
+  S.entries = E;
+  S.dia_slices = ns;
+  S.base.alloc((size_t)ns + 1);
+  S.meta.alloc((size_t)std::max(1, ns));
+  S.aux.alloc((size_t)std::max(1, ns));
+  S.offs.alloc(std::max<size_t>(1, offs.size()));
+  S.ecol.alloc(1);
+  S.srow.alloc(1);
+  S.map.alloc((size_t)std::max<int64_t>(1, E));
+  FH_HIP(hipMemcpy(S.base.p, sbase.data(), sizeof(int64_t) * sbase.size(), hipMemcpyHostToDevice));
+  FH_HIP(hipMemcpy(S.meta.p, meta.data(), sizeof(int32_t) * meta.size(), hipMemcpyHostToDevice));
+  FH_HIP(hipMemcpy(S.aux.p, aux.data(), sizeof(int32_t) * aux.size(), hipMemcpyHostToDevice));
+  if (!offs.empty()) FH_HIP(hipMemcpy(S.offs.p, offs.data(), sizeof(int32_t) * offs.size(), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_class_map, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, (int32_t)rows,
+                     op.rowptr.p, op.col.p, d_iperm.p, S.base.p, S.meta.p, S.aux.p, S.offs.p, S.map.p);
+  FH_HIP(hipGetLastError());
+  FH_HIP(hipStreamSynchronize(st));
+  S.built = true;
+  S.rows = (int32_t)rows;
+  S.permuted = true;
+  S.filled_version = S.filled_version32 = -1;
   return true;
 }
 
