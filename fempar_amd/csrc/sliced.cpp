@@ -170,11 +170,12 @@ __global__ void k_class_map(int32_t rows, const int32_t *__restrict__ rowptr, co
   }
 }
 
-// Class-blocked diagonal layout of a structured scalar space of order p >= 2.
-// The free DOFs sit on the nodes of the fine grid (p n_k + 1 nodes along k);
-// a node's class is its coordinate modulo p in every direction (p^d classes:
-// vertices, edge, face and cell nodes for Q2) and its class coordinate the
-// coordinate divided by p.  The solver numbers the rows class by class, each
+// Class-blocked diagonal layout of a structured space of order p >= 2, or of a
+// vector space of any order.  The free DOFs sit on the nodes of the fine grid
+// (p n_k + 1 nodes along k); a DOF's class is its component and its node's
+// coordinate modulo p in every direction (p^d classes per component:
+// vertices, edge, face and cell nodes for Q2; one per component for vector
+// Q1) and its class coordinate the node coordinate divided by p.  The solver numbers the rows class by class, each
 // class lexicographically in a box whose x (and y) strides are the same for
 // all classes, so the column offsets from a row of class A to its neighbours
 // of class B do not depend on the row: every 64-row slice of one class has
@@ -190,11 +191,12 @@ bool try_build_class_dia(femhip_operator &op) {
   if (!op.space || op.structured || op.n <= 0) return false;
   femhip_space &sp = *op.space;
   const RefFE &fe = sp.fe;
-  if (fe.ncomp != 1 || fe.order < 2) return false;
+  if (fe.order < 2 && fe.ncomp == 1) return false;   // scalar Q1: the symmetric-diagonal layout
   const int dim = fe.dim, p = fe.order, nd = fe.nd;
   const int32_t n = op.n;
-  int ncls = 1;
-  for (int k = 0; k < dim; ++k) ncls *= p;
+  int npar = 1;   // node-parity classes per component
+  for (int k = 0; k < dim; ++k) npar *= p;
+  const int ncls = npar * fe.ncomp;
   std::vector<int64_t> rep((size_t)n);
   FH_HIP(hipMemcpy(rep.data(), sp.d_rep.p, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost));
   const int mn[3] = {sp.mesh.n[0], dim > 1 ? sp.mesh.n[1] : 1, dim > 2 ? sp.mesh.n[2] : 1};
@@ -202,9 +204,9 @@ bool try_build_class_dia(femhip_operator &op) {
   std::vector<int32_t> lo((size_t)ncls * 3, INT32_MAX), hi((size_t)ncls * 3, -1);
   for (int32_t r = 0; r < n; ++r) {
     const int64_t cell = rep[(size_t)r] / nd;
-    const int node = (int)(rep[(size_t)r] % nd) % fe.nnode;
+    const int l = (int)(rep[(size_t)r] % nd), node = l % fe.nnode;   // local DOF = component * nodes + node
     int64_t cr = cell;
-    int c = 0, w = 1, nr = node;
+    int c = (l / fe.nnode) * npar, w = 1, nr = node;
     for (int k = 0; k < dim; ++k) {
       const int ijk = (int)(cr % mn[k]);
       cr /= mn[k];

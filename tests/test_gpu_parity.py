@@ -337,10 +337,11 @@ def test_q2_assembly_symmetric_and_repeatable(env):
     assert (M != M.T).nnz == 0
 
 
-@pytest.mark.parametrize("name", ["3d_q2_8_sin3", "2d_q2_12", "2d_q2_linear_g", "3d_q2_x0_7x5x6"])
+@pytest.mark.parametrize("name", ["3d_q2_8_sin3", "2d_q2_12", "2d_q2_linear_g", "3d_q2_x0_7x5x6", "elasticity_8"])
 def test_q2_class_layout_solves(env, name, monkeypatch):
-    """Q2: the solver's class-blocked diagonal layout (rows renumbered class by
-    class, identity rows in the padding) gives the oracle's FP64 solution within
+    """Q2 and vector Q1: the solver's class-blocked diagonal layout (rows
+    renumbered class by class: node parity and component; identity rows in the
+    padding) gives the oracle's FP64 solution within
     1e-10 in FP64 and within 1e-8 in FP32-refined, with the iteration count of
     the first-touch ELL layout (FEMHIP_CLASS_DIA=0) within 2."""
     print(f"[q2 layout] {name}", flush=True)
