@@ -1,9 +1,11 @@
 /*
  * femhip.h -- C-ABI of the MI355X (gfx950) finite-element hot path.
  *
- * One GPU, one subdomain: structured hex mesh -> first-touch DOF numbering ->
- * cell-wise Galerkin integration -> deterministic coloured scatter-add assembly
- * into CSR -> (Jacobi-)preconditioned CG, in plain FP64 or FP32-refined form.
+ * One GPU, one subdomain: structured hex mesh -> first-touch DOF numbering and
+ * CSR pattern (on the device) -> cell-wise Galerkin integration -> deterministic
+ * owner-computes assembly into CSR (each row summed over its cells in ascending
+ * order and written once) -> (Jacobi-)preconditioned CG, in plain FP64 or
+ * FP32-refined form.
  *
  * Conventions follow FEMPAR's Fortran side so the ISO_C_BINDING shim passes its
  * arrays unchanged: DOF ids are 1-based, strong-Dirichlet DOFs carry negative
