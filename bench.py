@@ -104,6 +104,10 @@ def measure_config(F, env, n, order, di, rtol, steps, warmup, with_mixed=True, e
                      "first_step_s": first}
         if exact:
             out[prec]["error_vs_u"] = s.error_norms(exact)
+        # the CG SpMV of this configuration back to back (HIP events, 50 launches)
+        sp_ms, sp_bytes = s.profile_spmv(50)
+        out[prec]["spmv"] = {"kernel_ms": sp_ms, "algorithmic_bytes": sp_bytes, "GBs": sp_bytes / sp_ms / 1e6,
+                             "frac_of_hbm_peak": sp_bytes / sp_ms / 1e6 / HBM_PEAK_GBS}
     if with_mixed:
         out["speedup_mixed_vs_fp64"] = out["FP32_REFINED"]["value"] / out["FP64"]["value"]
         out["rel_err_mixed_vs_fp64"] = float(np.linalg.norm(xs["FP32_REFINED"] - xs["FP64"]) /
