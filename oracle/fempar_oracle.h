@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-/* ---- problem description (concrete drivers of the BASELINE-style configs) ---- */
+/* ---- problem description (the structured Poisson and elasticity drivers) ---- */
 
 enum { ORC_BC_ALL_BOUNDARY = 0,  /* set_id=1 on every boundary vef (test_poisson_driver.f90:193-204) */
        ORC_BC_X0_FACE      = 1,  /* set_id=1 on vefs lying in the plane x = x_min (clamped face)      */
