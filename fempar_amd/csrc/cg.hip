@@ -1213,13 +1213,14 @@ void prepare(femhip_solver &s) {
     // sweep is fastest when its front (resident waves x one slice of values)
     // is about 2 MB, well inside the XCD's 4 MB L2: measured at 128^3, 1 block
     // per CU for the FP64 values (SpMV 54 us against 67 us at 4) and 2 for the
-    // FP32 values; the sliced formats (no re-read) keep 4 and 2 (Q2 64^3).
+    // FP32 values; the sliced formats (no re-read) take 2 for both (Q2 64^3:
+    // FP64 SpMV 370 us at 2, 374 at 1, 380 at 3, 385 at 4, 414 at 8).
     // Vector kernels: 4 blocks per CU for FP64, 2 for FP32.  FEMHIP_CG_BPC
     // sets both, FEMHIP_CG_BPC_MAT the matrix kernels alone.
     const char *bpc_env = getenv("FEMHIP_CG_BPC");
     const char *mat_env = getenv("FEMHIP_CG_BPC_MAT");
     const int bpc = bpc_env ? std::max(1, atoi(bpc_env)) : (mx ? 2 : 4);
-    const int bpc_mat = mat_env ? std::max(1, atoi(mat_env)) : bpc_env ? bpc : (mx ? 2 : op.sl.sym ? 1 : 4);
+    const int bpc_mat = mat_env ? std::max(1, atoi(mat_env)) : bpc_env ? bpc : (mx || !op.sl.sym ? 2 : 1);
     const int cap = std::max(8, bpc * std::max(1, s.ctx->num_cus));
     const int cap_mat = std::max(8, bpc_mat * std::max(1, s.ctx->num_cus));
     s.g_mat = std::max(8, std::min(cap_mat, (ns + 2 * kWaves - 1) / (2 * kWaves)));
