@@ -549,7 +549,7 @@ __global__ __launch_bounds__(RB) void k_assemble_gather(
     const int32_t *__restrict__ vtx_dof, const uint8_t *__restrict__ slot, const int32_t *__restrict__ rowptr,
     const double *__restrict__ tab, const double *__restrict__ fixed, double *__restrict__ val,
     double *__restrict__ rhs, SymOut so) {
-  constexpr int ND = 1 << DIM, NQ = ND, K = DIM == 2 ? 9 : 27, KS = DIM == 2 ? 16 : 32;
+  constexpr int ND = 1 << DIM, NQ = ND, K = DIM == 2 ? 9 : 27;
   __shared__ double sv[RB * K];
   const double *M = tab;                 // M[NQ][ND][ND]
   const double *Nw = tab + NQ * ND * ND; // Nw[NQ][ND]
@@ -967,7 +967,7 @@ void elasticity_element(const RefFE &fe, const femhip_physics_t &ph, double *Ke,
 void upload_physics(femhip_operator &op) {
   const RefFE &fe = op.space->fe;
   const femhip_physics_t &ph = op.phys;
-  const int nd = fe.nd, nq = fe.nq, nn = fe.nnode, dim = fe.dim;
+  const int nd = fe.nd, nq = fe.nq, dim = fe.dim;
   std::vector<double> t;
   if (ph.kind == FEMHIP_PHYS_POISSON) {
     FH_CHECK(fe.ncomp == 1, FEMHIP_ERR_INVALID, "Poisson plugin needs a scalar FE space");

@@ -205,8 +205,7 @@ __device__ __forceinline__ void st_nt(T *p, T v) {
   __builtin_nontemporal_store(v, p);
 }
 
-// The next CG direction p' = beta p + z, rounded once (fused multiply-add).
-__device__ __forceinline__ double dir_value(double beta, double p, double z) { return __builtin_fma(beta, p, z); }
+// The next FP32 CG direction p' = beta p + z, rounded once (fused multiply-add).
 __device__ __forceinline__ float dir_value(float beta, float p, float z) { return __builtin_fmaf(beta, p, z); }
 
 // Right-hand vectors of the slice kernels.  operator()(c) is element c;

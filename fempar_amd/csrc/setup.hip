@@ -363,7 +363,7 @@ __global__ void k_longest_row(int32_t n, const int32_t *__restrict__ rowptr, int
   sm[threadIdx.x] = key;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sm[threadIdx.x] = max(sm[threadIdx.x], sm[threadIdx.x + o]);
+    if ((int)threadIdx.x < o) sm[threadIdx.x] = max(sm[threadIdx.x], sm[threadIdx.x + o]);
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = sm[0];

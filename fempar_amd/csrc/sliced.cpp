@@ -377,7 +377,6 @@ bool try_build_class_dia(femhip_operator &op) {
 }  // namespace
 
 void build_sliced(femhip_operator &op) {
-  Sliced &S = op.sl;
   // symmetric-diagonal storage wherever the pattern allows it (half the matrix
   // stream); FEMHIP_SYMDIA=0 keeps the full diagonals
   static const char *sym_env = getenv("FEMHIP_SYMDIA");
