@@ -37,6 +37,7 @@ EXPORTS = [
     "femhip_operator_apply", "femhip_operator_destroy",
     "femhip_solver_create", "femhip_solver_solve", "femhip_solver_get_solution", "femhip_solver_destroy",
     "femhip_solver_profile_spmv", "femhip_solver_profile_cg", "femhip_solver_error_norms",
+    "femhip_solver_get_convergence_history",
     "femhip_batch_create", "femhip_batch_assemble", "femhip_batch_solve", "femhip_batch_get_solution",
     "femhip_batch_get_problem_matrix", "femhip_batch_autotune", "femhip_batch_destroy",
     "femhip_serve_create", "femhip_serve_estimate_bytes", "femhip_serve_submit", "femhip_serve_wait",
@@ -94,7 +95,7 @@ EXACT_SIN, EXACT_LINEAR = 0, 1
 
 class ServeParams(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_double), ("max_active", C.c_int32), ("setup_threads", C.c_int32),
-                ("cg", CgParams)]
+                ("cg", CgParams), ("admission", C.c_int32), ("aging_s", C.c_double)]
 
 
 class ServeRequest(C.Structure):
@@ -147,6 +148,7 @@ def load():
         "femhip_solver_profile_spmv": [P, I32, C.POINTER(D), C.POINTER(D)],
         "femhip_solver_profile_cg": [P, C.POINTER(D), C.POINTER(I32), C.POINTER(D)],
         "femhip_solver_error_norms": [P, I32, C.POINTER(ErrorNorms)],
+        "femhip_solver_get_convergence_history": [P, I32, P, C.POINTER(I32)],
         "femhip_batch_create": [P, I32, C.POINTER(BatchProblem), C.POINTER(CgParams), pP],
         "femhip_batch_assemble": [P, C.POINTER(D)],
         "femhip_batch_solve": [P, C.POINTER(BatchResult), C.POINTER(D)],

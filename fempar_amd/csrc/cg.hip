@@ -1137,6 +1137,7 @@ struct femhip_solver {
   hipEvent_t ev[2] = {nullptr, nullptr};       // completion of the chunk writing each slot
   hipGraphExec_t graphs[2][2] = {};            // [chunk size][result slot]
   int graph_kind = -1;
+  bool stream_launches = false;  // never build chunk graphs (set by the request server)
   bool prepared = false;
   bool have_solution = false;
   ~femhip_solver() {
@@ -1377,7 +1378,7 @@ constexpr int32_t kGraphMaxRows = 1 << 20;
 bool use_graph(const femhip_solver &s) {
   static const bool no_graph = getenv("FEMHIP_NO_GRAPH") != nullptr;
   static const char *force = getenv("FEMHIP_GRAPH");
-  if (no_graph) return false;
+  if (no_graph || s.stream_launches) return false;
   if (force && atoi(force) == 1) return true;
   return s.n < kGraphMaxRows;
 }
@@ -1620,6 +1621,8 @@ double true_residual(femhip_solver &s) {
 
 int solver_chunk() { return kChunk; }
 
+void solver_stream_launches(femhip_solver &s) { s.stream_launches = true; }
+
 void solver_start(femhip_solver &s) {
   prepare(s);
   FH_CHECK(s.n > 0, FEMHIP_ERR_STATE, "operator without free DOFs");
@@ -1815,6 +1818,29 @@ int femhip_solver_get_solution(const femhip_solver_t *s, double *x) {
     FH_HIP(hipSetDevice(s->ctx->device));
     if (s->n > 0)
       FH_HIP(hipMemcpy(x, solution_operator_order(*s), sizeof(double) * (size_t)s->op->n, hipMemcpyDeviceToHost));
+    return FEMHIP_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
+int femhip_solver_get_convergence_history(const femhip_solver_t *s, int32_t count, double *hist,
+                                          int32_t *num_iterations) {
+  try {
+    FH_CHECK(s && num_iterations && count >= 0 && (hist || count == 0), FEMHIP_ERR_INVALID, "bad argument");
+    FH_CHECK(s->have_solution, FEMHIP_ERR_STATE, "no solve yet");
+    FH_CHECK(!mixed(*s), FEMHIP_ERR_STATE, "the convergence history is tracked by the FP64 solver");
+    FH_HIP(hipSetDevice(s->ctx->device));
+    int32_t its = 0;
+    if (s->n > 0) {
+      CgResult r;
+      FH_HIP(hipMemcpy(&r, s->res.p, sizeof(r), hipMemcpyDeviceToHost));
+      its = r.iter;
+    }
+    const int32_t m = std::min(count, its);
+    if (m > 0) FH_HIP(hipMemcpy(hist, s->hist.p, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost));
+    *num_iterations = its;
     return FEMHIP_OK;
   } catch (const Error &e) {
     set_error(e.msg);

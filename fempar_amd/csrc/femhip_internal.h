@@ -281,6 +281,9 @@ void elasticity_element(const RefFE &fe, const femhip_physics_t &ph, double *Ke,
 // on the device of ctx; row (c, node) = c * blk + node, node lexicographic
 // over the free vertices (x index 1..n fastest, then y, then z).
 femhip_operator *create_structured_elasticity(femhip_ctx *ctx, int32_t n, const femhip_physics_t &ph);
+// whether create_structured_elasticity accepts n (its sliced format indexes
+// 243 x padded vertices entries with 32-bit integers)
+bool structured_elasticity_fits(int32_t n);
 
 // error norms of the FE function with free values d_x (norms.hip):
 // out = {l2_norm, h1_seminorm, h1_norm, linfty_norm}
@@ -292,7 +295,10 @@ void error_norms(femhip_operator &op, const double *d_x, int exact, double out[4
 // iterations whose last kernel writes the result block into pinned slot 0 or 1;
 // solver_poll returns 0 while that chunk is running, else 2 once the device
 // state says done and 1 otherwise; solver_finish waits for the stream.
+// solver_stream_launches makes the solver enqueue its chunks as plain stream
+// launches instead of hipGraphs (set before solver_start).
 int solver_chunk();
+void solver_stream_launches(femhip_solver &s);
 void solver_start(femhip_solver &s);
 void solver_push(femhip_solver &s, int slot);
 int solver_poll(femhip_solver &s, int slot);

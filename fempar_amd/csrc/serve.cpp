@@ -1,15 +1,19 @@
 // serve.cpp -- request loop with continuous batching for elasticity solves.
 //
 // Threads:
-//  * the scheduler admits arrived requests (first come, first served) while a
-//    slot is free and the request's device-memory estimate fits in the budget,
+//  * the scheduler admits arrived requests (first come, first served, or the
+//    smallest estimate first with an aging bound) while a slot is free and the
+//    request's device-memory estimate fits in the budget,
 //    and interleaves the CG chunks of all running requests: whenever the result
 //    copy of a request's earlier chunk has landed it either retires the request
 //    (device state says done, or the iteration budget is queued) or queues the
-//    next chunk on the request's slot stream, keeping two chunks in flight;
+//    next chunk on the request's slot stream, keeping two chunks in flight
+//    (plain stream launches: the scheduler is the only thread that launches
+//    CG kernels, and no chunk graph exists for a worker to create or destroy
+//    while it launches);
 //  * setup workers turn an admitted request into device state on its slot's
 //    context (structured operator built and assembled on the device, Jacobi,
-//    initial residual, chunk graph) and retire finished requests (result,
+//    initial residual) and retire finished requests (result,
 //    true residual, solution to the host, device memory freed).
 // Each slot owns a femhip context with its own non-blocking stream, so a
 // request's kernels, reductions and launch shapes are those of a solve alone.
@@ -118,6 +122,7 @@ void setup_request(femhip_server &S, Req &r) {
   for (int k = 0; k < 3; ++k) ph.body[k] = r.rq.body[k];
   r.op = create_structured_elasticity(c, r.rq.n, ph);
   check_rc(femhip_solver_create(c, r.op, &S.prm.cg, &r.solver));
+  solver_stream_launches(*r.solver);
   solver_start(*r.solver);
   FH_HIP(hipStreamSynchronize(c->stream));
   r.res.hbm_bytes = (double)(operator_device_bytes(*r.op) + solver_device_bytes(*r.solver));
@@ -219,6 +224,22 @@ void worker_loop(femhip_server &S) {
   }
 }
 
+// The pending request to admit next among those arrived by t (end() if none):
+// the earliest arrival (FEMHIP_ADMIT_FCFS), or the smallest memory estimate,
+// ties by arrival (FEMHIP_ADMIT_SMALLEST; the estimate grows with the mesh,
+// so small, short solves pass large ones), except that a request waiting
+// aging_s or longer goes first so large requests are never starved.
+std::multimap<std::pair<double, int64_t>, ReqP>::iterator next_admission(femhip_server &S, double t) {
+  auto first = S.pending.begin();
+  if (first == S.pending.end() || first->second->arrival > t) return S.pending.end();
+  if (S.prm.admission != FEMHIP_ADMIT_SMALLEST) return first;
+  if (S.prm.aging_s > 0 && t - first->second->arrival >= S.prm.aging_s) return first;
+  auto best = first;
+  for (auto it = first; it != S.pending.end() && it->second->arrival <= t; ++it)
+    if (it->second->estimate < best->second->estimate) best = it;
+  return best;
+}
+
 void scheduler_loop(femhip_server &S) {
   (void)hipSetDevice(S.ctx->device);
   const int chunk = solver_chunk();
@@ -229,12 +250,13 @@ void scheduler_loop(femhip_server &S) {
     {
       std::unique_lock<std::mutex> lk(S.mu);
       if (S.stop && running.empty() && S.active == 0) return;
-      // admissions: arrived requests in order while a slot and memory are free
+      // admissions: arrived requests by the policy while a slot and memory are free
       const double t = S.now();
       while (!S.stop && !S.pending.empty()) {
-        auto it = S.pending.begin();
+        auto it = next_admission(S, t);
+        if (it == S.pending.end()) break;
         ReqP r = it->second;
-        if (r->arrival > t || S.free_slots.empty() || S.hbm_in_use + r->estimate > S.prm.hbm_budget_bytes) break;
+        if (S.free_slots.empty() || S.hbm_in_use + r->estimate > S.prm.hbm_budget_bytes) break;
         S.pending.erase(it);
         r->slot = S.free_slots.back();
         S.free_slots.pop_back();
@@ -354,6 +376,9 @@ int femhip_serve_create(femhip_ctx_t *ctx, const femhip_serve_params_t *params, 
     FH_CHECK(p.cg.preconditioner == FEMHIP_PREC_IDENTITY || p.cg.preconditioner == FEMHIP_PREC_JACOBI,
              FEMHIP_ERR_INVALID, "unknown preconditioner");
     FH_CHECK(p.cg.inner_rtol >= 0 && p.cg.inner_rtol < 1, FEMHIP_ERR_INVALID, "reliable-update factor in [0,1)");
+    FH_CHECK(p.admission == FEMHIP_ADMIT_FCFS || p.admission == FEMHIP_ADMIT_SMALLEST, FEMHIP_ERR_INVALID,
+             "unknown admission policy");
+    FH_CHECK(p.aging_s >= 0, FEMHIP_ERR_INVALID, "aging_s must be >= 0");
     FH_HIP(hipSetDevice(ctx->device));
     auto *S = new femhip_server();
     S->ctx = ctx_retain(ctx);
@@ -377,13 +402,23 @@ int femhip_serve_create(femhip_ctx_t *ctx, const femhip_serve_params_t *params, 
   } catch (const Error &e) {
     set_error(e.msg);
     return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return FEMHIP_ERR_OOM;
+  } catch (const std::exception &e) {   // e.g. std::system_error when a thread cannot start
+    set_error(std::string("server start failed: ") + e.what());
+    return FEMHIP_ERR_STATE;
+  } catch (...) {
+    set_error("server start failed");
+    return FEMHIP_ERR_STATE;
   }
 }
 
 int femhip_serve_submit(femhip_server_t *S, const femhip_serve_request_t *rq, int64_t *ticket) {
   try {
     FH_CHECK(S && rq && ticket, FEMHIP_ERR_INVALID, "null argument");
-    FH_CHECK(rq->n >= 1 && rq->n <= 256, FEMHIP_ERR_INVALID, "request needs 1 <= n <= 256");
+    FH_CHECK(structured_elasticity_fits(rq->n), FEMHIP_ERR_INVALID,
+             "request needs 1 <= n <= 206 (the operator's 32-bit entry index)");
     FH_CHECK(rq->E > 0 && rq->nu > -1.0 && rq->nu < 0.5, FEMHIP_ERR_INVALID, "invalid E or nu");
     const double est = estimate_bytes(rq->n, S->prm.cg.max_num_iterations);
     FH_CHECK(est <= S->prm.hbm_budget_bytes, FEMHIP_ERR_ADMISSION,

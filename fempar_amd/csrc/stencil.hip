@@ -85,15 +85,22 @@ __global__ __launch_bounds__(256) void k_stencil_assemble(StencilDev g, const do
 
 }  // namespace
 
+bool structured_elasticity_fits(int32_t n) {
+  if (n < 1 || n > 1024) return false;
+  const int64_t nodes = (int64_t)n * (n + 1) * (n + 1);
+  const int64_t blk = (nodes + 63) / 64 * 64;
+  return 3 * blk * kK < (int64_t)INT32_MAX;
+}
+
 femhip_operator *create_structured_elasticity(femhip_ctx *ctx, int32_t n, const femhip_physics_t &ph) {
-  FH_CHECK(n >= 1 && n <= 1024, FEMHIP_ERR_INVALID, "structured elasticity needs 1 <= n <= 1024");
+  FH_CHECK(structured_elasticity_fits(n), FEMHIP_ERR_INVALID,
+           "structured elasticity needs 1 <= n with 243 x padded vertices < 2^31 (n <= 206)");
   FH_CHECK(ph.kind == FEMHIP_PHYS_ELASTICITY && ph.E > 0 && ph.nu > -1.0 && ph.nu < 0.5, FEMHIP_ERR_INVALID,
            "invalid elasticity parameters");
   const int64_t nodes = (int64_t)n * (n + 1) * (n + 1);
   const int64_t blk = (nodes + 63) / 64 * 64;
   const int64_t rows = 3 * blk;
   const int64_t entries = rows * kK;
-  FH_CHECK(entries < (int64_t)INT32_MAX, FEMHIP_ERR_INVALID, "structured elasticity operator too large");
   auto *op = new femhip_operator();
   op->ctx = ctx_retain(ctx);
   try {

@@ -368,6 +368,16 @@ class iterative_linear_solver_t:
         N.check(self._lib.femhip_solver_get_solution(self.h, N.ptr(x)))
         return x
 
+    def get_error_estimate_history_convergence_test(self):
+        """error_estimate_history_convergence_test of the last FP64 solve: the
+        quantity the stopping criterion compared at each iteration
+        (base_iterative_linear_solver.f90:520-525, filled at cg.f90:388-389)."""
+        its = C.c_int32()
+        N.check(self._lib.femhip_solver_get_convergence_history(self.h, 0, None, C.byref(its)))
+        h = np.empty(max(its.value, 1))
+        N.check(self._lib.femhip_solver_get_convergence_history(self.h, its.value, N.ptr(h), C.byref(its)))
+        return h[: its.value]
+
     def error_norms(self, exact="sin"):
         """error_norms_scalar_t%compute of the last solution: l2_norm, h1_seminorm,
         h1_norm, linfty_norm against u = prod sin(pi x_d) ("sin") or x + y (+ z) ("linear")."""
@@ -485,12 +495,15 @@ class serve_t:
     a vector-Q1 linear elasticity solve on the unit cube with n cells per
     direction, clamped on x = 0, constant body force, FP32-refined Jacobi-PCG."""
 
+    ADMISSION = {"fcfs": 0, "smallest": 1}
+
     def __init__(self, environment: environment_t, hbm_budget_bytes=64 << 30, max_active=8, setup_threads=4,
-                 rtol=1e-8, max_num_iterations=20000, delta=0.0):
+                 rtol=1e-8, max_num_iterations=20000, delta=0.0, admission="fcfs", aging_s=0.0):
         self._lib = N.load()
         self.environment = environment
         p = N.ServeParams()
         p.hbm_budget_bytes, p.max_active, p.setup_threads = float(hbm_budget_bytes), max_active, setup_threads
+        p.admission, p.aging_s = self.ADMISSION[admission], float(aging_s)
         p.cg.rtol, p.cg.atol, p.cg.max_num_iterations = rtol, 0.0, max_num_iterations
         p.cg.stopping_criterion = N.STOPPING_CRITERIA.index("RES_RES")
         p.cg.preconditioner, p.cg.precision, p.cg.inner_rtol = N.PREC_JACOBI, N.PRECISION_FP32_REFINED, delta

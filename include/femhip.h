@@ -215,6 +215,14 @@ int femhip_solver_profile_cg(femhip_solver_t *solver, double *spmv_ms, int32_t *
 /* copy the last solution to the host (n values) */
 int femhip_solver_get_solution(const femhip_solver_t *solver, double *x);
 
+/* error_estimate_history_convergence_test of the last FP64 solve, the
+ * quantity the stopping criterion compared per iteration (cg.f90:388-389,
+ * base_iterative_linear_solver.f90:520-525): the first min(count, iterations)
+ * entries into hist (NULL allowed with count 0), the iteration count in
+ * *num_iterations.  FEMHIP_ERR_STATE for an FP32_REFINED solver. */
+int femhip_solver_get_convergence_history(const femhip_solver_t *solver, int32_t count, double *hist,
+                                          int32_t *num_iterations);
+
 /* Error of the last solution (fixed DOFs: the values given to
  * femhip_operator_compute, 0 if none) against an exact solution, scalar
  * spaces; replaces error_norms_scalar_t%compute
@@ -291,25 +299,32 @@ int femhip_batch_destroy(femhip_batch_t *batch);
  *
  * The server owns `max_active` slots, each with its own HIP stream.  A request
  * is admitted when its arrival time has come, a slot is free and its device
- * memory estimate fits in what is left of `hbm_budget_bytes` (first come,
- * first served).  Admitted requests are set up by worker threads (operator
- * build and assembly on the slot's stream) and then iterate on the device
- * in hipGraph chunks of CG iterations; a scheduler thread interleaves the
- * chunks of every running request, so requests that arrive while others are
- * on the GPU join as slots and memory free up, and leave as they converge.
+ * memory estimate fits in what is left of `hbm_budget_bytes`, in the order of
+ * the admission policy (first come, first served, or the smallest estimate
+ * first with an aging bound).  Admitted requests are set up by worker threads
+ * (operator build and assembly on the slot's stream) and then iterate on the
+ * device in chunks of CG iterations; a scheduler thread launches and
+ * interleaves the chunks of every running request, so requests that arrive
+ * while others are on the GPU join as slots and memory free up, and leave as
+ * they converge.
  * Mesh, matrix and Krylov state stay on the device until the request ends;
  * its device memory is then released and the solution kept on the host until
  * femhip_serve_release.  A request's arithmetic does not depend on the other
  * requests in flight: its result is the same alone and in a mix. */
+#define FEMHIP_ADMIT_FCFS         0   /* arrived requests in arrival order */
+#define FEMHIP_ADMIT_SMALLEST     1   /* smallest estimate first; a request waiting aging_s goes first */
+
 typedef struct {
   double hbm_budget_bytes;  /* device memory admitted requests may hold together */
   int32_t max_active;       /* slots (requests set up or iterating at once), >= 1 */
   int32_t setup_threads;    /* host threads that set up and retire requests, >= 1 */
   femhip_cg_params_t cg;    /* rtol, max_num_iterations, inner_rtol; FP32_REFINED, RES_RES/RES_RHS */
+  int32_t admission;        /* FEMHIP_ADMIT_* */
+  double aging_s;           /* FEMHIP_ADMIT_SMALLEST: wait after which arrival order wins (0: never) */
 } femhip_serve_params_t;
 
 typedef struct {
-  int32_t n;                /* cells per direction of the unit cube, 1..256 */
+  int32_t n;                /* cells per direction of the unit cube, 1..206 */
   double E;                 /* Young's modulus (> 0) */
   double nu;                /* Poisson ratio (-1, 0.5) */
   double body[3];           /* body force */

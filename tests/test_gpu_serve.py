@@ -124,7 +124,10 @@ def test_serve_admission_budget(env):
 def test_serve_errors(env):
     srv = F.serve_t(env, max_active=2, setup_threads=1)
     try:
-        for bad in [dict(n=0, E=1.0, nu=0.3), dict(n=4, E=-1.0, nu=0.3), dict(n=4, E=1.0, nu=0.5)]:
+        # n = 207 would overflow the operator's 32-bit entry index: refused at
+        # submit, before it takes a slot or budget
+        for bad in [dict(n=0, E=1.0, nu=0.3), dict(n=4, E=-1.0, nu=0.3), dict(n=4, E=1.0, nu=0.5),
+                    dict(n=207, E=1.0, nu=0.3)]:
             with pytest.raises(N.FemhipError) as ei:
                 srv.submit(**bad)
             assert ei.value.code == N.ERR_INVALID
@@ -137,5 +140,27 @@ def test_serve_errors(env):
         assert ei.value.code == N.ERR_STATE
         res = srv.wait(t, timeout_s=30)
         assert res["did_converge"] == 1 and res["queue_ms"] < 100.0
+    finally:
+        srv.free()
+
+
+@pytest.mark.parametrize("policy,aging_s", [("fcfs", 0.0), ("smallest", 0.0), ("smallest", 1e-9)])
+def test_serve_admission_order(env, policy, aging_s):
+    """One slot, a large request submitted before three small ones, all arriving
+    at the same instant: first come, first served admits the large one first;
+    smallest-first admits the small ones first; with an aging bound already
+    passed, arrival order wins again."""
+    srv = F.serve_t(env, max_active=1, setup_threads=1, rtol=RTOL, admission=policy, aging_s=aging_s)
+    try:
+        t0 = srv.now() + 0.3
+        big = srv.submit(16, 40.0, 0.3, arrival_s=t0)
+        small = [srv.submit(3, 10.0 + i, 0.3, arrival_s=t0) for i in range(3)]
+        rb = srv.wait(big, timeout_s=120)
+        rs = [srv.wait(t, timeout_s=120) for t in small]
+        assert rb["did_converge"] == 1 and all(r["did_converge"] == 1 for r in rs)
+        small_first = all(r["queue_ms"] < rb["queue_ms"] for r in rs)
+        print(f"[serve] {policy} aging={aging_s}: big queued {rb['queue_ms']:.1f} ms, "
+              f"small {[round(r['queue_ms'], 1) for r in rs]}", flush=True)
+        assert small_first == (policy == "smallest" and aging_s == 0.0)
     finally:
         srv.free()

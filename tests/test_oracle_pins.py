@@ -362,3 +362,143 @@ def test_error_norms_converge_at_the_theoretical_rates(order, l2_rate, h1_rate):
     for a, b in zip(errs, errs[1:]):
         assert abs(a["l2_norm"] / b["l2_norm"] / l2_rate - 1) < 0.02
         assert abs(a["h1_seminorm"] / b["h1_seminorm"] / h1_rate - 1) < 0.02
+
+
+# ------------------------------------------- variable-coefficient branches -----
+# The 3-D BASELINE configurations run the oracle's k(x_q) and manufactured-source
+# code (k = 1 + 0.5 sin(2 pi x) sin(2 pi y) sin(2 pi z), f = -div(k grad u)); the
+# pins below check those branches without the oracle's own formulas.
+
+def _lagrange_1d(p, x):
+    """Values and derivatives of the p+1 Lagrange polynomials through equispaced
+    nodes of [-1, 1] at the points x (shape (len(x), p+1))."""
+    t = np.linspace(-1.0, 1.0, p + 1)
+    v = np.ones((len(x), p + 1))
+    dv = np.zeros((len(x), p + 1))
+    for m in range(p + 1):
+        others = [t[j] for j in range(p + 1) if j != m]
+        den = np.prod([t[m] - o for o in others])
+        v[:, m] = np.prod([x - o for o in others], axis=0) / den if others else 1.0
+        for skip in range(len(others)):
+            dv[:, m] += np.prod([x - o for j, o in enumerate(others) if j != skip], axis=0) / den
+    return v, dv
+
+
+def _k_sin3(x):
+    return 1.0 + 0.5 * np.prod(np.sin(2 * np.pi * x), axis=-1)
+
+
+def _source_fd(x, eps=1e-3):
+    """f = -div(k grad u) for u = prod sin(pi x_d), with the divergence taken by a
+    fourth-order central difference of the flux k du/dx_k (not by the closed form)."""
+    d = x.shape[-1]
+
+    def flux(y, k):
+        g = np.pi * np.cos(np.pi * y[..., k])
+        for j in range(d):
+            if j != k:
+                g = g * np.sin(np.pi * y[..., j])
+        return _k_sin3(y) * g
+
+    f = np.zeros(x.shape[:-1])
+    for k in range(d):
+        e = np.zeros(d)
+        e[k] = eps
+        f -= (-flux(x + 2 * e, k) + 8 * flux(x + e, k) - 8 * flux(x - e, k) + flux(x - 2 * e, k)) / (12 * eps)
+    return f
+
+
+def _element_numpy(dim, order, n, lim, cell, k_fun, f_fun):
+    """Element matrix and vector of one box cell from numpy's Gauss-Legendre rule
+    and an explicit tensor Lagrange basis (nodes x fastest), k and f evaluated at
+    the mapped points x = x0 + (xi + 1) h / 2."""
+    xg, wg = np.polynomial.legendre.leggauss(order + 1)
+    v1, dv1 = _lagrange_1d(order, xg)
+    ijk = np.unravel_index(cell, n, order="F")
+    h = np.array([(lim[2 * k + 1] - lim[2 * k]) / n[k] for k in range(dim)])
+    x0 = np.array([lim[2 * k] + ijk[k] * h[k] for k in range(dim)])
+    p1 = order + 1
+    nd = p1 ** dim
+    em = np.zeros((nd, nd))
+    ev = np.zeros(nd)
+    for qi in np.ndindex(*([p1] * dim)):
+        xi = np.array([xg[qi[k]] for k in range(dim)])
+        w = np.prod([wg[qi[k]] for k in range(dim)]) * np.prod(h / 2)
+        xq = x0 + (xi + 1) * h / 2
+        phi = np.zeros(nd)
+        grad = np.zeros((nd, dim))
+        for a in range(nd):
+            ma = np.unravel_index(a, [p1] * dim, order="F")
+            vals = [v1[qi[k], ma[k]] for k in range(dim)]
+            phi[a] = np.prod(vals)
+            for k in range(dim):
+                g = dv1[qi[k], ma[k]] * 2 / h[k]
+                for j in range(dim):
+                    if j != k:
+                        g *= vals[j]
+                grad[a, k] = g
+        em += w * k_fun(xq) * grad @ grad.T
+        ev += w * f_fun(xq) * phi
+    return em, ev
+
+
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2)])
+def test_variable_coefficient_element_against_independent_quadrature(dim, order):
+    """K_SIN3 + F_MANUFACTURED element matrices and vectors on an anisotropic box
+    (unequal cell counts and lengths per axis) equal an independent numpy
+    integration: elmat to 1e-13, elvec to the finite-difference accuracy of the
+    source (a wrong axis, sign of grad k . grad u, factor or cell index fails)."""
+    n = (4, 3, 5)[:dim]
+    lim = [0.0, 2.0, -1.0, 0.5, 0.0, 3.0] if dim == 3 else [0.0, 2.0, -1.0, 0.5, 0.0, 0.0]
+    sp = O.Space(O.make_problem(dim, n, order, lim=lim))
+    ph = O.make_physics(k_kind=O.K_SIN3, f_kind=O.F_MANUFACTURED)
+    for cell in (0, 5, sp.num_cells - 2):
+        em, ev = sp.element(ph, cell)
+        em_ref, ev_ref = _element_numpy(dim, order, n, lim, cell, _k_sin3, _source_fd)
+        assert np.max(np.abs(em - em_ref)) <= 1e-13 * np.max(np.abs(em_ref)), cell
+        assert np.max(np.abs(ev - ev_ref)) <= 1e-9 * np.max(np.abs(ev_ref)), cell
+
+
+@pytest.mark.parametrize("order,n_pair,l2_rate,h1_rate", [(1, (16, 32), 4.0, 2.0), (2, (8, 16), 8.0, 4.0)])
+def test_3d_variable_coefficient_converges_at_the_theoretical_rates(order, n_pair, l2_rate, h1_rate):
+    """-div(k grad u) = f in 3-D with k = 1 + 0.5 sin sin sin and the manufactured
+    source of u = sin sin sin: halving h divides the L2 / H1-seminorm errors by
+    2^(p+1) / 2^p within 2 %.  Operator and source must agree on k and on the
+    grad k . grad u term: a dropped or sign-flipped term leaves a fixed error
+    (ratios near 1)."""
+    errs = []
+    for n in n_pair:
+        sp = O.Space(O.make_problem(3, (n, n, n), order))
+        ph = O.make_physics(k_kind=O.K_SIN3, f_kind=O.F_MANUFACTURED)
+        irp, ja, val, rhs = sp.assemble(ph, True)
+        x, info = O.cg(irp, ja, val, rhs, True, precond="jacobi", rtol=1e-13, maxit=100000)
+        assert info["converged"]
+        errs.append(sp.error_norms(x))
+    a, b = errs
+    assert abs(a["l2_norm"] / b["l2_norm"] / l2_rate - 1) < 0.02
+    assert abs(a["h1_seminorm"] / b["h1_seminorm"] / h1_rate - 1) < 0.02
+
+
+def test_cell_coefficient_reads_its_own_cell():
+    """K_CELL: each cell's element equals the K_CONST element with k0 = k_cell[c]
+    bit for bit, so the per-cell field is read at the cell's own index; the
+    assembled quadratic form v^T A v equals sum_c k_c v_c^T E v_c computed from
+    the DOF lists (no Dirichlet), and a constant field reproduces K_CONST."""
+    n = (5, 4, 3)
+    sp = O.Space(O.make_problem(3, n, 1, bc=O.BC_NONE))
+    kc = np.random.default_rng(7).uniform(0.1, 10.0, sp.num_cells)
+    ph = O.make_physics(k_kind=O.K_CELL, k_cell=kc)
+    for c in (0, 1, 7, sp.num_cells - 1):
+        em, _ = sp.element(ph, c)
+        em_c, _ = sp.element(O.make_physics(k_kind=O.K_CONST, k0=float(kc[c])), c)
+        assert np.array_equal(em, em_c), c
+    irp, ja, val, rhs = sp.assemble(ph, False)
+    v = np.random.default_rng(8).standard_normal(sp.n_free)
+    E1, _ = sp.element(O.make_physics(k_kind=O.K_CONST, k0=1.0), 0)   # every cell is a translate
+    d = sp.dofs() - 1
+    quad = sum(kc[c] * v[d[c]] @ E1 @ v[d[c]] for c in range(sp.num_cells))
+    assert abs(v @ O.spmv(irp, ja, val, v, False) - quad) <= 1e-12 * abs(quad)
+    const = O.make_physics(k_kind=O.K_CELL, k_cell=np.full(sp.num_cells, 2.5))
+    _, _, val_c, _ = sp.assemble(const, False)
+    _, _, val_k, _ = sp.assemble(O.make_physics(k_kind=O.K_CONST, k0=2.5), False)
+    assert np.array_equal(val_c, val_k)
