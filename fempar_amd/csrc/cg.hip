@@ -65,7 +65,7 @@ struct CgState {
   double rmax, delta;          // reliable updates
   double ru_rz, ru_rmax, ru_rnorm, ru_beta;
   double rscale, ru_rscale;    // ||S r|| / ||r|| at the last update (scaled inner residual)
-  int iter, done, converged, maxit, crit, ru, updates, ru_conv, ru_done, pad;
+  int iter, done, converged, maxit, crit, ru, updates, ru_conv, ru_done, first;
 };
 
 struct CgResult {
@@ -682,6 +682,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_init_fin(const double *part, in
   s.done = s.converged || (zero_diag && *zero_diag);
   s.ru = 0;
   s.updates = 0;
+  s.first = 1;
   S[1] = s;
   R->est = rnorm;
   R->rhs_test = s.rhs_test;
@@ -915,6 +916,189 @@ __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restric
   for (int k = 0; k < kPrefetch; ++k)
     if (i0 + k * stride < n) row(i0 + k * stride, pv[k], dv[k], rv[k]);
   for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) row(i, p[i], d[i], r[i]);
+}
+
+// ---- two kernels per iteration slot ----
+// The slot's direction step and product come from a recurrence: with
+// p' = r + beta p, the product q' = A p' = A r + beta q, so one kernel can take
+// the stopping/update decision, move x, p and d and form q' from an SpMV of r
+// (final after the previous slot), and a second one makes the r update:
+//   slot A: decision (reduces slot B's partials) ; d += alpha p ;
+//           p = r + beta p ; q = A r + beta q ; partials <p,q>, ||p||^2
+//           or, when a reliable update is due, x += S (d + alpha p) ; d = 0
+//   slot B: alpha = rz/<p,q> ; r -= alpha q ; partials <r,r>
+//           or, after a fold, r = S (b - A64 x) in FP64 ; partials <r,r>, ||b - A x||^2
+// The bytes per row are those of the three-kernel form (40 in FP32); the
+// third launch and its fixed cost go.  Both kernels run on the matrix grid
+// over the same slices, so rows are visited as in the SpMV.
+// State parity: slot A reads S[par] and writes S[par ^ 1]; slot B reads
+// S[par ^ 1], its block 0 stores alpha there for the next slot A.
+
+__global__ __launch_bounds__(kThreads, 4) void k_rf_a(SlicedDev m, SlicedDev m32, const float *__restrict__ v32,
+                                                   const float *__restrict__ r, float *__restrict__ p,
+                                                   float *__restrict__ q, float *__restrict__ d,
+                                                   double *__restrict__ x, const double *__restrict__ sc,
+                                                   CgState *S, int par, const double *part_b, int gb, double *part,
+                                                   CgResult *R, CgResult *Rh) {
+  const CgState &cur = S[par];
+  if (cur.done) {   // keep `done` sticky in both parity slots
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S[par ^ 1].done = 1;
+      if (Rh) *Rh = *R;
+    }
+    return;
+  }
+  double red[2] = {0.0, 0.0};
+  if (!cur.first) reduce_partials<2>(part_b, gb, red);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (lead) {
+    S[par ^ 1] = cur;
+    S[par ^ 1].first = 0;
+  }
+  int fold = 0, upd_d = 0, done = 0;
+  double beta = 0.0;
+  if (cur.first) {           // p = r, q = A r
+  } else if (!cur.ru) {      // an iteration's r update has been made
+    const int it = cur.iter + 1;
+    const double rnorm = sqrt(red[0]);   // of the scaled FP32 residual
+    // residual fell by delta since the last update, looks converged (threshold
+    // carried into the scaled norm by the last update's ratio), or budget spent
+    fold = rnorm < cur.delta * cur.rmax || rnorm <= cur.rhs_test * cur.rscale || it >= cur.maxit;
+    if (!fold) {
+      beta = 1.0 / cur.rz;
+      beta = beta * red[0];
+      upd_d = 1;
+    }
+    if (lead) {
+      CgState &w = S[par ^ 1];
+      w.iter = it;
+      w.rnorm = rnorm;
+      w.ru = fold;
+      if (!fold) {
+        w.beta = beta;
+        w.rz = red[0];
+        w.rmax = fmax(cur.rmax, rnorm);
+      }
+      R->iter = it;
+      R->prog = cur.rscale > 0.0 ? rnorm / cur.rscale : rnorm;   // estimate in the stopping test's norm
+      R->prog_it = it;
+    }
+  } else {                   // slot B replaced r by the FP64 residual of the folded x
+    const double rn64 = sqrt(red[1]);
+    const int conv = rn64 <= cur.rhs_test;
+    beta = 1.0 / cur.rz;
+    beta = beta * red[0];
+    done = conv || cur.iter >= cur.maxit;
+    if (lead) {
+      CgState &w = S[par ^ 1];
+      w.ru = 0;
+      w.rz = red[0];
+      w.beta = beta;
+      w.rmax = sqrt(red[0]);
+      w.rscale = rn64 > 0.0 ? sqrt(red[0]) / rn64 : cur.rscale;
+      w.updates = cur.updates + 1;
+      w.est = rn64;
+      w.rnorm = rn64;
+      w.converged = conv;
+      w.done = done;
+      R->iter = cur.iter;
+      R->est = rn64;
+      R->rnorm = rn64;
+      R->converged = conv;
+      R->done = done;
+      R->updates = cur.updates + 1;
+      R->prog = rn64;
+      R->prog_it = cur.iter;
+    }
+  }
+  if (lead && Rh) *Rh = *R;
+  if (done) return;
+  const int lane = threadIdx.x & 63;
+  const WaveRange wr = wave_slices(m.ns, m.order, m.pstride);
+  const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
+  const float a = (float)cur.alpha;
+  if (fold) {   // the next slot B recomputes the residual of the new x
+    for (int s = s0; s < s1; s += sstep) {
+      const int row = slice_row_index(m, s, lane);
+      if (row < m.n) {
+        x[row] = new_x(x[row], d[row] + a * p[row], sc, row);
+        d[row] = 0.0f;
+      }
+    }
+    return;
+  }
+  const float bt = (float)beta;
+  const bool first = cur.first;
+  float pq = 0.f, pp = 0.f;
+  for (int s = s0; s < s1; s += sstep) {
+    // FP32 values and vectors, the row sum held in FP64 registers
+    const float w = (float)slice_row<float, double>(m32, v32, r, s, lane);
+    const int row = slice_row_index(m, s, lane);
+    if (row < m.n) {
+      const float ri = r[row];
+      float pi = ri, qi = w;
+      if (!first) {
+        const float po = p[row];
+        if (upd_d) d[row] = d[row] + a * po;
+        pi = dir_value(bt, po, ri);
+        qi = __builtin_fmaf(bt, q[row], w);
+      }
+      p[row] = pi;
+      q[row] = qi;
+      pq += pi * qi;
+      pp += pi * pi;
+    }
+  }
+  double out[2] = {(double)pq, (double)pp};
+  write_partials<2>(out, part);
+}
+
+__global__ __launch_bounds__(kThreads, 4) void k_rf_b(SlicedDev m, const double *__restrict__ v64,
+                                                   const double *__restrict__ b, const double *__restrict__ x,
+                                                   float *__restrict__ r, const float *__restrict__ q,
+                                                   const double *__restrict__ sc, CgState *S, int par,
+                                                   const double *part_a, int ga, double *part) {
+  CgState &st = S[par ^ 1];
+  if (st.done) return;
+  const int lane = threadIdx.x & 63;
+  const WaveRange wr = wave_slices(m.ns, m.order, m.pstride);
+  const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
+  double acc[2] = {0, 0};
+  if (st.ru) {
+    for (int s = s0; s < s1; s += sstep) {
+      const double ax = slice_row<double, double>(m, v64, x, s, lane);
+      const int row = slice_row_index(m, s, lane);
+      if (row < m.n) {
+        const double ri = b[row] - ax;
+        const float rf = (float)(sc ? sc[row] * ri : ri);
+        r[row] = rf;
+        acc[0] += (double)(rf * rf);
+        acc[1] += ri * ri;
+      }
+    }
+    write_partials<2>(acc, part);
+    return;
+  }
+  double red[2];
+  reduce_partials<2>(part_a, ga, red);
+  const double alpha = (red[0] != 0.0) ? st.rz / red[0] : 0.0;   // cg.f90:195-199
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st.alpha = alpha;
+    st.pap = red[0];
+    st.pp = red[1];
+  }
+  const float a = (float)alpha;
+  float rr = 0.f;
+  for (int s = s0; s < s1; s += sstep) {
+    const int row = slice_row_index(m, s, lane);
+    if (row < m.n) {
+      const float ri = r[row] + (-a) * q[row];
+      r[row] = ri;
+      rr += ri * ri;
+    }
+  }
+  acc[0] = rr;
+  write_partials<2>(acc, part);
 }
 
 // ------------------------------------------------------------------ helpers ---
@@ -1229,7 +1413,7 @@ void prepare(femhip_solver &s) {
     s.x.alloc(n); s.b.alloc(n); s.r.alloc(n); s.p.alloc(n); s.q.alloc(n);
     s.part_i.alloc((size_t)4 * s.g_mat);
     s.part_s.alloc((size_t)3 * s.g_mat);
-    s.part_u.alloc((size_t)2 * s.g_vec);
+    s.part_u.alloc((size_t)2 * std::max(s.g_vec, s.g_mat));   // also the two-kernel slot B's partials
     s.part_r.alloc((size_t)3 * s.g_mat);
     s.hist.alloc((size_t)std::max(1, s.prm.max_num_iterations));
     s.st.alloc(2);
@@ -1331,6 +1515,13 @@ struct Launcher {
   }
 };
 
+// FP32-refined iteration slots as two kernels (k_rf_a, k_rf_b; default) or as
+// the three of the FP64 form (k_ru_spmv, k_ru_update, k_ru_dir; FEMHIP_RF3=1).
+bool three_kernel_refined() {
+  static const char *env = getenv("FEMHIP_RF3");
+  return env && atoi(env) == 1;
+}
+
 // Enqueue one iteration of parity `par`: SpMV, update, direction.  With
 // `ev`, events are recorded on the launcher's stream around the SpMV; with
 // `rh`, the direction kernel also writes the result block there.
@@ -1347,6 +1538,14 @@ void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = 
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
     L(k_update<double, double>, GV, B, n, s.r.p, s.q.p, invd, s.st.p, par, s.part_s.p, s.g_mat, s.part_u.p);
     L(k_dir<double>, GV, B, n, s.x.p, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.hist.p, s.res.p, rh);
+  } else if (!three_kernel_refined()) {
+    const double *sc = jac ? s.invd.p : nullptr;   // S = diag^(-1/2)
+    const SlicedDev m32 = fp32_view(s, m);
+    L(k_rf_a, GM, B, m, m32, op.sl.v32.p, s.r32.p, s.p32.p, s.q32.p, s.d32.p, s.x.p, sc, s.st.p, par, s.part_u.p,
+      s.g_mat, s.part_s.p, s.res.p, rh);
+    if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
+    L(k_rf_b, GM, B, m, op.sl.v64.p, s.b.p, s.x.p, s.r32.p, s.q32.p, sc, s.st.p, par, s.part_s.p, s.g_mat,
+      s.part_u.p);
   } else {
     const double *sc = jac ? s.invd.p : nullptr;   // S = diag^(-1/2)
     const SlicedDev m32 = fp32_view(s, m);
