@@ -63,6 +63,7 @@ struct orc_space {
   int64_t ncells;
   int32_t *dofs;           /* ncells x nd, signed 1-based */
   int nfree, nfixed;
+  int nq1_req;             /* quadrature points per direction requested by a multi-field space (0: order + 1) */
 };
 
 /* ------------------------------------------------------------------ mesh --- */
@@ -263,7 +264,7 @@ static void build_reffe(orc_space_t *s) {
   /* quadrature: degree 2*order -> order+1 points per direction
      (sbm_lagrangian_reference_fe.i90:1414-1424, sbm_hex_lagrangian_reference_fe.i90:320-328),
      first index fastest (sbm_quadrature.i90:797-815) */
-  s->nq1 = s->order + 1;
+  s->nq1 = s->nq1_req > s->order ? s->nq1_req : s->order + 1;
   double x1[8], w1[8];
   orc_gauss_legendre(s->nq1, x1, w1);
   s->nq = 1;
@@ -438,7 +439,11 @@ static void number_dofs(orc_space_t *s) {
   free(vis_lid);
 }
 
-orc_space_t *orc_space_create(const orc_problem_t *prob) {
+static orc_space_t *space_create(const orc_problem_t *prob, int nq1_req);
+
+orc_space_t *orc_space_create(const orc_problem_t *prob) { return space_create(prob, 0); }
+
+static orc_space_t *space_create(const orc_problem_t *prob, int nq1_req) {
   if (prob->dim != 2 && prob->dim != 3) return NULL;
   if (prob->order < 1 || prob->order > 2) return NULL;
   if (prob->ncomp != 1 && prob->ncomp != prob->dim) return NULL;
@@ -448,6 +453,7 @@ orc_space_t *orc_space_create(const orc_problem_t *prob) {
   if (s->d == 2) { s->p.n[2] = 1; s->p.lim[4] = 0.0; s->p.lim[5] = 0.0; }
   s->order = prob->order;
   s->ncomp = prob->ncomp;
+  s->nq1_req = nq1_req;
   s->ncells = 1;
   for (int k = 0; k < s->d; ++k) s->ncells *= prob->n[k];
   build_polytope(s);
@@ -650,6 +656,71 @@ static void coo_append(coo_t *m, int32_t i, int32_t j, double v) {
   m->ia[m->nnz] = i; m->ja[m->nnz] = j; m->val[m->nnz] = v; ++m->nnz;
 }
 
+/* sort_and_compress_numeric by rows (base_sparse_matrix.f90:4482-4827):
+   stable counting sort on rows, stable column sort inside each row
+   (mergesort_link_list, sparse_matrix_utils.f90:136-250), duplicates summed in
+   insertion order; then move_from_coo builds irp (csr_sparse_matrix.f90:416-479).
+   Rows 1..n; the COO arrays are consumed.  Returns nnz. */
+static int64_t coo_compress(coo_t *m, int n, int32_t **irp_out, int32_t **ja_out, double **val_out) {
+  int64_t nnz = m->nnz;
+  int64_t *cnt = (int64_t *)xcalloc((size_t)n + 2, sizeof(int64_t));
+  for (int64_t k = 0; k < nnz; ++k) cnt[m->ia[k]]++;
+  int64_t acc = 0;
+  for (int r = 1; r <= n; ++r) { int64_t t = cnt[r]; cnt[r] = acc; acc += t; }
+  int32_t *ja2 = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+  double *v2 = (double *)xmalloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+  int64_t *rs = (int64_t *)xmalloc(sizeof(int64_t) * ((size_t)n + 2));
+  for (int r = 1; r <= n; ++r) rs[r] = cnt[r];
+  rs[n + 1] = acc;
+  for (int64_t k = 0; k < nnz; ++k) {
+    int64_t p = cnt[m->ia[k]]++;
+    ja2[p] = m->ja[k]; v2[p] = m->val[k];
+  }
+  free(m->ia); free(m->ja); free(m->val); free(cnt);
+  m->ia = NULL; m->ja = NULL; m->val = NULL; m->nnz = m->cap = 0;
+
+  int32_t *irp = (int32_t *)xmalloc(sizeof(int32_t) * ((size_t)n + 1));
+  int32_t *jo = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+  double *vo = (double *)xmalloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+  /* per-row stable merge sort buffers */
+  int32_t *tj = NULL; double *tv = NULL; int64_t tcap = 0;
+  int64_t k = 0;
+  irp[0] = 1;
+  for (int r = 1; r <= n; ++r) {
+    int64_t b = rs[r], e = (r < n) ? rs[r + 1] : acc, len = e - b;
+    if (len > tcap) {
+      tcap = len;
+      tj = (int32_t *)realloc(tj, sizeof(int32_t) * (size_t)tcap);
+      tv = (double *)realloc(tv, sizeof(double) * (size_t)tcap);
+    }
+    /* bottom-up stable merge sort of (ja2, v2)[b:e) by column */
+    int32_t *aj = ja2 + b; double *av = v2 + b;
+    for (int64_t w = 1; w < len; w *= 2) {
+      for (int64_t lo = 0; lo < len; lo += 2 * w) {
+        int64_t mid = lo + w < len ? lo + w : len, hi = lo + 2 * w < len ? lo + 2 * w : len;
+        int64_t i = lo, j = mid, o = lo;
+        while (i < mid && j < hi) {
+          if (aj[j] < aj[i]) { tj[o] = aj[j]; tv[o] = av[j]; ++j; }
+          else { tj[o] = aj[i]; tv[o] = av[i]; ++i; }
+          ++o;
+        }
+        while (i < mid) { tj[o] = aj[i]; tv[o] = av[i]; ++i; ++o; }
+        while (j < hi) { tj[o] = aj[j]; tv[o] = av[j]; ++j; ++o; }
+      }
+      memcpy(aj, tj, sizeof(int32_t) * (size_t)len);
+      memcpy(av, tv, sizeof(double) * (size_t)len);
+    }
+    for (int64_t i = 0; i < len; ++i) {
+      if (i > 0 && aj[i] == aj[i - 1]) vo[k - 1] = vo[k - 1] + av[i];
+      else { jo[k] = aj[i]; vo[k] = av[i]; ++k; }
+    }
+    irp[r] = (int32_t)(k + 1);
+  }
+  free(tj); free(tv); free(ja2); free(v2); free(rs);
+  *irp_out = irp; *ja_out = jo; *val_out = vo;
+  return k;
+}
+
 int64_t orc_assemble(const orc_space_t *s, const orc_physics_t *ph, int sym,
                      const double *fixed, int32_t **irp_out, int32_t **ja_out, double **val_out,
                      double **rhs_out) {
@@ -702,65 +773,8 @@ int64_t orc_assemble_cells(const orc_space_t *s, const orc_physics_t *ph, int sy
   free(elmat);
   free(elvec);
 
-  /* sort_and_compress_numeric by rows (base_sparse_matrix.f90:4482-4827):
-     stable counting sort on rows, stable column sort inside each row
-     (mergesort_link_list, sparse_matrix_utils.f90:136-250), duplicates summed in
-     insertion order; then move_from_coo builds irp (csr_sparse_matrix.f90:416-479). */
-  int64_t nnz = m.nnz;
-  int64_t *cnt = (int64_t *)xcalloc((size_t)n + 2, sizeof(int64_t));
-  for (int64_t k = 0; k < nnz; ++k) cnt[m.ia[k]]++;
-  int64_t acc = 0;
-  for (int r = 1; r <= n; ++r) { int64_t t = cnt[r]; cnt[r] = acc; acc += t; }
-  int32_t *ja2 = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
-  double *v2 = (double *)xmalloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
-  int64_t *rs = (int64_t *)xmalloc(sizeof(int64_t) * ((size_t)n + 2));
-  for (int r = 1; r <= n; ++r) rs[r] = cnt[r];
-  rs[n + 1] = acc;
-  for (int64_t k = 0; k < nnz; ++k) {
-    int64_t p = cnt[m.ia[k]]++;
-    ja2[p] = m.ja[k]; v2[p] = m.val[k];
-  }
-  free(m.ia); free(m.ja); free(m.val); free(cnt);
-
-  int32_t *irp = (int32_t *)xmalloc(sizeof(int32_t) * ((size_t)n + 1));
-  int32_t *jo = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
-  double *vo = (double *)xmalloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
-  /* per-row stable merge sort buffers */
-  int32_t *tj = NULL; double *tv = NULL; int64_t tcap = 0;
-  int64_t k = 0;
-  irp[0] = 1;
-  for (int r = 1; r <= n; ++r) {
-    int64_t b = rs[r], e = (r < n) ? rs[r + 1] : acc, len = e - b;
-    if (len > tcap) {
-      tcap = len;
-      tj = (int32_t *)realloc(tj, sizeof(int32_t) * (size_t)tcap);
-      tv = (double *)realloc(tv, sizeof(double) * (size_t)tcap);
-    }
-    /* bottom-up stable merge sort of (ja2, v2)[b:e) by column */
-    int32_t *aj = ja2 + b; double *av = v2 + b;
-    for (int64_t w = 1; w < len; w *= 2) {
-      for (int64_t lo = 0; lo < len; lo += 2 * w) {
-        int64_t mid = lo + w < len ? lo + w : len, hi = lo + 2 * w < len ? lo + 2 * w : len;
-        int64_t i = lo, j = mid, o = lo;
-        while (i < mid && j < hi) {
-          if (aj[j] < aj[i]) { tj[o] = aj[j]; tv[o] = av[j]; ++j; }
-          else { tj[o] = aj[i]; tv[o] = av[i]; ++i; }
-          ++o;
-        }
-        while (i < mid) { tj[o] = aj[i]; tv[o] = av[i]; ++i; ++o; }
-        while (j < hi) { tj[o] = aj[j]; tv[o] = av[j]; ++j; ++o; }
-      }
-      memcpy(aj, tj, sizeof(int32_t) * (size_t)len);
-      memcpy(av, tv, sizeof(double) * (size_t)len);
-    }
-    for (int64_t i = 0; i < len; ++i) {
-      if (i > 0 && aj[i] == aj[i - 1]) vo[k - 1] = vo[k - 1] + av[i];
-      else { jo[k] = aj[i]; vo[k] = av[i]; ++k; }
-    }
-    irp[r] = (int32_t)(k + 1);
-  }
-  free(tj); free(tv); free(ja2); free(v2); free(rs);
-  *irp_out = irp; *ja_out = jo; *val_out = vo; *rhs_out = rhs;
+  int64_t k = coo_compress(&m, n, irp_out, ja_out, val_out);
+  *rhs_out = rhs;
   return k;
 }
 
@@ -957,4 +971,194 @@ void orc_error_norms(const orc_space_t *s, const double *x_free, const double *f
   out[1] = sqrt(gradients_norm);
   out[2] = sqrt(values_norm + gradients_norm);
   out[3] = linf;
+}
+
+/* ------------------------------------------------- multi-field spaces --- */
+
+struct orc_mf {
+  int nf, nb, ndt;
+  orc_space_t *f[ORC_MF_MAXF];
+  int block[ORC_MF_MAXF];      /* 1-based block of each field (block_layout_t%field_id_to_block_id) */
+  int eoff[ORC_MF_MAXF + 1];   /* first row of each field in the cell's element matrix */
+  int ndofs_block[ORC_MF_MAXF + 1];
+  int32_t *dofs[ORC_MF_MAXF];  /* cells x nd_f, block-local free ids and global fixed ids */
+};
+
+/* serial_fe_space_t with several fields on one triangulation
+   (sbm_serial_fe_space.i90:40-140 create, block_layout_t%create
+   Integration/FEOperator/block_layout.f90:66-117).  Strong Dirichlet ids: the
+   sweep runs field by field over all cells with one counter
+   (sbm_serial_fe_space.i90:584-640), so field f's fixed ids follow those of
+   fields 1..f-1.  Free ids: each field has its own counter, started at the
+   number of free DOFs of the earlier fields of its block
+   (sbm_serial_fe_space.i90:2676-2682), and numbers first touch by cells as
+   one field alone (:2684-2700).  Every field is integrated with the
+   quadrature of the highest order in the cell (:1891-2006). */
+orc_mf_t *orc_mf_create(int nfields, const orc_problem_t *fields, const int *field_blocks) {
+  if (nfields < 1 || nfields > ORC_MF_MAXF) return NULL;
+  int maxo = 0, ndt = 0;
+  for (int i = 0; i < nfields; ++i) {
+    const orc_problem_t *a = &fields[i], *b = &fields[0];
+    if (a->dim != b->dim || a->ncomp != 1 || field_blocks[i] < 1 || field_blocks[i] > nfields) return NULL;
+    for (int k = 0; k < 3; ++k) if (k < a->dim && a->n[k] != b->n[k]) return NULL;
+    for (int k = 0; k < 2 * a->dim; ++k) if (a->lim[k] != b->lim[k]) return NULL;
+    if (a->order > maxo) maxo = a->order;
+  }
+  orc_mf_t *m = (orc_mf_t *)xcalloc(1, sizeof(orc_mf_t));
+  m->nf = nfields;
+  for (int i = 0; i < nfields; ++i) {
+    m->f[i] = space_create(&fields[i], maxo + 1);
+    if (!m->f[i]) { orc_mf_free(m); return NULL; }
+    m->block[i] = field_blocks[i];
+    if (field_blocks[i] > m->nb) m->nb = field_blocks[i];
+    m->eoff[i] = ndt;
+    ndt += m->f[i]->nd;
+  }
+  m->eoff[nfields] = ndt;
+  m->ndt = ndt;
+  int free_off[ORC_MF_MAXF] = {0}, fixed_off = 0;
+  for (int i = 0; i < nfields; ++i) {   /* offsets in field order */
+    free_off[i] = m->ndofs_block[m->block[i]];
+    m->ndofs_block[m->block[i]] += m->f[i]->nfree;
+  }
+  for (int i = 0; i < nfields; ++i) {
+    const orc_space_t *s = m->f[i];
+    size_t cnt = (size_t)(s->ncells * s->nd);
+    m->dofs[i] = (int32_t *)xmalloc(sizeof(int32_t) * (cnt ? cnt : 1));
+    for (size_t k = 0; k < cnt; ++k) {
+      const int32_t g = s->dofs[k];
+      m->dofs[i][k] = g > 0 ? g + free_off[i] : g - fixed_off;
+    }
+    fixed_off += s->nfixed;
+  }
+  return m;
+}
+
+void orc_mf_free(orc_mf_t *m) {
+  if (!m) return;
+  for (int i = 0; i < m->nf; ++i) { orc_space_free(m->f[i]); free(m->dofs[i]); }
+  free(m);
+}
+
+int orc_mf_num_blocks(const orc_mf_t *m) { return m->nb; }
+
+void orc_mf_field_sizes(const orc_mf_t *m, int field, int *nd, int *n_free, int *n_fixed) {
+  const orc_space_t *s = m->f[field];
+  *nd = s->nd; *n_free = s->nfree; *n_fixed = s->nfixed;
+}
+
+int orc_mf_block_size(const orc_mf_t *m, int block) { return m->ndofs_block[block]; }
+
+void orc_mf_get_dofs(const orc_mf_t *m, int field, int32_t *out) {
+  const orc_space_t *s = m->f[field];
+  memcpy(out, m->dofs[field], sizeof(int32_t) * (size_t)(s->ncells * s->nd));
+}
+
+/* Element of the coupled diffusion-reaction plugin on field-major element
+   rows (field 1's DOFs, then field 2's, ...), the q loop outermost and then
+   i and j as the Poisson plugin (poisson_discrete_integration.f90:116-129).
+   The bilinear form sum_f kappa_f (grad u_f, grad v_f) + sigma sum_{f<g}
+   (u_f - u_g, v_f - v_g) (symmetric positive definite with Dirichlet data on
+   some field) gives
+     (f, f): kappa_f grad phi_i . grad phi_j + (nf - 1) sigma phi_i phi_j
+     (f, g): -sigma phi_i psi_j           (f != g)
+     elvec_f: f_f phi_i
+   elmat column-major (ndt x ndt). */
+void orc_mf_element(const orc_mf_t *m, const orc_mf_physics_t *ph, int64_t cell, double *elmat,
+                    double *elvec) {
+  (void)cell;   /* constant coefficients on a uniform mesh */
+  const int ndt = m->ndt;
+  const orc_space_t *s0 = m->f[0];
+  for (int i = 0; i < ndt * ndt; ++i) elmat[i] = 0.0;
+  for (int i = 0; i < ndt; ++i) elvec[i] = 0.0;
+  for (int q = 0; q < s0->nq; ++q) {
+    const double dv = s0->detj[q] * s0->qw[q];
+    for (int fa = 0; fa < m->nf; ++fa) {
+      const orc_space_t *A = m->f[fa];
+      for (int i = 0; i < A->nd; ++i) {
+        for (int fb = 0; fb < m->nf; ++fb) {
+          const orc_space_t *B = m->f[fb];
+          for (int j = 0; j < B->nd; ++j) {
+            double v;
+            if (fa == fb) {
+              double dot = 0.0;
+              for (int k = 0; k < 3; ++k) dot = dot + A->grad[j][k][q] * A->grad[i][k][q];
+              v = ph->kappa[fa] * dot + (m->nf - 1) * ph->sigma * (A->shp[i][q] * A->shp[j][q]);
+            } else {
+              v = -ph->sigma * (A->shp[i][q] * B->shp[j][q]);
+            }
+            elmat[(m->eoff[fa] + i) + (size_t)(m->eoff[fb] + j) * ndt] += dv * v;
+          }
+        }
+        elvec[m->eoff[fa] + i] += dv * ph->f[fa] * A->shp[i][q];
+      }
+    }
+  }
+}
+
+/* Block (ib, jb) of the block_sparse_matrix_t (1-based blocks): the cells in
+   order, and per cell element_block_sparse_matrix_assembly
+   (Integration/FEOperator/block_sparse_assembler.f90:182-222): row fields in
+   order, column fields in order, each coupled pair's dense sub-block
+   inserted (j outer, i inner; columns or rows outside the block's range and,
+   with symmetric storage of a diagonal block, row > col skipped:
+   base_sparse_matrix.f90:3469-3513), then sort and compress.  field_coupling
+   is nf x nf, [f * nf + g] nonzero when field f's rows take field g's columns
+   (block_layout_t field_coupling). Homogeneous Dirichlet: no lifts. */
+int64_t orc_mf_assemble_block(const orc_mf_t *m, const orc_mf_physics_t *ph, const int *field_coupling, int ib,
+                              int jb, int symmetric_storage, int32_t **irp, int32_t **ja, double **val) {
+  const int ndt = m->ndt, nr = m->ndofs_block[ib], nc = m->ndofs_block[jb];
+  const int sym = symmetric_storage && ib == jb;
+  double *elmat = (double *)xmalloc(sizeof(double) * (size_t)(ndt * ndt));
+  double *elvec = (double *)xmalloc(sizeof(double) * (size_t)ndt);
+  coo_t c = {0};
+  const int64_t ncells = m->f[0]->ncells;
+  for (int64_t cell = 0; cell < ncells; ++cell) {
+    orc_mf_element(m, ph, cell, elmat, elvec);
+    for (int fa = 0; fa < m->nf; ++fa) {
+      if (m->block[fa] != ib) continue;
+      const int nda = m->f[fa]->nd;
+      const int32_t *ga = m->dofs[fa] + cell * nda;
+      for (int fb = 0; fb < m->nf; ++fb) {
+        if (m->block[fb] != jb || !field_coupling[fa * m->nf + fb]) continue;
+        const int ndb = m->f[fb]->nd;
+        const int32_t *gb = m->dofs[fb] + cell * ndb;
+        for (int j = 0; j < ndb; ++j) {
+          const int32_t jas = gb[j];
+          if (jas < 1 || jas > nc) continue;
+          for (int i = 0; i < nda; ++i) {
+            const int32_t ias = ga[i];
+            if (ias < 1 || ias > nr || (sym && ias > jas)) continue;
+            coo_append(&c, ias, jas, elmat[(m->eoff[fa] + i) + (size_t)(m->eoff[fb] + j) * ndt]);
+          }
+        }
+      }
+    }
+  }
+  free(elmat);
+  free(elvec);
+  return coo_compress(&c, nr, irp, ja, val);
+}
+
+/* Block ib of the serial_block_array_t right-hand side
+   (element_serial_block_array_assembly, block_sparse_assembler.f90:157-180):
+   cells in order, fields of the block in order. */
+void orc_mf_assemble_rhs(const orc_mf_t *m, const orc_mf_physics_t *ph, int ib, double *rhs) {
+  const int ndt = m->ndt, nr = m->ndofs_block[ib];
+  double *elmat = (double *)xmalloc(sizeof(double) * (size_t)(ndt * ndt));
+  double *elvec = (double *)xmalloc(sizeof(double) * (size_t)ndt);
+  for (int i = 0; i < nr; ++i) rhs[i] = 0.0;
+  const int64_t ncells = m->f[0]->ncells;
+  for (int64_t cell = 0; cell < ncells; ++cell) {
+    orc_mf_element(m, ph, cell, elmat, elvec);
+    for (int fa = 0; fa < m->nf; ++fa) {
+      if (m->block[fa] != ib) continue;
+      const int nda = m->f[fa]->nd;
+      const int32_t *ga = m->dofs[fa] + cell * nda;
+      for (int i = 0; i < nda; ++i)
+        if (ga[i] > 0) rhs[ga[i] - 1] = rhs[ga[i] - 1] + elvec[m->eoff[fa] + i];
+    }
+  }
+  free(elmat);
+  free(elvec);
 }

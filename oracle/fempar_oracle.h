@@ -134,6 +134,34 @@ enum { ORC_EXACT_SIN = 0,      /* u = prod_d sin(pi x_d)   (the manufactured sol
 void orc_error_norms(const orc_space_t *s, const double *x_free, const double *fixed_values, int exact,
                      double out[4]);
 
+/* ---- multi-field spaces, block layout and block assembly ----
+ * serial_fe_space_t with several scalar Lagrangian fields on one structured
+ * mesh, their block_layout_t (field -> block, 1-based) and the
+ * block_sparse_assembler_t.  The integration plugin is the coupled
+ * diffusion-reaction system sum_f kappa_f (grad u_f, grad v_f) +
+ * sigma sum_{f<g} (u_f - u_g, v_f - v_g) + source f_f per field (entries
+ * in orc_mf_element); homogeneous Dirichlet per field (bc_kind of each
+ * field's problem). */
+#define ORC_MF_MAXF 4
+typedef struct orc_mf orc_mf_t;
+typedef struct {
+  double kappa[ORC_MF_MAXF];
+  double sigma;
+  double f[ORC_MF_MAXF];
+} orc_mf_physics_t;
+orc_mf_t *orc_mf_create(int nfields, const orc_problem_t *fields, const int *field_blocks);
+void orc_mf_free(orc_mf_t *m);
+int orc_mf_num_blocks(const orc_mf_t *m);
+void orc_mf_field_sizes(const orc_mf_t *m, int field, int *nd, int *n_free, int *n_fixed);
+int orc_mf_block_size(const orc_mf_t *m, int block);
+/* field's lst_dofs_gids (cells x nd): free ids 1-based inside the field's block, fixed ids negative */
+void orc_mf_get_dofs(const orc_mf_t *m, int field, int32_t *out);
+void orc_mf_element(const orc_mf_t *m, const orc_mf_physics_t *ph, int64_t cell, double *elmat,
+                    double *elvec);
+int64_t orc_mf_assemble_block(const orc_mf_t *m, const orc_mf_physics_t *ph, const int *field_coupling, int ib,
+                              int jb, int symmetric_storage, int32_t **irp, int32_t **ja, double **val);
+void orc_mf_assemble_rhs(const orc_mf_t *m, const orc_mf_physics_t *ph, int ib, double *rhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,13 @@ class CgResult(C.Structure):
                 ("rhs_test", C.c_double)]
 
 
+MF_MAXF = 4
+
+
+class MFPhysics(C.Structure):
+    _fields_ = [("kappa", C.c_double * MF_MAXF), ("sigma", C.c_double), ("f", C.c_double * MF_MAXF)]
+
+
 def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
@@ -79,6 +86,18 @@ def lib():
         L.orc_cg.argtypes = [C.c_int, P, P, P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                              C.c_int, P, P, C.POINTER(CgResult), P]
         L.orc_error_norms.argtypes = [P, P, P, C.c_int, P]
+        L.orc_mf_create.restype = P
+        L.orc_mf_create.argtypes = [C.c_int, P, P]
+        L.orc_mf_free.argtypes = [P]
+        L.orc_mf_num_blocks.argtypes = [P]
+        L.orc_mf_field_sizes.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_mf_block_size.argtypes = [P, C.c_int]
+        L.orc_mf_get_dofs.argtypes = [P, C.c_int, P]
+        L.orc_mf_element.argtypes = [P, C.POINTER(MFPhysics), C.c_int64, P, P]
+        L.orc_mf_assemble_block.restype = C.c_int64
+        L.orc_mf_assemble_block.argtypes = [P, C.POINTER(MFPhysics), P, C.c_int, C.c_int, C.c_int, C.POINTER(P),
+                                            C.POINTER(P), C.POINTER(P)]
+        L.orc_mf_assemble_rhs.argtypes = [P, C.POINTER(MFPhysics), C.c_int, P]
         _lib = L
     return _lib
 
@@ -224,3 +243,101 @@ def cg(irp, ja, val, b, symmetric_storage=True, precond="jacobi", criterion="RES
     if history:
         out["history"] = hist[: res.iterations].copy()
     return x, out
+
+
+def _take(ptr, count, ctype, dtype):
+    if count == 0:
+        lib().orc_free(ptr)
+        return np.zeros(0, dtype=dtype)
+    arr = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,)).copy()
+    lib().orc_free(ptr)
+    return arr.astype(dtype, copy=False)
+
+
+def make_mf_physics(kappa, sigma, f):
+    ph = MFPhysics()
+    for i in range(MF_MAXF):
+        ph.kappa[i] = kappa[i] if i < len(kappa) else 0.0
+        ph.f[i] = f[i] if i < len(f) else 0.0
+    ph.sigma = sigma
+    return ph
+
+
+class MultiFieldSpace:
+    """Several scalar fields on one structured mesh with a block layout
+    (oracle): per-field numbering with block offsets, block assembly."""
+
+    def __init__(self, problems, field_blocks):
+        self.nf = len(problems)
+        arr = (Problem * self.nf)(*problems)
+        fb = np.ascontiguousarray(field_blocks, dtype=np.int32)
+        self._keep = (arr, fb)
+        self.h = lib().orc_mf_create(self.nf, C.cast(arr, C.c_void_p), _ptr(fb))
+        if not self.h:
+            raise ValueError("oracle: unsupported multi-field problem")
+        self.field_blocks = list(field_blocks)
+        self.num_blocks = lib().orc_mf_num_blocks(self.h)
+        self.nd, self.n_free, self.n_fixed = [], [], []
+        for f in range(self.nf):
+            a, b, c = C.c_int(), C.c_int(), C.c_int()
+            lib().orc_mf_field_sizes(self.h, f, C.byref(a), C.byref(b), C.byref(c))
+            self.nd.append(a.value); self.n_free.append(b.value); self.n_fixed.append(c.value)
+        self.block_sizes = [lib().orc_mf_block_size(self.h, b) for b in range(1, self.num_blocks + 1)]
+        self.num_cells = 1
+        for k in range(problems[0].dim):
+            self.num_cells *= problems[0].n[k]
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_mf_free(self.h)
+        except Exception:
+            pass
+
+    def dofs(self, field):
+        out = np.empty((self.num_cells, self.nd[field]), dtype=np.int32)
+        lib().orc_mf_get_dofs(self.h, field, _ptr(out))
+        return out
+
+    def element(self, phys, cell=0):
+        ndt = sum(self.nd)
+        em = np.empty((ndt, ndt), dtype=np.float64, order="F")
+        ev = np.empty(ndt)
+        lib().orc_mf_element(self.h, C.byref(phys), cell, em.ctypes.data_as(C.c_void_p), _ptr(ev))
+        return em, ev
+
+    def assemble_block(self, phys, coupling, ib, jb, symmetric_storage=False):
+        """Block (ib, jb), 1-based: irp, ja (1-based), val."""
+        cp = np.ascontiguousarray(np.asarray(coupling, dtype=bool).astype(np.int32).ravel())
+        irp, ja, val = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        nnz = lib().orc_mf_assemble_block(self.h, C.byref(phys), _ptr(cp), ib, jb, int(bool(symmetric_storage)),
+                                          C.byref(irp), C.byref(ja), C.byref(val))
+        n = self.block_sizes[ib - 1]
+        return (_take(irp, n + 1, C.c_int32, np.int32), _take(ja, nnz, C.c_int32, np.int32),
+                _take(val, nnz, C.c_double, np.float64))
+
+    def assemble_rhs(self, phys, ib):
+        out = np.zeros(max(1, self.block_sizes[ib - 1]))
+        lib().orc_mf_assemble_rhs(self.h, C.byref(phys), ib, _ptr(out))
+        return out[: self.block_sizes[ib - 1]]
+
+    def merged(self, phys, coupling):
+        """The block system as one CSR (block rows in order, block columns
+        shifted by the sizes of the earlier blocks) and its RHS."""
+        offs = np.concatenate([[0], np.cumsum(self.block_sizes)]).astype(np.int64)
+        rows = []
+        for ib in range(1, self.num_blocks + 1):
+            parts = [self.assemble_block(phys, coupling, ib, jb) for jb in range(1, self.num_blocks + 1)]
+            for r in range(self.block_sizes[ib - 1]):
+                cols, vals = [], []
+                for jb, (irp, ja, val) in enumerate(parts):
+                    a, b = irp[r] - 1, irp[r + 1] - 1
+                    cols.append(ja[a:b] + offs[jb]); vals.append(val[a:b])
+                rows.append((np.concatenate(cols), np.concatenate(vals)))
+        irp = np.zeros(len(rows) + 1, dtype=np.int32)
+        irp[1:] = np.cumsum([len(c) for c, _ in rows])
+        irp += 1
+        ja = np.concatenate([c for c, _ in rows]).astype(np.int32)
+        val = np.concatenate([v for _, v in rows])
+        rhs = np.concatenate([self.assemble_rhs(phys, ib) for ib in range(1, self.num_blocks + 1)])
+        return irp, ja, val, rhs
