@@ -34,7 +34,7 @@ EXPORTS = [
     "femhip_space_get_dofs", "femhip_space_interpolate_dirichlet", "femhip_space_destroy",
     "femhip_operator_create", "femhip_operator_info", "femhip_operator_get_pattern",
     "femhip_operator_set_physics", "femhip_operator_compute", "femhip_operator_get_values",
-    "femhip_operator_apply", "femhip_operator_destroy",
+    "femhip_operator_apply", "femhip_operator_apply_noalloc", "femhip_operator_destroy",
     "femhip_solver_create", "femhip_solver_solve", "femhip_solver_get_solution", "femhip_solver_destroy",
     "femhip_solver_profile_spmv", "femhip_solver_profile_cg", "femhip_solver_error_norms",
     "femhip_solver_get_convergence_history",
@@ -141,7 +141,8 @@ def load():
         "femhip_operator_info": [P, C.POINTER(I32), C.POINTER(I64), C.POINTER(I64)],
         "femhip_operator_get_pattern": [P, P, P], "femhip_operator_set_physics": [P, C.POINTER(Physics)],
         "femhip_operator_compute": [P, P, C.POINTER(D)], "femhip_operator_get_values": [P, P, P],
-        "femhip_operator_apply": [P, P, P], "femhip_operator_destroy": [P],
+        "femhip_operator_apply": [P, P, P], "femhip_operator_apply_noalloc": [P, P, P],
+        "femhip_operator_destroy": [P],
         "femhip_solver_create": [P, P, C.POINTER(CgParams), pP],
         "femhip_solver_solve": [P, P, P, C.POINTER(CgResult)], "femhip_solver_get_solution": [P, P],
         "femhip_solver_destroy": [P],

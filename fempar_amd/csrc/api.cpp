@@ -89,7 +89,13 @@ int femhip_ctx_create(int32_t device, femhip_ctx_t **ctx) {
   hipDeviceProp_t prop;
   FH_HIP(hipGetDeviceProperties(&prop, device));
   c->num_cus = prop.multiProcessorCount;
-  FH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  try {
+    noalloc_setup(device);
+    FH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  } catch (...) {
+    delete c;
+    throw;
+  }
   *ctx = c;
   return FEMHIP_OK;
   FH_CATCH

@@ -272,6 +272,9 @@ void assemble(femhip_operator &op, const double *fixed_host, float *ms);
 void setup_sym_gather(femhip_operator &op);
 void upload_physics(femhip_operator &op);
 void spmv_full(femhip_operator &op, const double *d_x, double *d_y);
+// address of the no-allocation staging array on `device` (noalloc.hip); called
+// by femhip_ctx_create with that device current
+void noalloc_setup(int device);
 
 // element matrix (row-major) and body-force vector of the elasticity plugin
 void elasticity_element(const RefFE &fe, const femhip_physics_t &ph, double *Ke, double *Fe);

@@ -185,6 +185,15 @@ int femhip_operator_compute(femhip_operator_t *op, const double *fixed_values, d
 int femhip_operator_get_values(const femhip_operator_t *op, double *val, double *rhs);
 /* sparse_matrix_t%apply (sparse_matrix.f90:1264-1276): y = A x, host vectors */
 int femhip_operator_apply(femhip_operator_t *op, const double *x, double *y);
+/* No counterpart in the reference: femhip_operator_apply for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same result bit for bit; x and y
+ * are staged in a fixed 32 MiB device array per GPU instead of per-call
+ * allocations, so the operator may have at most 2,097,152 rows
+ * (FEMHIP_ERR_INVALID beyond).  Calls on one GPU share that array: they are
+ * serialised within the process, and calls from two processes on one GPU
+ * never share it (each process has its own copy). */
+int femhip_operator_apply_noalloc(femhip_operator_t *op, const double *x, double *y);
 int femhip_operator_destroy(femhip_operator_t *op);
 
 /* ---- iterative_linear_solver_t of type 'CG' ----
