@@ -136,9 +136,45 @@ def cpu_baseline(n, iters_full, op_matrix=None, rhs=None):
     O.cg(irp, ja, val, rhs, True, precond="jacobi", rtol=1e-30, maxit=k)
     t_it = (time.perf_counter() - t0) / k
     total = t_asm + t_it * iters_full
-    return {"value": sp.n_free / total, "unit": "DOFs/s", "cores": 1, "kind": "port",
-            "sample": f"oracle assembly of cells [0,{slab}) of the {n}^3 mesh (x8) + {k} Jacobi-PCG iterations "
-                      f"on the full matrix (x{iters_full} iterations); t_asm={t_asm:.2f}s t_iter={t_it*1e3:.1f}ms"}
+    out = {"value": sp.n_free / total, "unit": "DOFs/s", "cores": 1, "kind": "port",
+           "sample": f"oracle assembly of cells [0,{slab}) of the {n}^3 mesh (x8) + {k} Jacobi-PCG iterations "
+                     f"on the full matrix (x{iters_full} iterations); t_asm={t_asm:.2f}s t_iter={t_it*1e3:.1f}ms",
+           "includes": "COO append + sort/compress of the assembly (the GPU value starts from a resident pattern)"}
+    out["all_cores"] = cpu_baseline_all_cores(sp, ph, irp, ja, val, rhs, iters_full)
+    return out
+
+
+def cpu_baseline_all_cores(sp, ph, irp, ja, val, rhs, iters_full):
+    """All host threads (not the reference's serial algorithm): the oracle's
+    assembly of one z-slab per thread run concurrently (ctypes calls release
+    the GIL; the slabs' COO merge is not timed), plus 10 iterations of an
+    OpenMP Jacobi-PCG on the full-storage matrix, scaled to the solve."""
+    import concurrent.futures as cf
+    import scipy.sparse as sps
+    from oracle import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+    cells = sp.num_cells
+    bounds = [(cells * i // cores, cells * (i + 1) // cores) for i in range(cores)]
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(lambda c: sp.assemble(ph, True, cells=c), bounds))
+    t_asm = time.perf_counter() - t0
+    n = len(irp) - 1
+    U = sps.csr_matrix((val, ja - 1, irp - 1), shape=(n, n))
+    A = (U + U.T - sps.diags(U.diagonal())).tocsr()
+    A.sort_indices()
+    firp, fja = (A.indptr + 1).astype(np.int32), (A.indices + 1).astype(np.int32)
+    k = 10
+    t0 = time.perf_counter()
+    O.cg_threads(firp, fja, A.data, rhs, rtol=1e-30, maxit=k)
+    t_it = (time.perf_counter() - t0) / k
+    total = t_asm + t_it * iters_full
+    return {"value": sp.n_free / total, "unit": "DOFs/s", "cores": cores,
+            "kind": "port (threaded slab assembly + OpenMP CG; not the reference's serial algorithm)",
+            "sample": f"{cores} concurrent slab assemblies covering all cells + {k} OpenMP Jacobi-PCG iterations "
+                      f"(x{iters_full}); t_asm={t_asm:.2f}s t_iter={t_it*1e3:.1f}ms"}
 
 
 def batched_problems(count=4096, seed=4096):
@@ -172,11 +208,23 @@ def run_batched(F, env, steps, rtol=1e-8):
            "latency_ms_max": float(lat.max()), "converged": int(np.sum(last["did_converge"])),
            "max_true_rel_residual": float(rel.max()), "threads_per_block": threads,
            "iterations_total": int(np.sum(last["num_iterations"]))}
+    # roofline of the solve launch: FP32 inner iterations of every problem,
+    # (4 3^d + 40) bytes per row and iteration (the diagonals once, ten 4-byte
+    # vector passes), over the launch time
+    dims = np.array([p[0] for p in problems])
+    per_row = 4.0 * 3.0 ** dims + 40.0
+    alg = float(np.sum(np.asarray(last["num_iterations"], dtype=float) * np.asarray(last["num_dofs"], dtype=float)
+                       * per_row))
+    gbs = alg / (ms.mean() / 1e3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": alg,
+                       "definition": "sum over problems of inner iterations x free DOFs x (4*3^d + 40) B / solve_ms"}
     b.free()
     return out
 
 
-SERVE_RATES = (100, 200, 500, 1000, 2000, 5000)
+SERVE_RATES = (10, 20, 30, 100, 200, 500, 1000, 2000, 5000)   # below and above the ~30 solves/s capacity
+SERVE_POLICY_RATES = (30, 1000)                                   # also served smallest-first
 
 
 def serving_requests(count, seed=5000):
@@ -190,15 +238,19 @@ def serving_requests(count, seed=5000):
     return [(int(n[i]), float(E[i]), float(nu[i])) for i in range(count)]
 
 
-def run_serving(F, env, count, rates=SERVE_RATES, rtol=1e-8):
+def run_serving(F, env, count, rates=SERVE_RATES, rtol=1e-8, policy_rates=SERVE_POLICY_RATES):
     """Open-loop Poisson arrivals at each rate: the same `count` requests are
     submitted with arrival times from exponential gaps (seed 5000 + rate); the
-    server admits them as slots and HBM budget free up.  Latency = arrival to
-    solution on the host."""
+    server admits them as slots and HBM budget free up, first come first
+    served ("rates") and, at policy_rates, smallest estimate first with a 2 s
+    aging bound ("smallest_first").  Latency = arrival to solution on the
+    host.  Roofline: FP32 inner iterations x DOFs x 364 B (81 FP32 stencil
+    values and ten 4-byte vector passes per row) over the span."""
     reqs = serving_requests(count)
-    out = {"requests_per_rate": count, "rates": {}}
-    for lam in rates:
-        srv = F.serve_t(env, hbm_budget_bytes=96 << 30, max_active=8, setup_threads=8, rtol=rtol)
+    out = {"requests_per_rate": count, "rates": {}, "smallest_first": {}}
+    for lam, policy in [(r, "fcfs") for r in rates] + [(r, "smallest") for r in policy_rates]:
+        srv = F.serve_t(env, hbm_budget_bytes=96 << 30, max_active=8, setup_threads=8, rtol=rtol,
+                        admission=policy, aging_s=2.0 if policy == "smallest" else 0.0)
         gaps = np.random.default_rng(5000 + lam).exponential(1.0 / lam, count)
         t0 = srv.now() + 0.02
         arrivals = t0 + np.cumsum(gaps) - gaps[0]
@@ -210,8 +262,10 @@ def run_serving(F, env, count, rates=SERVE_RATES, rtol=1e-8):
         lat = np.array([r["latency_ms"] for r in res])
         dofs = sum(r["num_dofs"] for r in res)
         span = end - arrivals[0]
-        out["rates"][str(lam)] = {
+        alg = float(sum(r["num_iterations"] * r["num_dofs"] for r in res)) * 364.0
+        out["rates" if policy == "fcfs" else "smallest_first"][str(lam)] = {
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "hbm_GBs": alg / span / 1e9, "hbm_frac": alg / span / 1e9 / HBM_PEAK_GBS,
             "max_ms": float(lat.max()), "solves_per_s": count / span, "dofs_per_s": dofs / span,
             "mean_queue_ms": float(np.mean([r["queue_ms"] for r in res])),
             "mean_setup_ms": float(np.mean([r["setup_ms"] for r in res])),
@@ -220,7 +274,8 @@ def run_serving(F, env, count, rates=SERVE_RATES, rtol=1e-8):
             "max_true_rel_residual": float(max(r["true_residual"] / max(r["rhs_norm"], 1e-300) for r in res)),
             "peak_active": st["peak_active"], "running_peak": st["running_peak"],
             "hbm_peak_GB": st["hbm_peak"] / 1e9}
-        print(f"[serve] rate {lam}/s: {out['rates'][str(lam)]}", file=sys.stderr, flush=True)
+        print(f"[serve] rate {lam}/s {policy}: {out['rates' if policy == 'fcfs' else 'smallest_first'][str(lam)]}",
+              file=sys.stderr, flush=True)
     return out
 
 
@@ -241,7 +296,7 @@ def main():
     ap.add_argument("--no-q2", action="store_true", help="skip the 64^3 Q2 configuration")
     ap.add_argument("--no-grf", action="store_true", help="skip the 192^3 random-field mixed-precision configuration")
     ap.add_argument("--configs-only", action="store_true", help="only the Q2 and random-field configurations")
-    ap.add_argument("--serve-requests", type=int, default=32, help="requests per arrival rate")
+    ap.add_argument("--serve-requests", type=int, default=200, help="requests per arrival rate")
     ap.add_argument("--serving-only", action="store_true", help="only the serving sweep (printed as JSON)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only launch the FP64 and FP32 SpMV kernels 50 times each (for rocprofv3 --pmc runs)")
@@ -388,6 +443,10 @@ def main():
         "setup_s": t_setup, "warmup_step_s": t_first,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": (f"profiles/spmv_traffic_poisson3d_q1_{n}.json: rocprofv3 --pmc FETCH_SIZE and "
+                                        "WRITE_SIZE of this kernel at this size, calibrated by a plain copy; "
+                                        "read from the committed file, not measured in this run")
+                     if traffic is not None else None,
                      "kernel": "k_spmv<double> (FP64 CG SpMV on the sliced solver copy)",
                      "kernel_ms": spmv_ms, "algorithmic_bytes": spmv_bytes,
                      "kernel_ms_in_solve_events": insolve_ms, "in_solve_launches": insolve_its},

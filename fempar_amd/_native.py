@@ -43,7 +43,22 @@ EXPORTS = [
     "femhip_serve_create", "femhip_serve_estimate_bytes", "femhip_serve_submit", "femhip_serve_wait",
     "femhip_serve_get_solution", "femhip_serve_release", "femhip_serve_stats", "femhip_serve_now",
     "femhip_serve_destroy",
+    "femhip_block_operator_create", "femhip_block_operator_info", "femhip_block_operator_get_dofs",
+    "femhip_block_operator_get_block_pattern", "femhip_block_operator_set_physics", "femhip_block_operator_compute",
+    "femhip_block_operator_get_block_values", "femhip_block_operator_get_rhs", "femhip_block_operator_apply",
+    "femhip_block_operator_solve", "femhip_block_operator_destroy",
 ]
+
+MAX_FIELDS = 4
+
+
+class BlockLayout(C.Structure):
+    _fields_ = [("num_fields", C.c_int32), ("fields", C.c_void_p * MAX_FIELDS),
+                ("field_blocks", C.c_int32 * MAX_FIELDS), ("field_coupling", C.c_int32 * (MAX_FIELDS * MAX_FIELDS))]
+
+
+class MultiFieldPhysics(C.Structure):
+    _fields_ = [("kappa", C.c_double * MAX_FIELDS), ("sigma", C.c_double), ("f", C.c_double * MAX_FIELDS)]
 
 
 class BatchProblem(C.Structure):
@@ -166,6 +181,17 @@ def load():
         "femhip_serve_stats": [P, C.POINTER(ServeStats)],
         "femhip_serve_now": [P],
         "femhip_serve_destroy": [P],
+        "femhip_block_operator_create": [P, C.POINTER(BlockLayout), I32, pP],
+        "femhip_block_operator_info": [P, C.POINTER(I32), P, P],
+        "femhip_block_operator_get_dofs": [P, I32, P],
+        "femhip_block_operator_get_block_pattern": [P, I32, I32, P, P],
+        "femhip_block_operator_set_physics": [P, C.POINTER(MultiFieldPhysics)],
+        "femhip_block_operator_compute": [P, C.POINTER(D)],
+        "femhip_block_operator_get_block_values": [P, I32, I32, P],
+        "femhip_block_operator_get_rhs": [P, I32, P],
+        "femhip_block_operator_apply": [P, P, P],
+        "femhip_block_operator_solve": [P, C.POINTER(CgParams), P, P, C.POINTER(CgResult)],
+        "femhip_block_operator_destroy": [P],
     }
     restype = {"femhip_last_error": C.c_char_p, "femhip_serve_estimate_bytes": C.c_double,
                "femhip_serve_now": C.c_double}

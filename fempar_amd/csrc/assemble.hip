@@ -691,6 +691,115 @@ __global__ __launch_bounds__(kMfmaRows) void k_assemble_gather_mfma(
   for (int32_t e = tid; e < b1 - b0; e += kMfmaRows) val[b0 + e] = lds[e];   // rows <= K entries each
 }
 
+// k alone at quadrature point q (the FP32 integration needs no source)
+template <int DIM, int NQ1 = 2>
+__device__ __forceinline__ double coef_k(const PhysDev &ph, const TrigTab &tt, const int ijk[3], int q,
+                                         int64_t cell) {
+  if (ph.k_kind == FEMHIP_K_CELL) return ph.k_cell[cell];
+  if (ph.k_kind != FEMHIP_K_SIN3) return ph.k0;
+  double s[3] = {1, 1, 1};
+  int qr = q;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const int qd = qr % NQ1;
+    qr /= NQ1;
+    s[d] = tt.t[tt.off[d] + ijk[d] * NQ1 + qd].z;
+  }
+  return 1.0 + 0.5 * (s[0] * s[1] * s[2]);
+}
+
+// Stored (upper) diagonals of the FP32 symmetric-diagonal copy: diagonal d
+// holds neighbour code code[d] (-1: none) at column offset off[d].
+struct Sym32Out {
+  float *v32;
+  int64_t stride;
+  int32_t nd;
+  int8_t code[16];
+  int32_t off[16];
+};
+
+// FP32 integration of the refined solver's matrix (scalar Q1, symmetric-
+// diagonal layout) on the matrix cores with FP32 accumulation.  For a tile of
+// 16 rows, acc[row][d] = sum over the (cell e around the row's vertex, point q)
+// pairs of k_e(q) W[NQ e + q][d], with W[NQ e + q][d] = M_q[row's local vertex
+// in e][local vertex of the neighbour on diagonal d] (0 when that neighbour is
+// not a vertex of e): one v_mfma_f32_16x16x4f32 per 4 pairs and tile, the
+// MFMA's FP32 accumulator holding the sum.  Each stored value is then scaled
+// by S_r S_c (S = diag(A)^-1/2 of the FP64 matrix; none without Jacobi) and
+// written once; a fixed or absent neighbour and the padding rows get 0.
+// Operand layouts of the instruction: A[m][k] in lane m + 16 k, B[k][n] in
+// lane n + 16 k, D[m][n] in lane n + 16 (m / 4), slot m % 4.
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <int DIM>
+__global__ __launch_bounds__(kMfmaRows) void k_assemble_sym32_mfma(
+    VtxGrid g, PhysDev ph, TrigTab tt, int32_t n, int64_t rows, const int32_t *__restrict__ row_vtx,
+    const uint8_t *__restrict__ slot, const double *__restrict__ tab, const double *__restrict__ sc, Sym32Out so) {
+  constexpr int ND = 1 << DIM, NQ = ND, KK = ND * NQ, KS = DIM == 2 ? 16 : 32;
+  __shared__ float w[KK * 16];
+  __shared__ float out[kMfmaRows * 17];   // row values, padded against bank conflicts
+  const double *M = tab;                  // M[NQ][ND][ND]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int64_t r0 = (int64_t)blockIdx.x * kMfmaRows;
+  for (int i = tid; i < KK * 16; i += kMfmaRows) {
+    const int kk = i / 16, d = i % 16, e = kk / NQ, q = kk % NQ;
+    float v = 0.f;
+    const int code = d < so.nd ? so.code[d] : -1;
+    if (code >= 0) {
+      const int jx = code % 3 - (e & 1), jy = (code / 3) % 3 - ((e >> 1) & 1);
+      const int jz = DIM > 2 ? code / 9 - (e >> 2) : 0;
+      if (jx >= 0 && jx <= 1 && jy >= 0 && jy <= 1 && jz >= 0 && jz <= 1)
+        v = (float)M[(q * ND + (ND - 1 - e)) * ND + jx + 2 * jy + 4 * jz];
+    }
+    w[i] = v;
+  }
+  int ax[4][3];
+  bool ok[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {   // the lane's row in each of the wave's four tiles
+    const int64_t row = r0 + wv * 64 + mt * 16 + (lane & 15);
+    ok[mt] = row < n;
+    ax[mt][0] = ax[mt][1] = ax[mt][2] = 0;
+    if (ok[mt]) vtx_coords(g, row_vtx[row], ax[mt]);
+  }
+  __syncthreads();
+  v4f acc[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) acc[mt] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int ks = 0; ks < KK / 4; ++ks) {
+    const int kk = ks * 4 + (lane >> 4), e = kk / NQ, q = kk % NQ;
+    const float wb = w[kk * 16 + (lane & 15)];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      float kq = 0.f;
+      const int ci = ax[mt][0] - 1 + (e & 1), cj = ax[mt][1] - 1 + ((e >> 1) & 1);
+      const int ck = DIM > 2 ? ax[mt][2] - 1 + (e >> 2) : 0;
+      if (ok[mt] && ci >= 0 && ci < g.n[0] && cj >= 0 && cj < g.n[1] && (DIM == 2 || (ck >= 0 && ck < g.n[2]))) {
+        const int64_t cell = ci + (int64_t)g.n[0] * (cj + (int64_t)g.n[1] * ck);
+        const int ijk[3] = {ci, cj, ck};
+        kq = (float)coef_k<DIM>(ph, tt, ijk, q, cell);
+      }
+      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(kq, wb, acc[mt], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[(wv * 64 + mt * 16 + 4 * (lane >> 4) + i) * 17 + (lane & 15)] = acc[mt][i];
+  __syncthreads();
+  const int64_t r = r0 + tid;
+  if (r >= rows) return;
+  const double sr = (sc && r < n) ? sc[r] : 1.0;
+  for (int d = 0; d < so.nd; ++d) {
+    float a = 0.f;
+    const int code = so.code[d];
+    if (r < n && code >= 0 && slot[r * KS + code] != 255) {
+      a = out[tid * 17 + d];
+      if (sc) a = (float)((double)a * (sr * sc[min(r + (int64_t)so.off[d], (int64_t)n - 1)]));
+    }
+    so.v32[(int64_t)d * so.stride + r] = a;
+  }
+}
+
 __global__ void k_spmv_csr(int32_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                            const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y) {
   int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -892,6 +1001,42 @@ void setup_sym_gather(femhip_operator &op) {
   FH_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
   FH_HIP(hipStreamSynchronize(st));
   if (!h_bad) op.sym_gather = 1;
+}
+
+// FP32 symmetric-diagonal copy of A (sc == nullptr) or of S A S integrated on
+// the matrix cores (k_assemble_sym32_mfma); false where that kernel does not
+// apply (the copy is then derived from the FP64 one by the solver).
+bool assemble_sym32(femhip_operator &op, const double *sc, float *v32) {
+  if (op.gather != 1 || op.sym_gather != 1 || !op.sl.sym || op.tab_kind != 0 || op.n <= 0 || op.sl.kplus > 16)
+    return false;
+  femhip_space &s = *op.space;
+  const int dim = s.fe.dim;
+  if (dim != 2 && dim != 3) return false;
+  Sym32Out so{};
+  so.v32 = v32;
+  so.stride = op.sl.stride;
+  so.nd = op.sl.kplus;
+  const int K = dim == 2 ? 9 : 27;
+  for (int d = 0; d < 16; ++d) {
+    so.code[d] = -1;
+    so.off[d] = d < op.sl.kplus ? op.sl.h_soffs[(size_t)d] : 0;
+  }
+  for (int k = 0; k < K; ++k)
+    if (op.sym_kidx[k] >= 0) so.code[op.sym_kidx[k]] = (int8_t)k;
+  const VtxGrid g = vtx_grid(s.mesh);
+  const TrigTab tt{reinterpret_cast<const double4 *>(op.trig.p), {op.trig_off[0], op.trig_off[1], op.trig_off[2]}};
+  const PhysDev ph{op.phys.kind, op.phys.k_kind, op.phys.f_kind, op.phys.k0, op.phys.f0, op.k_cell.p};
+  const int64_t rows = op.sl.stride;   // padding rows [n, stride) are cleared
+  const dim3 G((unsigned)((rows + kMfmaRows - 1) / kMfmaRows)), B(kMfmaRows);
+  hipStream_t st = op.ctx->stream;
+  if (dim == 2)
+    hipLaunchKernelGGL(k_assemble_sym32_mfma<2>, G, B, 0, st, g, ph, tt, op.n, rows, op.row_vtx.p, op.slot.p,
+                       op.tab.p, sc, so);
+  else
+    hipLaunchKernelGGL(k_assemble_sym32_mfma<3>, G, B, 0, st, g, ph, tt, op.n, rows, op.row_vtx.p, op.slot.p,
+                       op.tab.p, sc, so);
+  FH_HIP(hipGetLastError());
+  return true;
 }
 
 namespace {

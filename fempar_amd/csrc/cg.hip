@@ -1379,6 +1379,19 @@ SlicedDev fp32_view(const femhip_solver &s, SlicedDev m) {
   return m;
 }
 
+// The refined solver's FP32 matrix is the FP64 one rounded (k_scale32) unless
+// FEMHIP_FP32_INTEGRATION=1 asks for it integrated in FP32 on the matrix cores
+// (assemble_sym32, scalar Q1).  Rounding is the default because it was
+// measured faster: the integrated matrix carries a few FP32 ulps of
+// accumulation error against the half ulp of rounding, and the reliable
+// updates, which compare against the FP64 residual, then take more inner
+// iterations (128^3 Q1: 336 against 263; 192^3 random field: 940 against
+// 842).  Read per call.
+bool fp32_integration() {
+  const char *e = getenv("FEMHIP_FP32_INTEGRATION");
+  return e && atoi(e) == 1;
+}
+
 void prepare(femhip_solver &s) {
   femhip_operator &op = *s.op;
   FH_CHECK(op.computed, FEMHIP_ERR_STATE, "solve before fe_affine_operator%compute");
@@ -1467,10 +1480,14 @@ void prepare(femhip_solver &s) {
   // FP32 copy: of S A S with Jacobi, of A without
   if (mx && (op.sl.filled_version32 != op.version || op.sl.scaled32 != jac)) {
     if (!op.sl.v32.p) op.sl.v32.alloc((size_t)std::max<int64_t>(1, op.sl.entries));
-    const int64_t threads = (int64_t)op.sl.ns * kSlice;
-    hipLaunchKernelGGL(k_scale32, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, sliced_dev(op),
-                       op.sl.v64.p, jac ? s.invd.p : nullptr, op.sl.v32.p);
-    FH_HIP(hipGetLastError());
+    // rounded from the FP64 copy, or integrated in FP32 on the matrix cores
+    // where asked for and supported (scalar Q1 on the symmetric-diagonal copy)
+    if (!fp32_integration() || !assemble_sym32(op, jac ? s.invd.p : nullptr, op.sl.v32.p)) {
+      const int64_t threads = (int64_t)op.sl.ns * kSlice;
+      hipLaunchKernelGGL(k_scale32, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, sliced_dev(op),
+                         op.sl.v64.p, jac ? s.invd.p : nullptr, op.sl.v32.p);
+      FH_HIP(hipGetLastError());
+    }
     op.sl.filled_version32 = op.version;
     op.sl.scaled32 = jac;
   }
@@ -1515,11 +1532,13 @@ struct Launcher {
   }
 };
 
-// FP32-refined iteration slots as two kernels (k_rf_a, k_rf_b; default) or as
-// the three of the FP64 form (k_ru_spmv, k_ru_update, k_ru_dir; FEMHIP_RF3=1).
+// FP32-refined iteration slots as the three kernels of the FP64 form
+// (k_ru_spmv, k_ru_update, k_ru_dir; default) or as two (k_rf_a, k_rf_b;
+// FEMHIP_RF2=1).  The two-kernel form was measured no faster: 128^3 Q1 CG
+// 17.34 against 17.48 ms on the same iterations, 64^3 Q2 95.6 ms against 85.9.
 bool three_kernel_refined() {
-  static const char *env = getenv("FEMHIP_RF3");
-  return env && atoi(env) == 1;
+  static const char *env = getenv("FEMHIP_RF2");
+  return !(env && atoi(env) == 1);
 }
 
 // Enqueue one iteration of parity `par`: SpMV, update, direction.  With

@@ -74,7 +74,7 @@ struct RefFE {
   double detj[kMaxNq];
   double grad[kMaxNd][3][kMaxNq];           // physical gradients of scalar shapes (first cell map)
 };
-void build_reffe(RefFE &fe, int dim, int order, int ncomp, const double h[3]);
+void build_reffe(RefFE &fe, int dim, int order, int ncomp, const double h[3], int nq1 = 0);
 void gauss_legendre(int n, double *x, double *w);
 
 // --------------------------------------------------------------- context ---
@@ -256,6 +256,9 @@ __device__ __forceinline__ int64_t cell_index(const FaceTab &t, const int (&ijk)
 // test (every row's column offsets in the longest row's set D, structurally
 // symmetric); the linear Dirichlet function at the fixed DOFs
 void number_structured(femhip_space &s);
+// exclusive scan of n counts on the stream (out may alias in); out[n] = total
+// when with_total; returns the total (checked below 2^31)
+int64_t exclusive_scan(hipStream_t st, const int32_t *in, int32_t *out, int64_t n, bool with_total);
 void representatives_from_dofs(femhip_space &s);
 void build_pattern(femhip_operator &op);
 void ensure_host_pattern(const femhip_operator &op);
@@ -270,6 +273,9 @@ void assemble(femhip_operator &op, const double *fixed_host, float *ms);
 // once the symmetric-diagonal solver copy exists: decide whether the
 // owner-computes assembly can write it directly (sets op.sym_gather)
 void setup_sym_gather(femhip_operator &op);
+// FP32 symmetric-diagonal copy of A, or of S A S with sc = S, integrated in
+// FP32 on the matrix cores (scalar Q1); false where that does not apply
+bool assemble_sym32(femhip_operator &op, const double *sc, float *v32);
 void upload_physics(femhip_operator &op);
 void spmv_full(femhip_operator &op, const double *d_x, double *d_y);
 // address of the no-allocation staging array on `device` (noalloc.hip); called

@@ -199,6 +199,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(int32_t *__restrict__
     if (i0 + k < n) out[i0 + k] = (int32_t)(out[i0 + k] + b);
 }
 
+}  // namespace
+
 // out[i] = sum of in[0..i) for i <= n (out holds n + 1 entries when with_total);
 // returns the total
 int64_t exclusive_scan(hipStream_t st, const int32_t *in, int32_t *out, int64_t n, bool with_total) {
@@ -222,6 +224,8 @@ int64_t exclusive_scan(hipStream_t st, const int32_t *in, int32_t *out, int64_t 
   if (with_total) FH_HIP(hipMemcpy(out + n, &total, sizeof(int32_t), hipMemcpyHostToDevice));
   return total;
 }
+
+namespace {
 
 // --------------------------------------------------------------- pattern ---
 // One wavefront per row.  The row's DOF sits on the vef of its representative

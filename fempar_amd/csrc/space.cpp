@@ -75,7 +75,7 @@ static void lagrange(int order, int i, double x, double &v, double &dv) {
   dv = p1 * c;
 }
 
-void build_reffe(RefFE &fe, int dim, int order, int ncomp, const double h[3]) {
+void build_reffe(RefFE &fe, int dim, int order, int ncomp, const double h[3], int nq1) {
   std::memset(&fe, 0, sizeof(fe));
   fe.dim = dim;
   fe.order = order;
@@ -120,8 +120,10 @@ void build_reffe(RefFE &fe, int dim, int order, int ncomp, const double h[3]) {
         if (node_face[a] == f) fe.own_lst[cnt++] = c * fe.nnode + a;
   }
   fe.own_ptr[fe.nnf] = cnt;
-  // quadrature of degree 2*order, first index fastest
-  fe.nq1 = order + 1;
+  // quadrature of degree 2*order (or nq1 points per direction when a
+  // multi-field cell integrates every field with its highest order's rule),
+  // first index fastest
+  fe.nq1 = nq1 > order ? nq1 : order + 1;
   double x1[8], w1[8];
   gauss_legendre(fe.nq1, x1, w1);
   fe.nq = 1;

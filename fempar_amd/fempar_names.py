@@ -343,9 +343,33 @@ class iterative_linear_solver_t:
             N.check(self._lib.femhip_solver_create(self.environment.h, self.A.h, C.byref(p), C.byref(h)))
             self.h = h
 
+    def _cg_params(self):
+        p = N.CgParams()
+        p.rtol = float(self.params[ILS_RTOL])
+        p.atol = float(self.params[ILS_ATOL])
+        p.max_num_iterations = int(self.params[ILS_MAX_NUM_ITERATIONS])
+        p.stopping_criterion = N.STOPPING_CRITERIA.index(self.params[ILS_STOPPING_CRITERIUM])
+        p.preconditioner = getattr(self.M, "kind", N.PREC_IDENTITY)
+        p.precision = {"FP64": N.PRECISION_FP64, "FP32_REFINED": N.PRECISION_FP32_REFINED}[self.params[ILS_PRECISION]]
+        p.inner_rtol = float(self.params[ILS_INNER_RTOL])
+        p.max_outer = int(self.params[ILS_MAX_OUTER])
+        return p
+
     def solve(self, b=None, x=None):
         """Solve A x = b from the initial guess x (zero if None); b None uses the
-        operator's assembled RHS on the device.  Returns x (host)."""
+        operator's assembled RHS on the device.  Returns x (host).  A block
+        operator is solved over its blocks' rows in block order."""
+        if isinstance(self.A, block_fe_affine_operator_t):
+            if self.type is None:
+                raise FemhipError(N.ERR_STATE, "set_type_from_string not called")
+            n = self.A.num_rows
+            bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+            xx = np.zeros(n) if x is None else np.array(x, dtype=np.float64)
+            res = N.CgResult()
+            p = self._cg_params()
+            N.check(self._lib.femhip_block_operator_solve(self.A.h, C.byref(p), N.ptr(bb), N.ptr(xx), C.byref(res)))
+            self.result = {f: getattr(res, f) for f, _ in N.CgResult._fields_}
+            return xx
         self._ensure()
         n = self.A.n
         bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
@@ -417,6 +441,152 @@ class iterative_linear_solver_t:
     def __del__(self):
         try:
             self._destroy()
+        except Exception:
+            pass
+
+
+class block_layout_t:
+    """block_layout_t (Integration/FEOperator/block_layout.f90:36-264): which
+    block each field's DOFs go to (1-based) and which field pairs couple."""
+
+    def create(self, num_fields, field_id_to_block_id=None, field_coupling=None):
+        if not 1 <= num_fields <= N.MAX_FIELDS:
+            raise FemhipError(N.ERR_INVALID, f"num_fields must be in [1, {N.MAX_FIELDS}]")
+        self.num_fields = num_fields
+        self.field_id_to_block_id = list(field_id_to_block_id) if field_id_to_block_id is not None \
+            else [1] * num_fields
+        self.field_coupling = np.ones((num_fields, num_fields), dtype=bool) if field_coupling is None \
+            else np.asarray(field_coupling, dtype=bool).reshape(num_fields, num_fields)
+        self.num_blocks = max(self.field_id_to_block_id)
+        # block_layout_create (block_layout.f90:106-113): blocks couple where any of their fields do
+        self.block_coupling = np.zeros((self.num_blocks, self.num_blocks), dtype=bool)
+        for i in range(num_fields):
+            for j in range(num_fields):
+                self.block_coupling[self.field_id_to_block_id[i] - 1, self.field_id_to_block_id[j] - 1] |= \
+                    self.field_coupling[i, j]
+        return self
+
+    def get_num_fields(self):
+        return self.num_fields
+
+    def get_num_blocks(self):
+        return self.num_blocks
+
+    def get_block_id(self, field_id):
+        return self.field_id_to_block_id[field_id - 1]
+
+    def fields_coupled(self, ifield, jfield):
+        return bool(self.field_coupling[ifield - 1, jfield - 1])
+
+    def blocks_coupled(self, iblock, jblock):
+        return bool(self.block_coupling[iblock - 1, jblock - 1])
+
+
+class coupled_diffusion_discrete_integration_t:
+    """Multi-field plugin: sum_f kappa_f (grad u_f, grad v_f) + sigma sum_{f<g}
+    (u_f - u_g, v_f - v_g) = sum_f (f_f, v_f)."""
+
+    def __init__(self, kappa, sigma=0.0, f=None):
+        self.kappa, self.sigma = list(kappa), float(sigma)
+        self.f = list(f) if f is not None else [0.0] * len(self.kappa)
+
+    def physics(self):
+        p = N.MultiFieldPhysics()
+        for i in range(N.MAX_FIELDS):
+            p.kappa[i] = self.kappa[i] if i < len(self.kappa) else 0.0
+            p.f[i] = self.f[i] if i < len(self.f) else 0.0
+        p.sigma = self.sigma
+        return p
+
+
+class block_fe_affine_operator_t:
+    """fe_affine_operator_t over several fields with a block_sparse_assembler_t
+    (Integration/FEOperator/block_sparse_assembler.f90:47-224): one
+    block_sparse_matrix_t block per coupled block pair and one RHS block per block."""
+
+    def create(self, sparse_matrix_storage_format, diagonal_blocks_symmetric_storage, fe_spaces, block_layout,
+               discrete_integration):
+        if sparse_matrix_storage_format != "CSR":
+            raise FemhipError(N.ERR_INVALID, "only CSR storage is supported")
+        self._lib = N.load()
+        self.fe_spaces, self.block_layout = list(fe_spaces), block_layout
+        self.symmetric_storage = bool(diagonal_blocks_symmetric_storage[0])
+        nf = block_layout.num_fields
+        if len(self.fe_spaces) != nf:
+            raise FemhipError(N.ERR_INVALID, "one FE space per field")
+        L = N.BlockLayout()
+        L.num_fields = nf
+        for f in range(nf):
+            L.fields[f] = self.fe_spaces[f].h.value if hasattr(self.fe_spaces[f].h, "value") else self.fe_spaces[f].h
+            L.field_blocks[f] = block_layout.field_id_to_block_id[f]
+            for g in range(nf):
+                L.field_coupling[f * nf + g] = int(block_layout.field_coupling[f, g])
+        env = self.fe_spaces[0].triangulation.environment
+        h = C.c_void_p()
+        N.check(self._lib.femhip_block_operator_create(env.h, C.byref(L), int(self.symmetric_storage), C.byref(h)))
+        self.h = h
+        nb = C.c_int32()
+        N.check(self._lib.femhip_block_operator_info(h, C.byref(nb), None, None))
+        self.num_blocks = nb.value
+        sizes = np.zeros(self.num_blocks, dtype=np.int32)
+        nnz = np.zeros(self.num_blocks * self.num_blocks, dtype=np.int64)
+        N.check(self._lib.femhip_block_operator_info(h, C.byref(nb), N.ptr(sizes), N.ptr(nnz)))
+        self.block_sizes = [int(v) for v in sizes]
+        self.block_nnz = nnz.reshape(self.num_blocks, self.num_blocks)
+        self.num_rows = int(sizes.sum())
+        self._phys = discrete_integration.physics()
+        N.check(self._lib.femhip_block_operator_set_physics(h, C.byref(self._phys)))
+        self.last_compute_ms = None
+        return self
+
+    def compute(self):
+        ms = C.c_double()
+        N.check(self._lib.femhip_block_operator_compute(self.h, C.byref(ms)))
+        self.last_compute_ms = ms.value
+
+    def get_lst_dofs_gids(self, field_id):
+        s = self.fe_spaces[field_id - 1]
+        out = np.empty((s.num_cells, s.num_dofs_x_cell), dtype=np.int32)
+        N.check(self._lib.femhip_block_operator_get_dofs(self.h, field_id, N.ptr(out)))
+        return out
+
+    def get_block(self, ib, jb) -> csr_matrix_view:
+        """block_sparse_matrix_t%get_block(ib, jb) (1-based)."""
+        n = self.block_sizes[ib - 1]
+        nnz = int(self.block_nnz[ib - 1, jb - 1])
+        irp = np.empty(n + 1, dtype=np.int32)
+        ja = np.empty(max(nnz, 1), dtype=np.int32)
+        val = np.empty(max(nnz, 1))
+        N.check(self._lib.femhip_block_operator_get_block_pattern(self.h, ib, jb, N.ptr(irp), N.ptr(ja)))
+        N.check(self._lib.femhip_block_operator_get_block_values(self.h, ib, jb, N.ptr(val)))
+        return csr_matrix_view(irp, ja[:nnz], val[:nnz], self.symmetric_storage and ib == jb)
+
+    def get_translation(self):
+        """The block RHS, all blocks' rows in block order."""
+        out = []
+        for ib in range(1, self.num_blocks + 1):
+            r = np.empty(max(self.block_sizes[ib - 1], 1))
+            N.check(self._lib.femhip_block_operator_get_rhs(self.h, ib, N.ptr(r)))
+            out.append(r[: self.block_sizes[ib - 1]])
+        return np.concatenate(out)
+
+    def get_tangent(self):
+        return self
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.num_rows)
+        N.check(self._lib.femhip_block_operator_apply(self.h, N.ptr(x), N.ptr(y)))
+        return y
+
+    def free(self):
+        if getattr(self, "h", None):
+            self._lib.femhip_block_operator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass
 

@@ -246,6 +246,62 @@ typedef struct {
 int femhip_solver_error_norms(femhip_solver_t *solver, int32_t exact, femhip_error_norms_t *norms);
 int femhip_solver_destroy(femhip_solver_t *solver);
 
+/* ---- multi-field spaces: block layout, block assembler, block CG ----
+ * A serial_fe_space_t with several fields (sbm_serial_fe_space.i90:40-140)
+ * given as one femhip space per field on the same mesh (each numbered as if
+ * alone), its block_layout_t (Integration/FEOperator/block_layout.f90:66-117:
+ * field -> block, field coupling) and the block_sparse_assembler_t
+ * (Integration/FEOperator/block_sparse_assembler.f90:47-224) filling a
+ * block_sparse_matrix_t and a serial_block_array_t.  Free ids of a field are
+ * its own ids shifted by the free DOFs of the earlier fields of its block
+ * (sbm_serial_fe_space.i90:2676-2682); fixed ids follow those of all earlier
+ * fields (:584-640).  Scalar fields, homogeneous Dirichlet per field. */
+#define FEMHIP_MAX_FIELDS 4
+typedef struct femhip_block_operator femhip_block_operator_t;
+typedef struct {
+  int32_t num_fields;
+  femhip_space_t *fields[FEMHIP_MAX_FIELDS];   /* scalar spaces on one mesh (retained) */
+  int32_t field_blocks[FEMHIP_MAX_FIELDS];     /* 1-based block of each field (FES_FIELD_BLOCKS) */
+  int32_t field_coupling[FEMHIP_MAX_FIELDS * FEMHIP_MAX_FIELDS]; /* [f * num_fields + g] != 0: rows of
+                                                   field f take columns of field g; [f][f] must be set */
+} femhip_block_layout_t;
+/* The coupled diffusion-reaction plugin: sum_f kappa_f (grad u_f, grad v_f) +
+ * sigma sum_{f<g} (u_f - u_g, v_f - v_g), source f_f per field; every field
+ * integrated with the quadrature of the highest order (sbm_serial_fe_space.i90:1891-2006). */
+typedef struct {
+  double kappa[FEMHIP_MAX_FIELDS];
+  double sigma;
+  double f[FEMHIP_MAX_FIELDS];
+} femhip_multifield_physics_t;
+/* fe_affine_operator_t%create with a block layout: the pattern of every
+ * coupled block, built on the device; diagonal_blocks_symmetric_storage is the
+ * view the host gets of the diagonal blocks (upper triangle). */
+int femhip_block_operator_create(femhip_ctx_t *ctx, const femhip_block_layout_t *layout,
+                                 int32_t diagonal_blocks_symmetric_storage, femhip_block_operator_t **bop);
+/* number of blocks, rows per block (num_blocks values), stored entries per
+ * block pair (num_blocks^2, row-major; 0 for an uncoupled pair) */
+int femhip_block_operator_info(const femhip_block_operator_t *bop, int32_t *num_blocks, int32_t *block_sizes,
+                               int64_t *nnz);
+/* lst_dofs_gids of field `field` (1-based) in the block numbering, cells x nd */
+int femhip_block_operator_get_dofs(const femhip_block_operator_t *bop, int32_t field, int32_t *lst_dofs_gids);
+/* block_sparse_matrix_t%get_block(ib, jb) -> get_irp/get_ja (1-based) */
+int femhip_block_operator_get_block_pattern(const femhip_block_operator_t *bop, int32_t ib, int32_t jb,
+                                            int32_t *irp, int32_t *ja);
+int femhip_block_operator_set_physics(femhip_block_operator_t *bop, const femhip_multifield_physics_t *phys);
+/* fe_affine_operator_t%compute: integrate every cell, assemble every coupled block and the block RHS */
+int femhip_block_operator_compute(femhip_block_operator_t *bop, double *ms);
+int femhip_block_operator_get_block_values(const femhip_block_operator_t *bop, int32_t ib, int32_t jb, double *val);
+/* serial_block_array_t%get_block(ib) of the translation */
+int femhip_block_operator_get_rhs(const femhip_block_operator_t *bop, int32_t ib, double *rhs);
+/* block_sparse_matrix_t%apply: y = A x, vectors of all blocks' rows in block order (host) */
+int femhip_block_operator_apply(femhip_block_operator_t *bop, const double *x, double *y);
+/* cg_solve_body (cg.f90:101-238) on the block operator (FP64, RES_RES or
+ * RES_RHS, identity or Jacobi on the diagonal blocks); b == NULL: the
+ * assembled RHS; x: x0 in, solution out (host, block order) */
+int femhip_block_operator_solve(femhip_block_operator_t *bop, const femhip_cg_params_t *params, const double *b,
+                                double *x, femhip_cg_result_t *res);
+int femhip_block_operator_destroy(femhip_block_operator_t *bop);
+
 /* ---- batched form: many independent small problems in one persistent launch ----
  * Each problem is FEMPAR's test_poisson setting on its own mesh: Q1 on the unit
  * square/cube with n cells per direction, strong Dirichlet (g = 0) on the whole

@@ -1162,3 +1162,76 @@ void orc_mf_assemble_rhs(const orc_mf_t *m, const orc_mf_physics_t *ph, int ib, 
   free(elmat);
   free(elvec);
 }
+
+/* ------------------------------------------ all-cores timing baseline --- */
+
+/* Jacobi-PCG with the RES_RES criterion on a full-storage CSR matrix, its
+   loops spread over the host's threads (OpenMP).  Not a restatement of the
+   reference (whose serial path has no threading and applies symmetric
+   storage): the same recurrence as orc_cg, used only to time an all-cores CPU
+   figure beside the one-core oracle. */
+void orc_cg_threads(int n, const int32_t *irp, const int32_t *ja, const double *val, double rtol, int maxit,
+                    const double *b, double *x, orc_cg_result_t *res) {
+  double *r = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *z = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *p = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *q = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *inv = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double rz = 0.0, rr = 0.0;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    double d = 0.0, acc = 0.0;
+    for (int32_t k = irp[i] - 1; k < irp[i + 1] - 1; ++k) {
+      if (ja[k] - 1 == i) d = val[k];
+      acc += val[k] * x[ja[k] - 1];
+    }
+    inv[i] = 1.0 / d;
+    r[i] = b[i] - acc;
+  }
+#pragma omp parallel for schedule(static) reduction(+ : rz, rr)
+  for (int i = 0; i < n; ++i) {
+    z[i] = inv[i] * r[i];
+    p[i] = z[i];
+    rz += r[i] * z[i];
+    rr += r[i] * r[i];
+  }
+  const double thr = rtol * sqrt(rr);
+  int it = 0, conv = 0;
+  double est = sqrt(rr);
+  while (it < maxit) {
+    ++it;
+    double pq = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : pq)
+    for (int i = 0; i < n; ++i) {
+      double acc = 0.0;
+      for (int32_t k = irp[i] - 1; k < irp[i + 1] - 1; ++k) acc += val[k] * p[ja[k] - 1];
+      q[i] = acc;
+      pq += acc * p[i];
+    }
+    const double alpha = pq != 0.0 ? rz / pq : 0.0;
+    double rr2 = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rr2)
+    for (int i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+      rr2 += r[i] * r[i];
+    }
+    est = sqrt(rr2);
+    if (est <= thr) { conv = 1; break; }
+    double rz2 = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rz2)
+    for (int i = 0; i < n; ++i) {
+      z[i] = inv[i] * r[i];
+      rz2 += r[i] * z[i];
+    }
+    const double beta = rz2 / rz;
+    rz = rz2;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+  }
+  res->iterations = it;
+  res->converged = conv;
+  res->error_estimate = est;
+  res->rhs_test = thr;
+  free(r); free(z); free(p); free(q); free(inv);
+}

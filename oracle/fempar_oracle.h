@@ -162,6 +162,11 @@ int64_t orc_mf_assemble_block(const orc_mf_t *m, const orc_mf_physics_t *ph, con
                               int jb, int symmetric_storage, int32_t **irp, int32_t **ja, double **val);
 void orc_mf_assemble_rhs(const orc_mf_t *m, const orc_mf_physics_t *ph, int ib, double *rhs);
 
+/* Jacobi-PCG, RES_RES, full-storage CSR, loops over the host threads (OpenMP):
+   the all-cores timing baseline only (not a restatement of the serial reference). */
+void orc_cg_threads(int n, const int32_t *irp, const int32_t *ja, const double *val, double rtol, int maxit,
+                    const double *b, double *x, orc_cg_result_t *res);
+
 #ifdef __cplusplus
 }
 #endif

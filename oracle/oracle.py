@@ -86,6 +86,7 @@ def lib():
         L.orc_cg.argtypes = [C.c_int, P, P, P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                              C.c_int, P, P, C.POINTER(CgResult), P]
         L.orc_error_norms.argtypes = [P, P, P, C.c_int, P]
+        L.orc_cg_threads.argtypes = [C.c_int, P, P, P, C.c_double, C.c_int, P, P, C.POINTER(CgResult)]
         L.orc_mf_create.restype = P
         L.orc_mf_create.argtypes = [C.c_int, P, P]
         L.orc_mf_free.argtypes = [P]
@@ -341,3 +342,14 @@ class MultiFieldSpace:
         val = np.concatenate([v for _, v in rows])
         rhs = np.concatenate([self.assemble_rhs(phys, ib) for ib in range(1, self.num_blocks + 1)])
         return irp, ja, val, rhs
+
+
+def cg_threads(irp, ja, val, b, rtol=1e-10, maxit=5000):
+    """Jacobi-PCG (RES_RES) on a full-storage CSR over the host threads: the
+    all-cores timing baseline (not the serial reference's algorithm)."""
+    n = len(irp) - 1
+    x = np.zeros(n)
+    res = CgResult()
+    lib().orc_cg_threads(n, _ptr(irp), _ptr(ja), _ptr(val), rtol, maxit,
+                         _ptr(np.ascontiguousarray(b, dtype=np.float64)), _ptr(x), C.byref(res))
+    return x, {"iterations": res.iterations, "converged": bool(res.converged)}
