@@ -1532,13 +1532,16 @@ struct Launcher {
   }
 };
 
-// FP32-refined iteration slots as the three kernels of the FP64 form
-// (k_ru_spmv, k_ru_update, k_ru_dir; default) or as two (k_rf_a, k_rf_b;
-// FEMHIP_RF2=1).  The two-kernel form was measured no faster: 128^3 Q1 CG
-// 17.34 against 17.48 ms on the same iterations, 64^3 Q2 95.6 ms against 85.9.
-bool three_kernel_refined() {
+// FP32-refined iteration slots as two kernels (k_rf_a, k_rf_b) on the
+// symmetric-diagonal layout, as the three kernels of the FP64 form (k_ru_spmv,
+// k_ru_update, k_ru_dir) on the sliced layouts; FEMHIP_RF2=1 / 0 forces two /
+// three.  Measured: 128^3 Q1 CG 13.47 ms in two kernels against 13.65 ms in
+// three (261 and 263 inner iterations); 64^3 Q2 (class-blocked slices) 95.6 ms
+// against 85.9 ms, and slower serving solves, in two.
+bool three_kernel_refined(const femhip_solver &s) {
   static const char *env = getenv("FEMHIP_RF2");
-  return !(env && atoi(env) == 1);
+  if (env && (atoi(env) == 0 || atoi(env) == 1)) return atoi(env) == 0;
+  return !s.op->sl.sym;
 }
 
 // Enqueue one iteration of parity `par`: SpMV, update, direction.  With
@@ -1557,7 +1560,7 @@ void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = 
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
     L(k_update<double, double>, GV, B, n, s.r.p, s.q.p, invd, s.st.p, par, s.part_s.p, s.g_mat, s.part_u.p);
     L(k_dir<double>, GV, B, n, s.x.p, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.hist.p, s.res.p, rh);
-  } else if (!three_kernel_refined()) {
+  } else if (!three_kernel_refined(s)) {
     const double *sc = jac ? s.invd.p : nullptr;   // S = diag^(-1/2)
     const SlicedDev m32 = fp32_view(s, m);
     L(k_rf_a, GM, B, m, m32, op.sl.v32.p, s.r32.p, s.p32.p, s.q32.p, s.d32.p, s.x.p, sc, s.st.p, par, s.part_u.p,
