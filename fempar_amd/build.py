@@ -66,15 +66,25 @@ FORTRAN_DIR = os.path.join(HERE, "fortran")
 FORTRAN_EXE = os.path.join(HERE, "test_poisson_hip")
 
 
+SEAMS_EXE = os.path.join(HERE, "test_poisson_seams")
+
+
 def build_fortran() -> str:
-    """Compile the ISO_C_BINDING module and the test_poisson driver with AMD flang."""
+    """Compile the ISO_C_BINDING module, the FEMPAR plug-in seams with their
+    HIP subtypes, and the two test_poisson drivers with AMD flang."""
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     mod = os.path.join(OBJ, "fortran")
     os.makedirs(mod, exist_ok=True)
     obj = os.path.join(mod, "femhip_names.o")
+    seams = os.path.join(mod, "fempar_seams.o")
+    plugins = os.path.join(mod, "hip_fempar_plugins.o")
+    link = ["-J", mod, "-L" + HERE, "-lfemhip", "-Wl,-rpath,$ORIGIN"]
     for cmd in ([fc, "-O2", "-c", os.path.join(FORTRAN_DIR, "femhip_names.f90"), "-J", mod, "-o", obj],
-                [fc, "-O2", os.path.join(FORTRAN_DIR, "test_poisson_hip.f90"), obj, "-J", mod, "-L" + HERE,
-                 "-lfemhip", "-Wl,-rpath,$ORIGIN", "-o", FORTRAN_EXE]):
+                [fc, "-O2", os.path.join(FORTRAN_DIR, "test_poisson_hip.f90"), obj, *link, "-o", FORTRAN_EXE],
+                [fc, "-O2", "-c", os.path.join(FORTRAN_DIR, "fempar_seams.f90"), "-J", mod, "-o", seams],
+                [fc, "-O2", "-c", os.path.join(FORTRAN_DIR, "hip_fempar_plugins.f90"), "-J", mod, "-o", plugins],
+                [fc, "-O2", os.path.join(FORTRAN_DIR, "test_poisson_seams.f90"), obj, seams, plugins, *link,
+                 "-o", SEAMS_EXE]):
         p = subprocess.run(cmd, capture_output=True, text=True)
         if p.returncode != 0:
             raise RuntimeError("Fortran build failed: " + " ".join(cmd) + "\n" + p.stdout + p.stderr)
