@@ -223,6 +223,55 @@ def run_batched(F, env, steps, rtol=1e-8):
     return out
 
 
+def run_multifield(F, env, n=48, rtol=1e-10):
+    """Two coupled fields on the unit cube, n^3 cells: u in Q2 and v in Q1
+    (the shapes of a velocity-pressure pair), each fixed on the boundary, in
+    two blocks; kappa = (1, 1), sigma = 1, f = (1, 1).  Block assembly on the
+    device, block Jacobi-PCG (RES_RES, rtol) over both blocks."""
+    lim = [0, 1, 0, 1, 0, 1]
+    tria = F.serial_triangulation_t().create(env, {
+        "STRUCT_HEX_MESH_GENERATOR_NUM_DIMS": 3, "STRUCT_HEX_MESH_GENERATOR_NUM_CELLS_X_DIM": [n, n, n],
+        "STRUCT_HEX_MESH_GENERATOR_DOMAIN_LIMITS": lim})
+    t0 = time.perf_counter()
+    spaces = [F.serial_fe_space_t().create(tria, order=2), F.serial_fe_space_t().create(tria, order=1)]
+    layout = F.block_layout_t().create(2, [1, 2])
+    op = F.block_fe_affine_operator_t()
+    op.create("CSR", [False], spaces, layout, F.coupled_diffusion_discrete_integration_t((1.0, 1.0), 1.0, (1.0, 1.0)))
+    setup_s = time.perf_counter() - t0
+    solver = F.iterative_linear_solver_t().create(env)
+    solver.set_type_from_string("CG")
+    solver.set_parameters_from_pl({"ILS_RTOL": rtol, "ILS_MAX_NUM_ITERATIONS": 20000})
+    solver.set_operators(op, F.jacobi_preconditioner_t(op))
+    op.compute()
+    solver.solve()                      # warm-up
+    asm, sol = [], []
+    for _ in range(2):
+        op.compute()
+        asm.append(op.last_compute_ms)
+        solver.solve()
+        sol.append(solver.result["solve_ms"])
+    r = solver.result
+    nnz = int(np.sum(op.block_nnz))
+    ndofs = op.num_rows
+    it = r["num_iterations"]
+    # per iteration: every block's CSR (8 B value + 4 B column per entry, row
+    # pointers) once in the SpMV, and about 13 vector passes of 8 B per row
+    per_it = 12.0 * nnz + 4.0 * (ndofs + 2) + 104.0 * ndofs
+    out = {"workload": f"two fields on {n}^3 cells: Q2 + Q1, two blocks", "dofs": ndofs,
+           "block_sizes": op.block_sizes, "nnz": nnz, "setup_s": setup_s,
+           "t_assemble_ms": float(np.mean(asm)), "t_cg_ms": float(np.mean(sol)), "iterations": it,
+           "true_rel_residual": r["true_residual"] / r["rhs_norm"],
+           "value": ndofs / ((np.mean(asm) + np.mean(sol)) / 1e3), "unit": "DOFs/s",
+           "ms_per_iteration": float(np.mean(sol)) / max(1, it),
+           "cg_GBs": per_it * it / (np.mean(sol) / 1e3) / 1e9}
+    out["cg_frac_of_hbm_peak"] = out["cg_GBs"] / HBM_PEAK_GBS
+    solver.free()
+    op.free()
+    for sp in spaces:
+        sp.free()
+    return out
+
+
 SERVE_RATES = (10, 20, 30, 100, 200, 500, 1000, 2000, 5000)   # below and above the ~30 solves/s capacity
 SERVE_POLICY_RATES = (30, 1000)                                   # also served smallest-first
 
@@ -293,6 +342,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-multifield", action="store_true")
+    ap.add_argument("--multifield-only", action="store_true", help="only the two-field block configuration")
     ap.add_argument("--no-q2", action="store_true", help="skip the 64^3 Q2 configuration")
     ap.add_argument("--no-grf", action="store_true", help="skip the 192^3 random-field mixed-precision configuration")
     ap.add_argument("--configs-only", action="store_true", help="only the Q2 and random-field configurations")
@@ -365,6 +416,9 @@ def main():
         os.environ.pop("FEMHIP_ASSEMBLY_MFMA")
         print(json.dumps(res))
         return
+    if args.multifield_only:
+        print(json.dumps(run_multifield(F, env)))
+        return
     if args.spmv_only:
         tria, fes, di, op = make_problem(F, env, n, order=args.order)
         op.compute()
@@ -423,6 +477,9 @@ def main():
     batched = None if args.no_batched else run_batched(F, env, args.steps)
     serving = None if args.no_serving else run_serving(F, env, args.serve_requests)
     configs = other_configs()
+    if not args.no_multifield and not args.configs_only:
+        configs["multifield_q2q1_48"] = run_multifield(F, env)
+        print(f"[bench] multifield: {configs['multifield_q2q1_48']}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -404,36 +404,34 @@ __global__ __launch_bounds__(64 * kQ2Rows) void k_gather_q2(FaceTab t, int32_t n
   const int32_t r = blockIdx.x * kQ2Rows + w;
   if (r >= n) return;   // no block-wide barrier below
   const int32_t b0 = rowptr[r], len = rowptr[r + 1] - b0;
-  for (int i = lane; i < kOwnMaxRow; i += 64) {
-    scol[i] = i < len ? col[b0 + i] : INT_MAX;
+  for (int i = lane; i < len; i += 64) {
+    scol[i] = col[b0 + i];
     for (int c = 0; c < kCells; ++c) slist[i][c] = -1;
   }
-  if (lane == 0) {   // the cells around the row's vef, ascending, and the row's DOF in each
-    int nc = 0;
-    {
-      const int64_t c0 = rep[r];
-      const int l0 = (int)(c0 % kQ2Nd), f = t.face_of[l0], pos = t.pos_of[l0];
-      int ijk[3];
-      cell_ijk(t, c0 / kQ2Nd, ijk);
-      int ax[3] = {0, 0, 0}, na = 0;
-      for (int k = 0; k < 3; ++k)
-        if (!((t.type[f] >> k) & 1)) ax[na++] = k;
-      for (int b = 0; b < (1 << na); ++b) {
-        int cj[3] = {ijk[0], ijk[1], ijk[2]}, anc = 0;
-        bool ok = true;
-        for (int e = 0; e < na; ++e) {
-          const int k = ax[e], wk = ijk[k] + ((t.anchor[f] >> k) & 1);
-          cj[k] = wk - 1 + ((b >> e) & 1);
-          ok = ok && cj[k] >= 0 && cj[k] < t.n[k];
-          anc |= (wk - cj[k]) << k;
-        }
-        if (!ok) continue;
-        scell[nc] = cell_index(t, cj);
-        slc[nc] = t.lst[t.ptr[t.face_by[t.type[f]][anc]] + pos];
-        ++nc;
-      }
+  {   // the cells around the row's vef, ascending (lane b tries cell b), and the row's DOF in each
+    const int64_t c0 = rep[r];
+    const int l0 = (int)(c0 % kQ2Nd), f = t.face_of[l0], pos = t.pos_of[l0];
+    int ijk[3];
+    cell_ijk(t, c0 / kQ2Nd, ijk);
+    int ax[3] = {0, 0, 0}, na = 0;
+    for (int k = 0; k < 3; ++k)
+      if (!((t.type[f] >> k) & 1)) ax[na++] = k;
+    const int b = lane;
+    int cj[3] = {ijk[0], ijk[1], ijk[2]}, anc = 0;
+    bool ok = b < (1 << na);
+    for (int e = 0; e < na; ++e) {
+      const int k = ax[e], wk = ijk[k] + ((t.anchor[f] >> k) & 1);
+      cj[k] = wk - 1 + ((b >> e) & 1);
+      ok = ok && cj[k] >= 0 && cj[k] < t.n[k];
+      anc |= (wk - cj[k]) << k;
     }
-    snc_[w] = nc;
+    const uint64_t mask = __ballot(ok);
+    if (ok) {
+      const int at = __popcll(mask & ((1ull << lane) - 1));   // ascending b = ascending cell index
+      scell[at] = cell_index(t, cj);
+      slc[at] = t.lst[t.ptr[t.face_by[t.type[f]][anc]] + pos];
+    }
+    if (lane == 0) snc_[w] = __popcll(mask);
   }
   wave_sync();
   const int nc = snc_[w];

@@ -248,17 +248,24 @@ __global__ void k_block_rhs(FaceTab t, int32_t nrows, int32_t row_off, const int
   rhs[row_off + fr] = acc;
 }
 
-// y = A x (add = 0) or y += A x, full-storage CSR, one thread per row; the
-// blocks of a block row are applied in block order, so every row's sum runs
-// over its block parts in order (a block_sparse_matrix_t apply).
-__global__ void k_block_spmv(int32_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                             const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
-                             int add) {
-  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  double acc = add ? y[r] : 0.0;
-  for (int32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) acc += x[col[k]] * val[k];
-  y[r] = acc;
+// y = A x (add = 0) or y += A x, full-storage CSR, kSpmvLanes lanes per row
+// (rows of 27 to 125 entries): the lanes stride over the row's entries, so
+// the value and column loads of a row are contiguous across the lanes, and a
+// fixed shuffle tree sums them; the blocks of a block row are applied in block
+// order (a block_sparse_matrix_t apply).
+constexpr int kSpmvLanes = 16, kSpmvThreads = 256;
+__global__ __launch_bounds__(kSpmvThreads) void k_block_spmv(int32_t n, const int32_t *__restrict__ rowptr,
+                                                             const int32_t *__restrict__ col,
+                                                             const double *__restrict__ val,
+                                                             const double *__restrict__ x, double *__restrict__ y,
+                                                             int add) {
+  const int sub = threadIdx.x % kSpmvLanes;
+  const int64_t r = ((int64_t)blockIdx.x * kSpmvThreads + threadIdx.x) / kSpmvLanes;
+  double acc = 0.0;
+  if (r < n)
+    for (int32_t k = rowptr[r] + sub; k < rowptr[r + 1]; k += kSpmvLanes) acc += val[k] * x[col[k]];
+  for (int o = kSpmvLanes / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, kSpmvLanes);
+  if (r < n && sub == 0) y[r] = add ? y[r] + acc : acc;
 }
 
 // dot products in a fixed order: kDotBlocks partial sums (grid-strided rows,
@@ -287,12 +294,18 @@ __global__ __launch_bounds__(kDotThreads) void k_dot_final(const double *__restr
   if (threadIdx.x == 0) *out = s;
 }
 
-// x += alpha p and r -= alpha q (alpha from the device scalars)
+// Device scalars of the block CG: kRz = <r,z> of the current direction,
+// kQp = <q,p>, kRzNew = the next <r,z>, kRr = ||r||^2 (the only one the host
+// reads each iteration, for the stopping test).
+enum { kRz = 0, kQp = 1, kRzNew = 2, kRr = 3, kDot = 4, kScalars = 5 };
+
+// alpha = <r,z> / <q,p> (0 when <q,p> = 0, cg.f90:195-199); x += alpha p, r -= alpha q
 __global__ void k_cg_xr(int64_t n, const double *__restrict__ sc, double *__restrict__ x,
                         const double *__restrict__ p, double *__restrict__ r, const double *__restrict__ q) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double alpha = sc[0];
+  const double qp = sc[kQp];
+  const double alpha = qp != 0.0 ? sc[kRz] / qp : 0.0;
   x[i] = x[i] + alpha * p[i];
   r[i] = r[i] + (-alpha) * q[i];
 }
@@ -303,12 +316,17 @@ __global__ void k_cg_z(int64_t n, const double *__restrict__ invd, const double 
   if (i >= n) return;
   z[i] = invd ? invd[i] * r[i] : r[i];
 }
-__global__ void k_cg_p(int64_t n, double beta, const double *__restrict__ z, double *__restrict__ p) {
+// beta = (1 / <r,z>_old) <r,z>_new (cg.f90's order); p = beta p + z
+__global__ void k_cg_p(int64_t n, const double *__restrict__ sc, const double *__restrict__ z,
+                       double *__restrict__ p) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const double beta = (1.0 / sc[kRz]) * sc[kRzNew];
   double v = beta * p[i];
   p[i] = v + 1.0 * z[i];
 }
+// the new <r,z> becomes the current one (after every k_cg_p block read both)
+__global__ void k_cg_shift(double *__restrict__ sc) { sc[kRz] = sc[kRzNew]; }
 // b - A x0 into r
 __global__ void k_sub(int64_t n, const double *__restrict__ b, double *__restrict__ r) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -514,8 +532,8 @@ void block_apply(femhip_block_operator &B, const double *x, double *y) {
     for (int jb = 0; jb < B.nb; ++jb) {
       auto &K = B.blk[ib][jb];
       if (!K.on) continue;
-      hipLaunchKernelGGL(k_block_spmv, dim3(nblocks(n, 256)), dim3(256), 0, st, n, K.rowptr.p, K.col.p, K.val.p,
-                         x + B.boff[jb], y + B.boff[ib], first ? 0 : 1);
+      hipLaunchKernelGGL(k_block_spmv, dim3(nblocks((int64_t)n * kSpmvLanes, kSpmvThreads)), dim3(kSpmvThreads), 0,
+                         st, n, K.rowptr.p, K.col.p, K.val.p, x + B.boff[jb], y + B.boff[ib], first ? 0 : 1);
       FH_HIP(hipGetLastError());
       first = 0;
     }
@@ -523,15 +541,24 @@ void block_apply(femhip_block_operator &B, const double *x, double *y) {
   }
 }
 
-double block_dot(femhip_block_operator &B, const double *a, const double *b) {
+// <a, b> into the device scalar `slot` (no host wait)
+void block_dot_dev(femhip_block_operator &B, const double *a, const double *b, int slot) {
   hipStream_t st = B.ctx->stream;
   hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(kDotThreads), 0, st, B.ntot, a, b, B.part.p);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, st, B.part.p, B.scal.p + 1);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, st, B.part.p, B.scal.p + slot);
   FH_HIP(hipGetLastError());
+}
+
+double read_scalar(femhip_block_operator &B, int slot) {
   double h = 0.0;
-  FH_HIP(hipMemcpyAsync(&h, B.scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, st));
-  FH_HIP(hipStreamSynchronize(st));
+  FH_HIP(hipMemcpyAsync(&h, B.scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, B.ctx->stream));
+  FH_HIP(hipStreamSynchronize(B.ctx->stream));
   return h;
+}
+
+double block_dot(femhip_block_operator &B, const double *a, const double *b) {
+  block_dot_dev(B, a, b, kDot);
+  return read_scalar(B, kDot);
 }
 
 // cg_solve_body (cg.f90:101-238) on the block operator with the RES_RES or
@@ -568,7 +595,7 @@ void block_cg(femhip_block_operator &B, const femhip_cg_params_t &prm, const dou
   hipLaunchKernelGGL(k_sub, G, T, 0, st, n, b, B.r.p);
   hipLaunchKernelGGL(k_cg_z, G, T, 0, st, n, invd, B.r.p, B.z.p);
   FH_HIP(hipGetLastError());
-  double r_z = block_dot(B, B.r.p, B.z.p);
+  block_dot_dev(B, B.r.p, B.z.p, kRz);
   FH_HIP(hipMemcpyAsync(B.p.p, B.z.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
   const double rhs_test =
       prm.stopping_criterion == FEMHIP_RES_RHS ? prm.rtol * std::sqrt(block_dot(B, b, b)) + prm.atol
@@ -576,23 +603,20 @@ void block_cg(femhip_block_operator &B, const femhip_cg_params_t &prm, const dou
   int it = 0;
   bool conv = false;
   double est = 0.0;
-  for (;;) {
+  for (;;) {   // alpha and beta stay on the device; the host reads ||r|| once per iteration
     ++it;
     block_apply(B, B.p.p, B.q.p);
-    const double qp = block_dot(B, B.q.p, B.p.p);
-    const double alpha = qp != 0.0 ? r_z / qp : 0.0;
-    FH_HIP(hipMemcpyAsync(B.scal.p, &alpha, sizeof(double), hipMemcpyHostToDevice, st));
+    block_dot_dev(B, B.q.p, B.p.p, kQp);
     hipLaunchKernelGGL(k_cg_xr, G, T, 0, st, n, B.scal.p, B.x.p, B.p.p, B.r.p, B.q.p);
     FH_HIP(hipGetLastError());
-    est = std::sqrt(block_dot(B, B.r.p, B.r.p));
+    block_dot_dev(B, B.r.p, B.r.p, kRr);
+    est = std::sqrt(read_scalar(B, kRr));
     conv = est <= rhs_test;
     if (conv || it >= prm.max_num_iterations) break;
     hipLaunchKernelGGL(k_cg_z, G, T, 0, st, n, invd, B.r.p, B.z.p);
-    FH_HIP(hipGetLastError());
-    double beta = 1.0 / r_z;
-    r_z = block_dot(B, B.r.p, B.z.p);
-    beta = beta * r_z;
-    hipLaunchKernelGGL(k_cg_p, G, T, 0, st, n, beta, B.z.p, B.p.p);
+    block_dot_dev(B, B.r.p, B.z.p, kRzNew);
+    hipLaunchKernelGGL(k_cg_p, G, T, 0, st, n, B.scal.p, B.z.p, B.p.p);
+    hipLaunchKernelGGL(k_cg_shift, dim3(1), dim3(1), 0, st, B.scal.p);
     FH_HIP(hipGetLastError());
   }
   FH_HIP(hipEventRecord(e1, st));
@@ -692,7 +716,7 @@ int femhip_block_operator_create(femhip_ctx_t *ctx, const femhip_block_layout_t 
       B->q.alloc(nt);
       B->invd.alloc(nt);
       B->part.alloc(kDotBlocks);
-      B->scal.alloc(4);
+      B->scal.alloc(kScalars);
     } catch (...) {
       delete B;
       throw;
