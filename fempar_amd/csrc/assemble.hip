@@ -459,12 +459,12 @@ __global__ __launch_bounds__(64 * kQ2Rows) void k_gather_q2(FaceTab t, int32_t n
     const int32_t gj = gv[k];
     double lf = 0.0;
     if (gj >= 0) {
-      int lo = 0, hi = len - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (scol[mid] < gj) lo = mid + 1;
-        else hi = mid;
-      }
+      // first slot whose column is not below gj, in a fixed 7 steps (len <=
+      // 128), so the lane's independent searches overlap
+      int lo = 0;   // = the number of the row's columns below gj
+#pragma unroll
+      for (int h = kOwnMaxRow / 2; h > 0; h >>= 1)
+        if (lo + h <= len && scol[lo + h - 1] < gj) lo += h;
       slist[lo][ci] = (int16_t)p;   // a cell's columns are distinct: one pair per (slot, cell)
       contrib[p] = e;
     } else if (fixed) {
