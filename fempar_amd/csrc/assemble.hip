@@ -257,12 +257,10 @@ __global__ __launch_bounds__(64) void k_assemble_owner(PhysDev ph, TrigTab tt, F
       const int32_t gj = dofs[cell * ND + lane];
       double fx = 0.0;
       if (gj >= 0) {
-        int lo = 0, hi = len - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (scol[mid] < gj) lo = mid + 1;
-          else hi = mid;
-        }
+        int lo = 0;   // = the number of the row's columns below gj (fixed steps, len <= 128)
+#pragma unroll
+        for (int h = kOwnMaxRow / 2; h > 0; h >>= 1)
+          if (lo + h <= len && scol[lo + h - 1] < gj) lo += h;
         sacc[lo] += e;   // the cell's columns are distinct: one lane per slot
       } else if (fixed) {
         fx = e * fixed[-gj - 1];

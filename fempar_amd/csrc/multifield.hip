@@ -216,12 +216,10 @@ __global__ __launch_bounds__(64) void k_block_assemble(FaceTab t, int32_t nrows,
         const int32_t d = cs.dofs[g][cell * cs.nd[g] + lane];
         if (d >= 0) {
           const int32_t c = d + cs.off[g];
-          int lo = 0, hi = len - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (sc[mid] < c) lo = mid + 1;
-            else hi = mid;
-          }
+          int lo = 0;   // = the number of the row's columns below c (fixed steps, len <= kCand)
+#pragma unroll
+          for (int h = kCand / 2; h > 0; h >>= 1)
+            if (lo + h <= len && sc[lo + h - 1] < c) lo += h;
           sv[lo] += tab[cs.tab[g] + l * cs.nd[g] + lane];
         }
       }
