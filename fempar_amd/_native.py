@@ -17,8 +17,8 @@ ERR_ZERO_DIAGONAL, ERR_STATE, ERR_ADMISSION, ERR_NO_DEVICE = 5, 6, 7, 8
 
 BC_ALL_BOUNDARY, BC_X0_FACE, BC_NONE = 0, 1, 2
 PHYS_POISSON, PHYS_ELASTICITY = 0, 1
-K_CONST, K_SIN3, K_CELL = 0, 1, 2
-F_CONST, F_MANUFACTURED, F_CELL_SIN = 0, 1, 2
+K_CONST, K_SIN3, K_CELL, K_QPOINT = 0, 1, 2, 3
+F_CONST, F_MANUFACTURED, F_CELL_SIN, F_QPOINT = 0, 1, 2, 3
 G_ZERO, G_LINEAR = 0, 1
 PREC_IDENTITY, PREC_JACOBI = 0, 1
 PRECISION_FP64, PRECISION_FP32_REFINED = 0, 1
@@ -32,6 +32,7 @@ EXPORTS = [
     "femhip_ctx_create", "femhip_ctx_destroy", "femhip_last_error",
     "femhip_space_create_structured", "femhip_space_create_from_dofs", "femhip_space_info",
     "femhip_space_get_dofs", "femhip_space_interpolate_dirichlet", "femhip_space_destroy",
+    "femhip_space_set_vertex_coords", "femhip_space_quadrature_points",
     "femhip_operator_create", "femhip_operator_info", "femhip_operator_get_pattern",
     "femhip_operator_set_physics", "femhip_operator_compute", "femhip_operator_get_values",
     "femhip_operator_apply", "femhip_operator_apply_noalloc", "femhip_operator_destroy",
@@ -85,7 +86,8 @@ class MeshDesc(C.Structure):
 class Physics(C.Structure):
     _fields_ = [("kind", C.c_int32), ("k_kind", C.c_int32), ("k0", C.c_double),
                 ("k_cell", C.POINTER(C.c_double)), ("f_kind", C.c_int32), ("f0", C.c_double),
-                ("g_kind", C.c_int32), ("E", C.c_double), ("nu", C.c_double), ("body", C.c_double * 3)]
+                ("g_kind", C.c_int32), ("E", C.c_double), ("nu", C.c_double), ("body", C.c_double * 3),
+                ("k_qp", C.POINTER(C.c_double)), ("f_qp", C.POINTER(C.c_double))]
 
 
 class CgParams(C.Structure):
@@ -152,6 +154,7 @@ def load():
         "femhip_space_info": [P, C.POINTER(I64), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)],
         "femhip_space_get_dofs": [P, P], "femhip_space_interpolate_dirichlet": [P, C.POINTER(Physics), P],
         "femhip_space_destroy": [P],
+        "femhip_space_set_vertex_coords": [P, P], "femhip_space_quadrature_points": [P, P],
         "femhip_operator_create": [P, P, I32, pP],
         "femhip_operator_info": [P, C.POINTER(I32), C.POINTER(I64), C.POINTER(I64)],
         "femhip_operator_get_pattern": [P, P, P], "femhip_operator_set_physics": [P, C.POINTER(Physics)],

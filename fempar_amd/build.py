@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfemhip.so")
-SOURCES = ["api.cpp", "space.cpp", "setup.hip", "sliced.cpp", "assemble.hip", "cg.hip", "batch.hip", "serve.cpp", "norms.hip", "stencil.hip", "noalloc.hip", "multifield.hip"]
+SOURCES = ["api.cpp", "space.cpp", "setup.hip", "sliced.cpp", "assemble.hip", "cg.hip", "batch.hip", "serve.cpp", "norms.hip", "stencil.hip", "noalloc.hip", "multifield.hip", "mapped.hip"]
 
 
 def _hipcc():

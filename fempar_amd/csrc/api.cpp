@@ -186,7 +186,27 @@ int femhip_space_interpolate_dirichlet(const femhip_space_t *s, const femhip_phy
   if (phys->g_kind == FEMHIP_G_ZERO) return FEMHIP_OK;
   FH_CHECK(phys->g_kind == FEMHIP_G_LINEAR, FEMHIP_ERR_INVALID, "unknown Dirichlet function");
   FH_HIP(hipSetDevice(s->ctx->device));
-  interpolate_dirichlet_linear(*s, out);   // g(x) = sum of the coordinates at each fixed node
+  // g(x) = sum of the coordinates at each fixed node
+  if (s->mapped) interpolate_dirichlet_mapped(*s, out);
+  else interpolate_dirichlet_linear(*s, out);
+  return FEMHIP_OK;
+  FH_CATCH
+}
+
+int femhip_space_set_vertex_coords(femhip_space_t *s, const double *coords) {
+  FH_TRY
+  FH_CHECK(s && coords, FEMHIP_ERR_INVALID, "null argument");
+  FH_HIP(hipSetDevice(s->ctx->device));
+  set_vertex_coords(*s, coords);
+  return FEMHIP_OK;
+  FH_CATCH
+}
+
+int femhip_space_quadrature_points(const femhip_space_t *s, double *xq) {
+  FH_TRY
+  FH_CHECK(s && xq, FEMHIP_ERR_INVALID, "null argument");
+  FH_HIP(hipSetDevice(s->ctx->device));
+  quadrature_points(*s, xq);
   return FEMHIP_OK;
   FH_CATCH
 }
@@ -262,7 +282,7 @@ int femhip_operator_set_physics(femhip_operator_t *op, const femhip_physics_t *p
   FH_CHECK(op && phys, FEMHIP_ERR_INVALID, "null argument");
   FH_CHECK(phys->kind == FEMHIP_PHYS_POISSON || phys->kind == FEMHIP_PHYS_ELASTICITY, FEMHIP_ERR_INVALID,
            "unknown physics");
-  FH_CHECK(phys->k_kind >= 0 && phys->k_kind <= 2 && phys->f_kind >= 0 && phys->f_kind <= 2, FEMHIP_ERR_INVALID,
+  FH_CHECK(phys->k_kind >= 0 && phys->k_kind <= 3 && phys->f_kind >= 0 && phys->f_kind <= 3, FEMHIP_ERR_INVALID,
            "unknown coefficient/source kind");
   if (phys->kind == FEMHIP_PHYS_ELASTICITY)
     FH_CHECK(phys->E > 0 && phys->nu > -1.0 && phys->nu < 0.5, FEMHIP_ERR_INVALID, "invalid E or nu");

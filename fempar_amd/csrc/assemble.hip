@@ -195,7 +195,10 @@ __device__ __forceinline__ void coef_tab(const PhysDev &ph, const TrigTab &tt, c
 // colours, no read-modify-write in HBM.
 // MODE 0: Poisson, entry = sum_q k(x_q) M_q[l][j], RHS sum_q f(x_q) Nw_q[l],
 // k and f from the per-direction sin/cos tables at the NQ1 1-D Gauss points;
-// MODE 1: one element matrix and vector for every cell (elasticity).
+// MODE 1: one element matrix and vector for every cell (elasticity);
+// MODE 2: each cell's own element matrix and vector, formed beforehand
+// (tab = cells x ND x ND row-major, evec = cells x ND; cells with their own map
+// or coefficients given at the quadrature points).
 constexpr int kOwnMaxRow = 128;
 template <int DIM, int ND, int NQ1, int MODE>
 __global__ __launch_bounds__(64) void k_assemble_owner(PhysDev ph, TrigTab tt, FaceTab t, int32_t n,
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(64) void k_assemble_owner(PhysDev ph, TrigTab tt, F
                                                        const int32_t *__restrict__ rowptr,
                                                        const int32_t *__restrict__ col, const double *__restrict__ tab,
                                                        const double *__restrict__ fixed, double *__restrict__ val,
-                                                       double *__restrict__ rhs) {
+                                                       double *__restrict__ rhs, const double *__restrict__ evec) {
   constexpr int NQ = MODE == 0 ? (DIM == 2 ? NQ1 * NQ1 : NQ1 * NQ1 * NQ1) : 1;
   __shared__ int32_t scol[kOwnMaxRow];
   __shared__ double sacc[kOwnMaxRow];
@@ -251,8 +254,10 @@ __global__ __launch_bounds__(64) void k_assemble_owner(PhysDev ph, TrigTab tt, F
       if (MODE == 0) {
         e = 0.0;
         for (int q = 0; q < NQ; ++q) e += sk[q] * M[(q * ND + lc) * ND + lane];
-      } else {
+      } else if (MODE == 1) {
         e = E[lc * ND + lane];
+      } else {
+        e = tab[(cell * ND + lc) * ND + lane];
       }
       const int32_t gj = dofs[cell * ND + lane];
       double fx = 0.0;
@@ -273,8 +278,10 @@ __global__ __launch_bounds__(64) void k_assemble_owner(PhysDev ph, TrigTab tt, F
       if (MODE == 0) {
         be = 0.0;
         for (int q = 0; q < NQ; ++q) be += sf[q] * Nw[q * ND + lc];
-      } else {
+      } else if (MODE == 1) {
         be = F[lc];
+      } else {
+        be = evec[cell * ND + lc];
       }
       for (int j = 0; j < ND; ++j) be -= sfix[j];
       bacc += be;
@@ -837,9 +844,35 @@ void launch_owner(femhip_operator &op, const PhysDev &ph, const double *d_fixed)
   FH_CHECK(row_max <= kOwnMaxRow, FEMHIP_ERR_INVALID, "row longer than the owner-computes kernel holds");
   if (MODE == 0 && op.trig.n == 0) build_trig(op);
   const TrigTab tt{reinterpret_cast<const double4 *>(op.trig.p), {op.trig_off[0], op.trig_off[1], op.trig_off[2]}};
+  const double *tab = MODE == 2 ? op.elmat.p : op.tab.p;
   hipLaunchKernelGGL((k_assemble_owner<DIM, ND, NQ1, MODE>), dim3((unsigned)op.n), dim3(64), 0, op.ctx->stream, ph,
-                     tt, face_tab(s), op.n, s.d_rep.p, s.d_dofs.p, op.rowptr.p, op.col.p, op.tab.p, d_fixed, op.val.p,
-                     op.rhs.p);
+                     tt, face_tab(s), op.n, s.d_rep.p, s.d_dofs.p, op.rowptr.p, op.col.p, tab, d_fixed, op.val.p,
+                     op.rhs.p, op.elvec.p);
+}
+
+// Cells with their own map, or coefficients at the quadrature points: every
+// cell's element matrix from its own Jacobians (mapped.hip), then the rows'
+// owners gather them in ascending cell order.
+void launch_general(femhip_operator &op, const PhysDev &ph, const double *d_fixed) {
+  femhip_space &s = *op.space;
+  const RefFE &fe = s.fe;
+  element_matrices_mapped(op);
+  if (op.n <= 0) return;
+  if (fe.dim == 3 && fe.nd == 27) {
+    hipLaunchKernelGGL(k_gather_q2, dim3((unsigned)((op.n + kQ2Rows - 1) / kQ2Rows)), dim3(64 * kQ2Rows), 0,
+                       op.ctx->stream, face_tab(s), op.n, s.d_rep.p, s.d_dofs.p, op.rowptr.p, op.col.p, op.elmat.p,
+                       op.elvec.p, d_fixed, op.val.p, op.rhs.p);
+  } else if (fe.dim == 2 && fe.nd == 4) {
+    launch_owner<2, 4, 1, 2>(op, ph, d_fixed);
+  } else if (fe.dim == 2 && fe.nd == 9) {
+    launch_owner<2, 9, 1, 2>(op, ph, d_fixed);
+  } else if (fe.dim == 3 && fe.nd == 8) {
+    launch_owner<3, 8, 1, 2>(op, ph, d_fixed);
+  } else if (fe.dim == 3 && fe.nd == 24) {
+    launch_owner<3, 24, 1, 2>(op, ph, d_fixed);
+  } else {
+    throw Error{FEMHIP_ERR_INVALID, "unsupported element for per-cell integration"};
+  }
 }
 
 VtxGrid vtx_grid(const femhip_mesh_desc_t &d) {
@@ -1003,7 +1036,8 @@ void setup_sym_gather(femhip_operator &op) {
 // the matrix cores (k_assemble_sym32_mfma); false where that kernel does not
 // apply (the copy is then derived from the FP64 one by the solver).
 bool assemble_sym32(femhip_operator &op, const double *sc, float *v32) {
-  if (op.gather != 1 || op.sym_gather != 1 || !op.sl.sym || op.tab_kind != 0 || op.n <= 0 || op.sl.kplus > 16)
+  if (op.gather != 1 || op.sym_gather != 1 || !op.sl.sym || op.tab_kind != 0 || op.n <= 0 || op.sl.kplus > 16 ||
+      general_integration(op))
     return false;
   femhip_space &s = *op.space;
   const int dim = s.fe.dim;
@@ -1124,6 +1158,7 @@ void upload_physics(femhip_operator &op) {
       for (int i = 0; i < nd; ++i) t[(size_t)(nq * nd * nd + q * nd + i)] = factor * fe.shp[i][q];
     }
     op.tab_kind = 0;
+    upload_qpoint_values(op);
     if (ph.k_kind == FEMHIP_K_CELL) {
       FH_CHECK(ph.k_cell != nullptr, FEMHIP_ERR_INVALID, "k_cell is NULL for FEMHIP_K_CELL");
       op.k_cell.alloc((size_t)op.space->ncells);
@@ -1152,7 +1187,8 @@ void assemble(femhip_operator &op, const double *fixed_host, float *ms) {
   }
   const RefFE &fe = s.fe;
   // the gather maps depend on the space only; whether they apply depends on the physics
-  const bool poisson_q1 = op.tab_kind == 0 && fe.order == 1 && fe.ncomp == 1 && fe.nd == (1 << fe.dim);
+  const bool general = general_integration(op);
+  const bool poisson_q1 = !general && op.tab_kind == 0 && fe.order == 1 && fe.ncomp == 1 && fe.nd == (1 << fe.dim);
   if (poisson_q1 && op.gather == 0) setup_gather(op);
   const bool gather = poisson_q1 && op.gather == 1;
   op.sym_written = 0;
@@ -1162,7 +1198,9 @@ void assemble(femhip_operator &op, const double *fixed_host, float *ms) {
   FH_HIP(hipEventRecord(e0, st));
   PhysDev ph{op.phys.kind, op.phys.k_kind, op.phys.f_kind, op.phys.k0, op.phys.f0, op.k_cell.p};
   // every value and RHS entry is written once by either path: no clearing pass
-  if (gather) {
+  if (general) {
+    launch_general(op, ph, d_fixed);
+  } else if (gather) {
     launch_gather(op, ph, d_fixed);
   } else if (op.tab_kind == 0) {
     if (fe.dim == 2 && fe.nd == 4) launch_owner<2, 4, 2, 0>(op, ph, d_fixed);

@@ -70,7 +70,9 @@ struct RefFE {
   int own_ptr[kMaxNf + 1], own_lst[kMaxNd]; // own DOFs per n-face (reference order)
   double qp[3][kMaxNq], qw[kMaxNq];
   double shp[kMaxNd][kMaxNq];               // scalar shape values
+  double dshp[kMaxNd][3][kMaxNq];           // reference gradients of the scalar shapes (cell [-1, 1]^d)
   double gshp[8][kMaxNq];                   // geometric Q1 shape values
+  double gdshp[8][3][kMaxNq];               // reference gradients of the geometric Q1 shapes
   double detj[kMaxNq];
   double grad[kMaxNd][3][kMaxNq];           // physical gradients of scalar shapes (first cell map)
 };
@@ -117,6 +119,10 @@ struct femhip_space {
   std::vector<int32_t> dofs;          // host copy (cells x nd), signed 1-based; made on request
   femhip::DevBuf<int32_t> d_dofs;     // device: free -> row (0-based), fixed -> -(fixed index)-1
   femhip::DevBuf<int64_t> d_rep;      // device: free row -> cell * nd + local DOF of one occurrence
+  // vertex coordinates (num_vertices x dim, x fastest) when the caller moved
+  // the vertices: every cell then has its own trilinear map
+  femhip::DevBuf<double> vcoords;
+  bool mapped = false;
 };
 
 namespace femhip {
@@ -204,6 +210,9 @@ struct femhip_operator {
   // 3-D Q2: every cell's element matrix (27 x 27, row-major) and vector,
   // formed on the matrix cores, then gathered by the rows' owners
   femhip::DevBuf<double> elmat, elvec;
+  // per-cell integration (cells with their own map, coefficients at the
+  // quadrature points): reference-FE table and the caller's point values
+  femhip::DevBuf<double> mtab, k_qp, f_qp;
   int trig_off[3] = {0, 0, 0};
   // the owner-computes path also writes the solver's symmetric-diagonal copy
   // when every neighbour code has one column offset over all rows: sym_kidx
@@ -282,6 +291,20 @@ void spmv_full(femhip_operator &op, const double *d_x, double *d_y);
 // by femhip_ctx_create with that device current
 void noalloc_setup(int device);
 
+// per-cell integration (mapped.hip): whether an operator's cells need it (the
+// space's vertices were moved, or k or f are given at the quadrature points);
+// the space's vertex coordinates; the physical quadrature points of every
+// cell; linear Dirichlet values at the mapped nodes; the caller's point values
+// copied to the device; every cell's element matrix and vector (op.elmat,
+// op.elvec); error norms with each cell's own map
+bool general_integration(const femhip_operator &op);
+void set_vertex_coords(femhip_space &s, const double *coords);
+void quadrature_points(const femhip_space &s, double *xq_host);
+void interpolate_dirichlet_mapped(const femhip_space &s, double *out_host);
+void upload_qpoint_values(femhip_operator &op);
+void element_matrices_mapped(femhip_operator &op);
+void error_norms_mapped(femhip_operator &op, const double *d_x, int exact, double out[4]);
+
 // element matrix (row-major) and body-force vector of the elasticity plugin
 void elasticity_element(const RefFE &fe, const femhip_physics_t &ph, double *Ke, double *Fe);
 
@@ -319,12 +342,13 @@ size_t solver_device_bytes(const femhip_solver &s);
 inline size_t operator_device_bytes(const femhip_operator &op) {
   size_t b = op.rowptr.bytes() + op.col.bytes() + op.val.bytes() + op.rhs.bytes() + op.val32.bytes() +
              op.k_cell.bytes() + op.tab.bytes() + op.fixed.bytes() + op.vtx_dof.bytes() +
-             op.row_vtx.bytes() + op.slot.bytes() + op.trig.bytes() + op.elmat.bytes() + op.elvec.bytes();
+             op.row_vtx.bytes() + op.slot.bytes() + op.trig.bytes() + op.elmat.bytes() + op.elvec.bytes() +
+             op.mtab.bytes() + op.k_qp.bytes() + op.f_qp.bytes();
   const Sliced &s = op.sl;
   b += s.perm.bytes() + s.soffs.bytes() + s.base.bytes() + s.meta.bytes() + s.srow.bytes() + s.aux.bytes() + s.offs.bytes() +
        s.ecol.bytes() +
        s.map.bytes() + s.v64.bytes() + s.v32.bytes();
-  if (op.space) b += op.space->d_dofs.bytes() + op.space->d_rep.bytes();
+  if (op.space) b += op.space->d_dofs.bytes() + op.space->d_rep.bytes() + op.space->vcoords.bytes();
   return b;
 }
 

@@ -130,6 +130,10 @@ void error_norms(femhip_operator &op, const double *d_x, int exact, double out[4
   const RefFE &fe = s.fe;
   FH_CHECK(fe.ncomp == 1, FEMHIP_ERR_INVALID, "error norms are implemented for scalar spaces");
   FH_CHECK(exact == FEMHIP_EXACT_SIN || exact == FEMHIP_EXACT_LINEAR, FEMHIP_ERR_INVALID, "unknown exact solution");
+  if (s.mapped) {   // every cell with its own map
+    error_norms_mapped(op, d_x, exact, out);
+    return;
+  }
   const int nd = fe.nd, nq = fe.nq, nv = 1 << fe.dim;
   std::vector<double> t((size_t)(nq * nd + nq * nd * 3 + nq * nv + nq), 0.0);
   for (int q = 0; q < nq; ++q) {

@@ -159,22 +159,29 @@ void build_reffe(RefFE &fe, int dim, int order, int ncomp, const double h[3], in
       double val = 1.0;
       for (int k = 0; k < dim; ++k) val *= v[k];
       fe.shp[a][q] = val;
-      for (int i = 0; i < 3; ++i) fe.grad[a][i][q] = 0.0;
+      for (int i = 0; i < 3; ++i) fe.grad[a][i][q] = fe.dshp[a][i][q] = 0.0;
       for (int i = 0; i < dim; ++i) {
         double g = dv[i];
         for (int j = 0; j < dim; ++j)
           if (j != i) g *= v[j];
+        fe.dshp[a][i][q] = g;
         fe.grad[a][i][q] = g * (2.0 / h[i]);
       }
     }
     for (int a = 0; a < (1 << dim); ++a) {
-      double val = 1.0;
+      double v[3], dv[3], val = 1.0;
       for (int k = 0; k < dim; ++k) {
-        double v, dv;
-        lagrange(1, (a >> k) & 1, fe.qp[k][q], v, dv);
-        val *= v;
+        lagrange(1, (a >> k) & 1, fe.qp[k][q], v[k], dv[k]);
+        val *= v[k];
       }
       fe.gshp[a][q] = val;
+      for (int i = 0; i < 3; ++i) fe.gdshp[a][i][q] = 0.0;
+      for (int i = 0; i < dim; ++i) {
+        double g = dv[i];
+        for (int j = 0; j < dim; ++j)
+          if (j != i) g *= v[j];
+        fe.gdshp[a][i][q] = g;
+      }
     }
   }
 }

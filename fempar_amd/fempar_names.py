@@ -74,19 +74,45 @@ class environment_t:
 
 class serial_triangulation_t:
     """Struct-hex triangulation (sbm_serial_triangulation.i90:61-118); only the
-    generator parameters are kept, the mesh is implicit."""
+    generator parameters are kept, the mesh is implicit.  vertex_coordinates
+    (num_vertices x num_dims, vertex (i, j, k) at row i + (n0+1)(j + (n1+1) k))
+    moves the vertices: the topology stays the structured one and every cell
+    is integrated with its own map."""
 
-    def create(self, environment: environment_t, parameter_list: dict):
+    def create(self, environment: environment_t, parameter_list: dict, vertex_coordinates=None):
         self.environment = environment
         self.num_dims = int(parameter_list.get(STRUCT_HEX_MESH_GENERATOR_NUM_DIMS, 3))
         n = list(parameter_list.get(STRUCT_HEX_MESH_GENERATOR_NUM_CELLS_X_DIM, [10, 10, 10]))
         self.num_cells_x_dim = (n + [1, 1, 1])[:3]
         lim = list(parameter_list.get(STRUCT_HEX_MESH_GENERATOR_DOMAIN_LIMITS, [0.0, 1.0, 0.0, 1.0, 0.0, 1.0]))
         self.domain_limits = (lim + [0.0] * 6)[:6]
+        self.vertex_coordinates = None
+        if vertex_coordinates is not None:
+            vc = np.ascontiguousarray(vertex_coordinates, dtype=np.float64).reshape(-1, self.num_dims)
+            if vc.shape[0] != self.get_num_vertices():
+                raise FemhipError(N.ERR_INVALID, "vertex_coordinates must have one row per vertex")
+            self.vertex_coordinates = vc
         return self
 
     def get_num_dims(self):
         return self.num_dims
+
+    def get_num_vertices(self):
+        m = 1
+        for k in range(self.num_dims):
+            m *= self.num_cells_x_dim[k] + 1
+        return m
+
+    def get_uniform_vertex_coordinates(self):
+        """The generator's vertex coordinates (uniform_hex_mesh_generator.f90:608-612,
+        633-635), num_vertices x num_dims, x fastest."""
+        axes = []
+        for k in range(self.num_dims):
+            nk = self.num_cells_x_dim[k]
+            a, b = self.domain_limits[2 * k], self.domain_limits[2 * k + 1]
+            axes.append((b - a) * (np.arange(nk + 1) / nk) + a)
+        grids = np.meshgrid(*axes, indexing="ij")
+        return np.stack([g.transpose().ravel() for g in grids], axis=1)
 
 
 class serial_fe_space_t:
@@ -115,6 +141,8 @@ class serial_fe_space_t:
             N.check(self._lib.femhip_space_create_from_dofs(env.h, C.byref(d), N.ptr(arr), int(num_free_dofs),
                                                             int(num_fixed_dofs), C.byref(h)))
         self.h = h
+        if triangulation.vertex_coordinates is not None:
+            N.check(self._lib.femhip_space_set_vertex_coords(h, N.ptr(triangulation.vertex_coordinates)))
         nc, nd, nq, nf, nx = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         N.check(self._lib.femhip_space_info(h, C.byref(nc), C.byref(nd), C.byref(nq), C.byref(nf), C.byref(nx)))
         self.num_cells, self.num_dofs_x_cell, self.num_quadrature_points = nc.value, nd.value, nq.value
@@ -132,10 +160,19 @@ class serial_fe_space_t:
         N.check(self._lib.femhip_space_get_dofs(self.h, N.ptr(out)))
         return out
 
+    def get_quadrature_points_coordinates(self):
+        """fe_cell_iterator_t%get_quadrature_points_coordinates for every cell:
+        (num_cells, num_quadrature_points, num_dims)."""
+        d = self.triangulation.num_dims
+        out = np.empty((self.num_cells, self.num_quadrature_points, d), dtype=np.float64)
+        N.check(self._lib.femhip_space_quadrature_points(self.h, N.ptr(out)))
+        return out
+
     def interpolate_dirichlet_values(self, discrete_integration):
         out = np.zeros(max(1, self.num_fixed_dofs))
-        N.check(self._lib.femhip_space_interpolate_dirichlet(self.h, C.byref(discrete_integration.physics()),
-                                                             N.ptr(out)))
+        p = N.Physics()   # only the Dirichlet function g is read
+        p.g_kind = {"zero": N.G_ZERO, "linear": N.G_LINEAR}[getattr(discrete_integration, "g", "zero")]
+        N.check(self._lib.femhip_space_interpolate_dirichlet(self.h, C.byref(p), N.ptr(out)))
         return out[: self.num_fixed_dofs]
 
     def free(self):
@@ -151,21 +188,47 @@ class serial_fe_space_t:
 
 
 class poisson_discrete_integration_t:
-    """-div(k grad u) = f (poisson_discrete_integration.f90:62-141 with k(x_q))."""
+    """-div(k grad u) = f (poisson_discrete_integration.f90:62-141 with k(x_q)).
+    k and f may also be callables of the quadrature points' coordinates (an
+    array (..., num_dims) -> values): like the reference plugin's
+    source_term%get_value(quad_coords(qpoint)) (:109-128), they are evaluated
+    at every cell's quadrature points and passed as point values."""
 
     def __init__(self, k="const", k0=1.0, k_cell=None, f="const", f0=0.0, g="zero"):
         self.k, self.k0, self.k_cell, self.f, self.f0, self.g = k, k0, k_cell, f, f0, g
 
-    def physics(self):
+    def physics(self, fe_space=None):
         p = N.Physics()
         p.kind = N.PHYS_POISSON
-        p.k_kind = {"const": N.K_CONST, "sin3": N.K_SIN3, "cell": N.K_CELL}[self.k]
+        self._keep = []
+        xq = None
+        if callable(self.k) or callable(self.f):
+            if fe_space is None:
+                raise FemhipError(N.ERR_INVALID, "a callable coefficient needs the FE space's quadrature points")
+            xq = fe_space.get_quadrature_points_coordinates()
+
+        def point_values(fn):
+            v = np.ascontiguousarray(np.asarray(fn(xq), dtype=np.float64).reshape(-1))
+            if v.size != xq.shape[0] * xq.shape[1]:
+                raise FemhipError(N.ERR_INVALID, "a callable coefficient must give one value per quadrature point")
+            self._keep.append(v)
+            return v.ctypes.data_as(C.POINTER(C.c_double))
+
+        if callable(self.k):
+            p.k_kind = N.K_QPOINT
+            p.k_qp = point_values(self.k)
+        else:
+            p.k_kind = {"const": N.K_CONST, "sin3": N.K_SIN3, "cell": N.K_CELL}[self.k]
         p.k0 = self.k0
-        self._kc = None
         if self.k_cell is not None:
-            self._kc = np.ascontiguousarray(self.k_cell, dtype=np.float64)
-            p.k_cell = self._kc.ctypes.data_as(C.POINTER(C.c_double))
-        p.f_kind = {"const": N.F_CONST, "manufactured": N.F_MANUFACTURED, "cell_sin": N.F_CELL_SIN}[self.f]
+            kc = np.ascontiguousarray(self.k_cell, dtype=np.float64)
+            self._keep.append(kc)
+            p.k_cell = kc.ctypes.data_as(C.POINTER(C.c_double))
+        if callable(self.f):
+            p.f_kind = N.F_QPOINT
+            p.f_qp = point_values(self.f)
+        else:
+            p.f_kind = {"const": N.F_CONST, "manufactured": N.F_MANUFACTURED, "cell_sin": N.F_CELL_SIN}[self.f]
         p.f0 = self.f0
         p.g_kind = {"zero": N.G_ZERO, "linear": N.G_LINEAR}[self.g]
         return p
@@ -177,7 +240,7 @@ class linear_elasticity_discrete_integration_t:
     def __init__(self, E=1.0, nu=0.2, body=(0.0, 0.0, -1.0)):
         self.E, self.nu, self.body = E, nu, body
 
-    def physics(self):
+    def physics(self, fe_space=None):
         p = N.Physics()
         p.kind = N.PHYS_ELASTICITY
         p.E, p.nu = self.E, self.nu
@@ -217,7 +280,7 @@ class fe_affine_operator_t:
         n, nnz, nnzf = C.c_int32(), C.c_int64(), C.c_int64()
         N.check(self._lib.femhip_operator_info(h, C.byref(n), C.byref(nnz), C.byref(nnzf)))
         self.n, self.nnz, self.nnz_full = n.value, nnz.value, nnzf.value
-        self._phys = discrete_integration.physics()
+        self._phys = discrete_integration.physics(fe_space)
         N.check(self._lib.femhip_operator_set_physics(h, C.byref(self._phys)))
         self.fixed_values = None
         self.last_compute_ms = None

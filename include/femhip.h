@@ -52,10 +52,15 @@ extern "C" {
 #define FEMHIP_K_CONST            0   /* k = k0 */
 #define FEMHIP_K_SIN3             1   /* k = 1 + 0.5 sin(2 pi x) sin(2 pi y) sin(2 pi z) */
 #define FEMHIP_K_CELL             2   /* k = k_cell[cell], piecewise constant */
+#define FEMHIP_K_QPOINT           3   /* k = k_qp[cell * nq + q] at each quadrature point */
 
 #define FEMHIP_F_CONST            0   /* f = f0 */
 #define FEMHIP_F_MANUFACTURED     1   /* f = -div(k grad u), u = prod_d sin(pi x_d) */
 #define FEMHIP_F_CELL_SIN         2   /* f = f0 k_cell d pi^2 prod_d sin(pi x_d) */
+#define FEMHIP_F_QPOINT           3   /* f = f_qp[cell * nq + q]: any source_term%get_value
+                                         evaluated by the caller at the points of
+                                         femhip_space_quadrature_points
+                                         (poisson_discrete_integration.f90:109-128) */
 
 #define FEMHIP_G_ZERO             0   /* homogeneous Dirichlet */
 #define FEMHIP_G_LINEAR           1   /* g = x + y (+ z), poisson_analytical_functions.f90:77-104 */
@@ -104,6 +109,8 @@ typedef struct {
   int32_t g_kind;           /* FEMHIP_G_* (used by femhip_space_interpolate_dirichlet) */
   double E, nu;             /* elasticity: lambda = nu E/((1+nu)(1-2nu)), mu = E/(2(1+nu)) */
   double body[3];           /* elasticity body force */
+  const double *k_qp;       /* host, num_cells x nq values for FEMHIP_K_QPOINT, read during set_physics */
+  const double *f_qp;       /* host, num_cells x nq values for FEMHIP_F_QPOINT, read during set_physics */
 } femhip_physics_t;
 
 typedef struct {
@@ -155,6 +162,24 @@ int femhip_space_create_from_dofs(femhip_ctx_t *ctx, const femhip_mesh_desc_t *m
 int femhip_space_info(const femhip_space_t *space, int64_t *num_cells, int32_t *nd,
                       int32_t *nq, int32_t *n_free, int32_t *n_fixed);
 int femhip_space_get_dofs(const femhip_space_t *space, int32_t *lst_dofs_gids);
+/* Vertex coordinates of the triangulation (serial_triangulation_t's vertex
+ * coordinates, read by fe_cell_iterator_t%get_nodes_coordinates,
+ * sbm_fe_cell_iterator.i90:966-967): num_vertices x dim doubles, vertex
+ * (i, j, k) at index i + (n0 + 1) (j + (n1 + 1) k), x fastest.  The topology
+ * (cells, vefs, DOF numbering, CSR pattern) stays that of the structured mesh;
+ * every cell then has its own trilinear map, recomputed per cell as the
+ * reference does when a cell is not a translate of the one before
+ * (cell_ressemblance = no_ressemblance, sbm_fe_cell_iterator.i90:970-987 with
+ * sbm_cell_map.i90:245-369).  Set before femhip_operator_compute; operators
+ * computed on the space afterwards integrate every cell with its own Jacobian.
+ * Scalar Poisson (Q1/Q2, 2-D/3-D) and vector Q1 elasticity. */
+int femhip_space_set_vertex_coords(femhip_space_t *space, const double *coords);
+/* fe_cell_iterator_t%get_quadrature_points_coordinates (sbm_cell_map.i90:673-687)
+ * for every cell: num_cells x nq x dim doubles (cell-major, then quadrature
+ * point in the reference order, first index fastest), so a caller can evaluate
+ * its own coefficient or source function there and pass FEMHIP_K_QPOINT /
+ * FEMHIP_F_QPOINT values. */
+int femhip_space_quadrature_points(const femhip_space_t *space, double *xq);
 /* serial_fe_space_t%interpolate_dirichlet_values (sbm_serial_fe_space.i90:997-1176):
  * nodal interpolation of g (phys->g_kind) into fixed_values (n_fixed). */
 int femhip_space_interpolate_dirichlet(const femhip_space_t *space, const femhip_physics_t *phys,

@@ -57,7 +57,6 @@ struct orc_space {
   double gdshp[8][3][MAXNQ];         /* geometric Q1 gradients */
   /* affine map of the first cell, reused by translation (sbm_fe_cell_iterator.i90:970-987) */
   double detj[MAXNQ];
-  double invj[MAXNQ][3][3];
   double grad[MAXND][3][MAXNQ];      /* physical scalar gradients (sbm_lagrangian_reference_fe.i90:1334-1341) */
   /* space */
   int64_t ncells;
@@ -312,21 +311,32 @@ static void build_reffe(orc_space_t *s) {
 }
 
 static void cell_vertices(const orc_space_t *s, int64_t c, double xv[8][3]) {
+  /* fe_cell_iterator_t%get_nodes_coordinates (sbm_fe_cell_iterator.i90:966-967):
+     the triangulation's vertex coordinates, generated (uniform) or given */
   int ijk[3];
   cell_ijk(s, c, ijk);
   for (int a = 0; a < (1 << s->d); ++a) {
     for (int k = 0; k < 3; ++k) xv[a][k] = 0.0;
-    for (int k = 0; k < s->d; ++k) xv[a][k] = vertex_coord(s, k, ijk[k] + ((a >> k) & 1));
+    if (s->p.vcoords) {
+      int64_t v = 0, mul = 1;
+      for (int k = 0; k < s->d; ++k) {
+        v += mul * (ijk[k] + ((a >> k) & 1));
+        mul *= s->p.n[k] + 1;
+      }
+      for (int k = 0; k < s->d; ++k) xv[a][k] = s->p.vcoords[v * s->d + k];
+    } else {
+      for (int k = 0; k < s->d; ++k) xv[a][k] = vertex_coord(s, k, ijk[k] + ((a >> k) & 1));
+    }
   }
 }
 
-static void build_map(orc_space_t *s) {
-  /* cell_map_update on the first cell (sbm_cell_map.i90:245-300): Jacobian as the
-     linear combination of geometric gradients and node coordinates
-     (field.f90:763-787), inverse and determinant by invmtx
-     (sbm_cell_integrator.i90:448-560); afterwards every cell is "only translated". */
+/* cell_map_update (sbm_cell_map.i90:245-369) on cell c: Jacobian as the linear
+   combination of geometric gradients and node coordinates (field.f90:763-787),
+   inverse and determinant by invmtx (sbm_cell_integrator.i90:448-560), physical
+   gradients (sbm_lagrangian_reference_fe.i90:1334-1341). */
+static void cell_map(const orc_space_t *s, int64_t c, double detj[MAXNQ], double grad[MAXND][3][MAXNQ]) {
   double xv[8][3];
-  cell_vertices(s, 0, xv);
+  cell_vertices(s, c, xv);
   int d = s->d;
   for (int q = 0; q < s->nq; ++q) {
     double J[3][3] = {{0}};
@@ -358,16 +368,21 @@ static void build_map(orc_space_t *s) {
       B[0][2] = (J[0][1] * J[1][2] - J[1][1] * J[0][2]) * den;
       B[1][2] = (-J[0][0] * J[1][2] + J[1][0] * J[0][2]) * den;
     }
-    s->detj[q] = fabs(det);
-    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) s->invj[q][a][b] = B[a][b];
+    detj[q] = fabs(det);
     /* phy(i) += invJ(j,i) * ref(j)  (sbm_lagrangian_reference_fe.i90:1334-1341) */
     for (int a = 0; a < s->nnode; ++a)
       for (int i = 0; i < 3; ++i) {
         double g = 0.0;
         for (int j = 0; j < 3; ++j) g = g + B[j][i] * s->dshp[a][j][q];
-        s->grad[a][i][q] = g;
+        grad[a][i][q] = g;
       }
   }
+}
+
+static void build_map(orc_space_t *s) {
+  /* the first cell's map; on the generator's uniform mesh every later cell is
+     "only translated" and reuses it (sbm_fe_cell_iterator.i90:970-987) */
+  cell_map(s, 0, s->detj, s->grad);
 }
 
 /* ------------------------------------------------------- DOF numbering --- */
@@ -494,25 +509,26 @@ void orc_reference_tables(const orc_space_t *s, double *qpts, double *qw, double
 
 static const double PI = 3.14159265358979323846;
 
-static double k_value(const orc_physics_t *ph, int d, const double x[3], int64_t cell) {
+static double k_value(const orc_physics_t *ph, int d, const double x[3], int64_t cell, int q, int nq) {
   switch (ph->k_kind) {
     case ORC_K_SIN3: {
       double v = 1.0 + 0.5 * sin(2 * PI * x[0]) * sin(2 * PI * x[1]) * (d == 3 ? sin(2 * PI * x[2]) : 1.0);
       return v;
     }
     case ORC_K_CELL: return ph->k_cell[cell];
+    case ORC_K_QPOINT: return ph->k_qp[cell * nq + q];
     default: return ph->k0;
   }
 }
 
-static double f_value(const orc_physics_t *ph, int d, const double x[3], int64_t cell) {
+static double f_value(const orc_physics_t *ph, int d, const double x[3], int64_t cell, int q, int nq) {
   switch (ph->f_kind) {
     case ORC_F_MANUFACTURED: {
       /* f = -div(k grad u) = k * d pi^2 u - grad k . grad u, u = prod sin(pi x_k) */
       double su[3], cu[3];
       for (int k = 0; k < 3; ++k) { su[k] = (k < d) ? sin(PI * x[k]) : 1.0; cu[k] = (k < d) ? cos(PI * x[k]) : 0.0; }
       double u = su[0] * su[1] * su[2];
-      double kk = k_value(ph, d, x, cell);
+      double kk = k_value(ph, d, x, cell, q, nq);
       double f = kk * d * PI * PI * u;
       if (ph->k_kind == ORC_K_SIN3) {
         double s2[3], c2[3];
@@ -533,8 +549,9 @@ static double f_value(const orc_physics_t *ph, int d, const double x[3], int64_t
     case ORC_F_CELL_SIN: {
       double u = 1.0;
       for (int k = 0; k < d; ++k) u *= sin(PI * x[k]);
-      return ph->f0 * k_value(ph, d, x, cell) * d * PI * PI * u;
+      return ph->f0 * k_value(ph, d, x, cell, q, nq) * d * PI * PI * u;
     }
+    case ORC_F_QPOINT: return ph->f_qp[cell * nq + q];
     default: return ph->f0;
   }
 }
@@ -560,11 +577,33 @@ static void qpoint_coords(const orc_space_t *s, int64_t c, double xq[MAXNQ][3]) 
   }
 }
 
+void orc_quadrature_points(const orc_space_t *s, double *out) {
+  double xq[MAXNQ][3];
+  for (int64_t c = 0; c < s->ncells; ++c) {
+    qpoint_coords(s, c, xq);
+    for (int q = 0; q < s->nq; ++q)
+      for (int k = 0; k < s->d; ++k) out[(c * s->nq + q) * s->d + k] = xq[q][k];
+  }
+}
+
+/* the cell's map: the space's first-cell map (uniform mesh, every cell a
+   translate) or the cell's own (given vertex coordinates) */
+typedef struct { double detj[MAXNQ]; double grad[MAXND][3][MAXNQ]; } cell_geo_t;
+static const cell_geo_t *cell_geometry(const orc_space_t *s, int64_t c, cell_geo_t *buf) {
+  if (!s->p.vcoords) return (const cell_geo_t *)0;
+  cell_map(s, c, buf->detj, buf->grad);
+  return buf;
+}
+
 void orc_element(const orc_space_t *s, const orc_physics_t *ph, int64_t c, double *elmat,
                  double *elvec) {
   int nd = s->nd, nn = s->nnode, d = s->d;
   double xq[MAXNQ][3];
   qpoint_coords(s, c, xq);
+  cell_geo_t geo;
+  const cell_geo_t *g = cell_geometry(s, c, &geo);
+  const double *detj = g ? g->detj : s->detj;
+  const double (*grad)[3][MAXNQ] = g ? g->grad : s->grad;
   for (int i = 0; i < nd * nd; ++i) elmat[i] = 0.0;
   for (int i = 0; i < nd; ++i) elvec[i] = 0.0;
   if (ph->kind == ORC_PHYS_POISSON) {
@@ -572,15 +611,15 @@ void orc_element(const orc_space_t *s, const orc_physics_t *ph, int64_t c, doubl
        coefficient at the quadrature point as in the heterogeneous-coefficient
        plugin of par_test_pb_bddc_poisson */
     for (int q = 0; q < s->nq; ++q) {
-      double factor = s->detj[q] * s->qw[q];
-      double kq = k_value(ph, d, xq[q], c);
+      double factor = detj[q] * s->qw[q];
+      double kq = k_value(ph, d, xq[q], c, q, s->nq);
       for (int i = 0; i < nd; ++i)
         for (int j = 0; j < nd; ++j) {
           double dot = 0.0;
-          for (int k = 0; k < 3; ++k) dot += s->grad[j][k][q] * s->grad[i][k][q];
+          for (int k = 0; k < 3; ++k) dot += grad[j][k][q] * grad[i][k][q];
           elmat[i + j * nd] = elmat[i + j * nd] + factor * kq * dot;
         }
-      double fq = f_value(ph, d, xq[q], c);
+      double fq = f_value(ph, d, xq[q], c, q, s->nq);
       for (int i = 0; i < nd; ++i) elvec[i] = elvec[i] + factor * fq * s->shp[i][q];
     }
   } else {
@@ -590,19 +629,19 @@ void orc_element(const orc_space_t *s, const orc_physics_t *ph, int64_t c, doubl
     double lambda = (nu * E) / ((1 + nu) * (1 - 2 * nu));
     double mu = E / (2 * (1 + nu));
     for (int q = 0; q < s->nq; ++q) {
-      double dV = s->detj[q] * s->qw[q];
+      double dV = detj[q] * s->qw[q];
       for (int i = 0; i < nd; ++i) {
         int ci = i / nn, ai = i % nn;
         for (int j = 0; j < nd; ++j) {
           int cj = j / nn, aj = j % nn;
           /* grad(phi_j)(r,s) = e_cj(r) * dN_aj/dx_s ; eps = sym part */
           double G[3][3] = {{0}}, eps[3][3], sig[3][3];
-          for (int k = 0; k < 3; ++k) G[cj][k] = s->grad[aj][k][q];
+          for (int k = 0; k < 3; ++k) G[cj][k] = grad[aj][k][q];
           for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) eps[a][b] = 0.5 * (G[a][b] + G[b][a]);
           double tr = eps[0][0] + eps[1][1] + eps[2][2];
           for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) sig[a][b] = lambda * tr * (a == b ? 1.0 : 0.0) + 2 * mu * eps[a][b];
           double dc = 0.0;
-          for (int k = 0; k < 3; ++k) dc += s->grad[ai][k][q] * sig[ci][k];
+          for (int k = 0; k < 3; ++k) dc += grad[ai][k][q] * sig[ci][k];
           elmat[i + j * nd] = elmat[i + j * nd] + dV * dc;
         }
         elvec[i] = elvec[i] + dV * ph->body[ci] * s->shp[ai][q];
@@ -943,8 +982,12 @@ void orc_error_norms(const orc_space_t *s, const double *x_free, const double *f
   const int nd = s->nd, d = s->d;
   double values_norm = 0.0, gradients_norm = 0.0, linf = 0.0;
   double xq[MAXNQ][3], nodal[MAXND];
+  cell_geo_t geo;
   for (int64_t c = 0; c < s->ncells; ++c) {   /* fe cell iterator, cells in order (:94-110) */
     qpoint_coords(s, c, xq);
+    const cell_geo_t *g = cell_geometry(s, c, &geo);
+    const double *detj = g ? g->detj : s->detj;
+    const double (*grad)[3][MAXNQ] = g ? g->grad : s->grad;
     const int32_t *cd = s->dofs + c * nd;
     for (int i = 0; i < nd; ++i)   /* fe_cell_function%update: nodal values of the cell */
       nodal[i] = cd[i] > 0 ? x_free[cd[i] - 1] : (fixed_values ? fixed_values[-cd[i] - 1] : 0.0);
@@ -953,13 +996,13 @@ void orc_error_norms(const orc_space_t *s, const double *x_free, const double *f
       double uh = 0.0, guh[3] = {0.0, 0.0, 0.0}, u, gu[3];
       for (int i = 0; i < nd; ++i) {   /* values and gradients at the quadrature point */
         uh += nodal[i] * s->shp[i][q];
-        for (int k = 0; k < 3; ++k) guh[k] += nodal[i] * s->grad[i][k][q];
+        for (int k = 0; k < 3; ++k) guh[k] += nodal[i] * grad[i][k][q];
       }
       exact_value(exact, d, xq[q], &u, gu);
       const double e = u - uh;
       double ge2 = 0.0;
       for (int k = 0; k < 3; ++k) ge2 += (gu[k] - guh[k]) * (gu[k] - guh[k]);
-      const double dv = s->detj[q] * s->qw[q];
+      const double dv = detj[q] * s->qw[q];
       vc += (e * e) * dv;      /* l2_norm, h1_norm (:200-205) */
       gc += ge2 * dv;          /* h1_norm, h1_seminorm (:220-225) */
       if (fabs(e) > linf) linf = fabs(e);

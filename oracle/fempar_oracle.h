@@ -32,11 +32,14 @@ enum { ORC_PHYS_POISSON = 0, ORC_PHYS_ELASTICITY = 1 };
 
 enum { ORC_K_CONST = 0,        /* k = k0                                                      */
        ORC_K_SIN3 = 1,         /* k = 1 + 0.5 sin(2pi x) sin(2pi y) sin(2pi z)                */
-       ORC_K_CELL = 2 };       /* k = k_cell[cell] (piecewise constant field)                 */
+       ORC_K_CELL = 2,         /* k = k_cell[cell] (piecewise constant field)                 */
+       ORC_K_QPOINT = 3 };     /* k = k_qp[cell * nq + q] (values at the quadrature points)   */
 
 enum { ORC_F_CONST = 0,        /* f = f0                                                      */
        ORC_F_MANUFACTURED = 1, /* f = -div(k grad u), u = prod sin(pi x_d), k of k_kind         */
-       ORC_F_CELL_SIN = 2 };   /* f = f0 * k_cell * d pi^2 prod sin(pi x_d) (cell-wise source)  */
+       ORC_F_CELL_SIN = 2,     /* f = f0 * k_cell * d pi^2 prod sin(pi x_d) (cell-wise source)  */
+       ORC_F_QPOINT = 3 };     /* f = f_qp[cell * nq + q]: source_term%get_value at each point
+                                  (poisson_discrete_integration.f90:109-128)                    */
 
 enum { ORC_G_ZERO = 0,         /* homogeneous Dirichlet                                        */
        ORC_G_LINEAR = 1 };     /* g = x + y (+ z)  (poisson_analytical_functions.f90:77-104)   */
@@ -48,6 +51,10 @@ typedef struct {
   int order;         /* Lagrangian order: 1 or 2 */
   int ncomp;         /* 1 (scalar) or dim (vector, elasticity) */
   int bc_kind;       /* ORC_BC_* */
+  const double *vcoords;  /* NULL: the generator's uniform vertices; else num_vertices x dim
+                             coordinates (vertex (i,j,k) at i + (n0+1)(j + (n1+1) k)), and every
+                             cell's map is computed from its own vertices
+                             (sbm_fe_cell_iterator.i90:970-987, no_ressemblance) */
 } orc_problem_t;
 
 typedef struct {
@@ -60,6 +67,8 @@ typedef struct {
   int g_kind;        /* ORC_G_* */
   double E, nu;      /* elasticity (lambda, mu as par_test_elasticity_constitutive_models.f90:34-37) */
   double body[3];    /* elasticity body force */
+  const double *k_qp;     /* num_cells x nq values when k_kind == ORC_K_QPOINT */
+  const double *f_qp;     /* num_cells x nq values when f_kind == ORC_F_QPOINT */
 } orc_physics_t;
 
 typedef struct orc_space orc_space_t;
@@ -83,6 +92,9 @@ void orc_gauss_legendre(int n, double *x, double *w);
 
 /* One element of cell `cell` (0-based): elmat (nd x nd, column-major) and
    elvec (nd), before strong-Dirichlet elimination. */
+/* physical coordinates of every quadrature point, num_cells x nq x dim
+   (cell_map_compute_quadrature_points_coordinates, sbm_cell_map.i90:673-687) */
+void orc_quadrature_points(const orc_space_t *s, double *xq);
 void orc_element(const orc_space_t *s, const orc_physics_t *phys, int64_t cell,
                  double *elmat, double *elvec);
 

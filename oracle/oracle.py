@@ -17,8 +17,8 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 BC_ALL, BC_X0, BC_NONE = 0, 1, 2
 PHYS_POISSON, PHYS_ELASTICITY = 0, 1
-K_CONST, K_SIN3, K_CELL = 0, 1, 2
-F_CONST, F_MANUFACTURED, F_CELL_SIN = 0, 1, 2
+K_CONST, K_SIN3, K_CELL, K_QPOINT = 0, 1, 2, 3
+F_CONST, F_MANUFACTURED, F_CELL_SIN, F_QPOINT = 0, 1, 2, 3
 G_ZERO, G_LINEAR = 0, 1
 EXACT_SIN, EXACT_LINEAR = 0, 1
 
@@ -29,13 +29,14 @@ CRITERIA = ["RES_NRMGIVEN_RHS_NRMGIVEN", "RES_NRMGIVEN_RES_NRMGIVEN", "DELTA_RHS
 
 class Problem(C.Structure):
     _fields_ = [("dim", C.c_int), ("n", C.c_int * 3), ("lim", C.c_double * 6), ("order", C.c_int),
-                ("ncomp", C.c_int), ("bc_kind", C.c_int)]
+                ("ncomp", C.c_int), ("bc_kind", C.c_int), ("vcoords", C.POINTER(C.c_double))]
 
 
 class Physics(C.Structure):
     _fields_ = [("kind", C.c_int), ("k_kind", C.c_int), ("k0", C.c_double),
                 ("k_cell", C.POINTER(C.c_double)), ("f_kind", C.c_int), ("f0", C.c_double),
-                ("g_kind", C.c_int), ("E", C.c_double), ("nu", C.c_double), ("body", C.c_double * 3)]
+                ("g_kind", C.c_int), ("E", C.c_double), ("nu", C.c_double), ("body", C.c_double * 3),
+                ("k_qp", C.POINTER(C.c_double)), ("f_qp", C.POINTER(C.c_double))]
 
 
 class CgResult(C.Structure):
@@ -74,6 +75,7 @@ def lib():
         L.orc_reference_tables.argtypes = [P, P, P, P, P]
         L.orc_gauss_legendre.argtypes = [C.c_int, P, P]
         L.orc_element.argtypes = [P, C.POINTER(Physics), C.c_int64, P, P]
+        L.orc_quadrature_points.argtypes = [P, P]
         L.orc_assemble.restype = C.c_int64
         L.orc_assemble.argtypes = [P, C.POINTER(Physics), C.c_int, P, C.POINTER(P), C.POINTER(P),
                                    C.POINTER(P), C.POINTER(P)]
@@ -107,8 +109,15 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
-def make_problem(dim, n, order=1, ncomp=1, bc=BC_ALL, lim=None):
+def make_problem(dim, n, order=1, ncomp=1, bc=BC_ALL, lim=None, vcoords=None):
+    """vcoords: optional (num_vertices, dim) vertex coordinates, vertex (i, j, k)
+    at row i + (n0+1)(j + (n1+1) k); every cell then has its own map."""
     p = Problem()
+    p._vc_ref = None
+    if vcoords is not None:
+        arr = np.ascontiguousarray(vcoords, dtype=np.float64).ravel()
+        p._vc_ref = arr
+        p.vcoords = arr.ctypes.data_as(C.POINTER(C.c_double))
     p.dim = dim
     nn = list(n) + [1] * (3 - len(n))
     for i in range(3):
@@ -121,14 +130,15 @@ def make_problem(dim, n, order=1, ncomp=1, bc=BC_ALL, lim=None):
 
 
 def make_physics(kind=PHYS_POISSON, k_kind=K_CONST, k0=1.0, k_cell=None, f_kind=F_CONST, f0=0.0,
-                 g_kind=G_ZERO, E=1.0, nu=0.3, body=(0.0, 0.0, -1.0)):
+                 g_kind=G_ZERO, E=1.0, nu=0.3, body=(0.0, 0.0, -1.0), k_qp=None, f_qp=None):
     ph = Physics()
     ph.kind, ph.k_kind, ph.k0 = kind, k_kind, k0
-    ph._k_cell_ref = None
-    if k_cell is not None:
-        arr = np.ascontiguousarray(k_cell, dtype=np.float64)
-        ph._k_cell_ref = arr
-        ph.k_cell = arr.ctypes.data_as(C.POINTER(C.c_double))
+    ph._refs = []
+    for name, a in (("k_cell", k_cell), ("k_qp", k_qp), ("f_qp", f_qp)):
+        if a is not None:
+            arr = np.ascontiguousarray(a, dtype=np.float64).ravel()
+            ph._refs.append(arr)
+            setattr(ph, name, arr.ctypes.data_as(C.POINTER(C.c_double)))
     ph.f_kind, ph.f0, ph.g_kind, ph.E, ph.nu = f_kind, f0, g_kind, E, nu
     for i in range(3):
         ph.body[i] = body[i]
@@ -182,6 +192,13 @@ class Space:
         lib().orc_error_norms(self.h, _ptr(x), _ptr(fx), {"sin": EXACT_SIN, "linear": EXACT_LINEAR}[exact],
                               _ptr(out))
         return {"l2_norm": out[0], "h1_seminorm": out[1], "h1_norm": out[2], "linfty_norm": out[3]}
+
+    def quadrature_points(self):
+        """Physical coordinates of every quadrature point, (num_cells, nq, dim)."""
+        d = self.problem.dim
+        out = np.empty((self.num_cells, self.nq, d), dtype=np.float64)
+        lib().orc_quadrature_points(self.h, _ptr(out))
+        return out
 
     def element(self, phys, cell):
         em = np.empty((self.nd, self.nd), dtype=np.float64, order="F")
