@@ -18,7 +18,9 @@ module femhip_names
   integer(c_int32_t), parameter, public :: FEMHIP_BC_ALL_BOUNDARY = 0, FEMHIP_BC_X0_FACE = 1, FEMHIP_BC_NONE = 2
   integer(c_int32_t), parameter, public :: FEMHIP_PHYS_POISSON = 0, FEMHIP_PHYS_ELASTICITY = 1
   integer(c_int32_t), parameter, public :: FEMHIP_K_CONST = 0, FEMHIP_K_SIN3 = 1, FEMHIP_K_CELL = 2
+  integer(c_int32_t), parameter, public :: FEMHIP_K_QPOINT = 3
   integer(c_int32_t), parameter, public :: FEMHIP_F_CONST = 0, FEMHIP_F_MANUFACTURED = 1, FEMHIP_F_CELL_SIN = 2
+  integer(c_int32_t), parameter, public :: FEMHIP_F_QPOINT = 3
   integer(c_int32_t), parameter, public :: FEMHIP_G_ZERO = 0, FEMHIP_G_LINEAR = 1
   integer(c_int32_t), parameter, public :: FEMHIP_RES_RES = 4, FEMHIP_RES_RHS = 5
   integer(c_int32_t), parameter, public :: FEMHIP_PREC_IDENTITY = 0, FEMHIP_PREC_JACOBI = 1
@@ -44,6 +46,8 @@ module femhip_names
     integer(c_int32_t) :: g_kind = 0
     real(c_double)     :: E = 1.0_c_double, nu = 0.3_c_double
     real(c_double)     :: body(3) = [0.0_c_double, 0.0_c_double, -1.0_c_double]
+    type(c_ptr)        :: k_qp = c_null_ptr     ! num_cells x nq values (FEMHIP_K_QPOINT)
+    type(c_ptr)        :: f_qp = c_null_ptr     ! num_cells x nq values (FEMHIP_F_QPOINT)
   end type
 
   type, bind(c), public :: femhip_cg_params_t
@@ -71,6 +75,7 @@ module femhip_names
   public :: femhip_ctx_create, femhip_ctx_destroy, femhip_last_error, femhip_check
   public :: femhip_space_create_structured, femhip_space_create_from_dofs, femhip_space_info
   public :: femhip_space_get_dofs, femhip_space_interpolate_dirichlet, femhip_space_destroy
+  public :: femhip_space_set_vertex_coords, femhip_space_quadrature_points
   public :: femhip_operator_create, femhip_operator_info, femhip_operator_get_pattern
   public :: femhip_operator_set_physics, femhip_operator_compute, femhip_operator_get_values
   public :: femhip_operator_apply, femhip_operator_destroy
@@ -124,6 +129,16 @@ module femhip_names
       type(c_ptr), value :: space
       type(femhip_physics_t), intent(in) :: phys
       real(c_double), intent(out) :: fixed_values(*)
+    end function
+    integer(c_int) function femhip_space_set_vertex_coords(space, coords) bind(c, name="femhip_space_set_vertex_coords")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: space
+      real(c_double), intent(in) :: coords(*)
+    end function
+    integer(c_int) function femhip_space_quadrature_points(space, xq) bind(c, name="femhip_space_quadrature_points")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: space
+      real(c_double), intent(out) :: xq(*)
     end function
     integer(c_int) function femhip_space_destroy(space) bind(c, name="femhip_space_destroy")
       import :: c_int, c_ptr

@@ -36,14 +36,25 @@ module fempar_seams
   end type
 
   ! serial_triangulation_t from the struct-hex generator parameters
-  ! (STRUCT_HEX_MESH_GENERATOR_NUM_DIMS / _NUM_CELLS_X_DIM / _DOMAIN_LIMITS)
+  ! (STRUCT_HEX_MESH_GENERATOR_NUM_DIMS / _NUM_CELLS_X_DIM / _DOMAIN_LIMITS),
+  ! optionally with moved vertex coordinates (num_dims x num_vertices, vertex
+  ! (i, j, k) at column i + (n0+1)(j + (n1+1) k) + 1)
   type, public :: serial_triangulation_t
     type(environment_t), pointer :: environment => null()
     integer(ip) :: num_dims = 3
     integer(ip) :: num_cells_x_dim(3) = 1
     real(rp)    :: domain_limits(6) = [0.0_rp, 1.0_rp, 0.0_rp, 1.0_rp, 0.0_rp, 1.0_rp]
+    real(rp), allocatable :: vertex_coordinates(:, :)
   contains
     procedure :: create => triangulation_create
+  end type
+
+  ! scalar_function_t: an analytical function evaluated at a point
+  ! (Integration/Function/function.f90:59-82, get_value_space)
+  type, public :: scalar_function_t
+  contains
+    procedure :: get_value_space => scalar_function_get_value_space
+    generic :: get_value => get_value_space
   end type
 
   ! serial_fe_space_t: one scalar Lagrangian field with strong Dirichlet data
@@ -246,17 +257,28 @@ contains
     this%ctx = c_null_ptr
   end subroutine
 
-  subroutine triangulation_create(this, environment, num_dims, num_cells_x_dim, domain_limits)
+  subroutine triangulation_create(this, environment, num_dims, num_cells_x_dim, domain_limits, vertex_coordinates)
     class(serial_triangulation_t), intent(inout) :: this
     type(environment_t), intent(in), target :: environment
     integer(ip), intent(in) :: num_dims, num_cells_x_dim(:)
     real(rp), intent(in) :: domain_limits(:)
+    real(rp), intent(in), optional :: vertex_coordinates(:, :)
     this%environment => environment
     this%num_dims = num_dims
     this%num_cells_x_dim = 1
     this%num_cells_x_dim(1:num_dims) = num_cells_x_dim(1:num_dims)
     this%domain_limits = 0.0_rp
     this%domain_limits(1:2 * num_dims) = domain_limits(1:2 * num_dims)
+    if (allocated(this%vertex_coordinates)) deallocate(this%vertex_coordinates)
+    if (present(vertex_coordinates)) this%vertex_coordinates = vertex_coordinates
+  end subroutine
+
+  ! the base type's value: 0 (concrete functions override get_value_space)
+  subroutine scalar_function_get_value_space(this, point, result)
+    class(scalar_function_t), intent(in) :: this
+    real(rp), intent(in) :: point(3)
+    real(rp), intent(inout) :: result
+    result = 0.0_rp
   end subroutine
 
   subroutine fe_space_create(this, triangulation, order)
@@ -275,6 +297,9 @@ contains
     mesh%dirichlet = FEMHIP_BC_ALL_BOUNDARY
     call femhip_check(femhip_space_create_structured(triangulation%environment%ctx, mesh, this%h), &
                       'serial_fe_space_t%create')
+    if (allocated(triangulation%vertex_coordinates)) &
+      call femhip_check(femhip_space_set_vertex_coords(this%h, triangulation%vertex_coordinates), &
+                        'serial_fe_space_t%create')
     call femhip_check(femhip_space_info(this%h, this%num_cells, this%nd, this%nq, nf, nx), 'serial_fe_space_t%create')
     this%num_free_dofs = nf
     this%num_fixed_dofs = nx

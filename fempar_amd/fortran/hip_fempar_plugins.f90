@@ -38,6 +38,9 @@ module hip_fempar_plugins
 
   type, extends(discrete_integration_t), public :: hip_poisson_discrete_integration_t
     type(femhip_physics_t) :: phys
+    ! when associated: the plug-in's source term, evaluated at every cell's
+    ! quadrature points as poisson_discrete_integration.f90:109-128 does
+    class(scalar_function_t), pointer :: source_term => null()
   contains
     procedure :: integrate_galerkin => hip_poisson_integrate_galerkin
   end type
@@ -143,9 +146,32 @@ contains
     class(hip_poisson_discrete_integration_t), intent(in) :: this
     class(serial_fe_space_t), intent(inout) :: fe_space
     class(assembler_t), intent(inout) :: assembler
+    type(femhip_physics_t) :: phys
+    real(rp), allocatable, target :: quad_coords(:), source_values(:)
+    real(rp) :: point(3)
+    integer(c_int64_t) :: i, npts
+    integer :: d, dim
+    phys = this%phys
+    if (associated(this%source_term)) then
+      ! quad_coords => fe%get_quadrature_points_coordinates() for every cell,
+      ! then source_term%get_value(quad_coords(qpoint), source_term_value)
+      dim = fe_space%triangulation%num_dims
+      npts = fe_space%num_cells * fe_space%nq
+      allocate(quad_coords(npts * dim), source_values(npts))
+      call femhip_check(femhip_space_quadrature_points(fe_space%h, quad_coords), 'integrate_galerkin')
+      do i = 0, npts - 1
+        point = 0.0_rp
+        do d = 1, dim
+          point(d) = quad_coords(i * dim + d)
+        end do
+        call this%source_term%get_value(point, source_values(i + 1))
+      end do
+      phys%f_kind = FEMHIP_F_QPOINT
+      phys%f_qp = c_loc(source_values)
+    end if
     select type (assembler)
     type is (hip_sparse_assembler_t)
-      call femhip_check(femhip_operator_set_physics(assembler%op, this%phys), 'integrate_galerkin')
+      call femhip_check(femhip_operator_set_physics(assembler%op, phys), 'integrate_galerkin')
       call femhip_check(femhip_operator_compute(assembler%op, c_loc(fe_space%fixed_dof_values), c_null_ptr), &
                         'integrate_galerkin')
     class default

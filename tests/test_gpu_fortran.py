@@ -33,3 +33,16 @@ def test_fortran_test_poisson_through_the_plugin_seams(dim, n, order, prec):
                        timeout=120)
     print(p.stdout, p.stderr)
     assert p.returncode == 0 and "PASSED" in p.stdout
+
+
+@pytest.mark.parametrize("dim,n,order,prec", [(3, 6, 1, 1), (2, 12, 2, 1), (3, 4, 2, 0)])
+def test_fortran_seams_moved_vertices_and_source_term(dim, n, order, prec):
+    """The same driver on a mesh with moved interior vertices, the plug-in
+    evaluating its scalar_function_t source term at every quadrature point
+    (femhip_space_quadrature_points, then FEMHIP_F_QPOINT values); the nodes
+    of the check are the mapped ones."""
+    assert os.path.exists(SEAMS), "build the Fortran drivers first (fempar_amd.build.build_fortran)"
+    p = subprocess.run([SEAMS, str(dim), str(n), str(order), str(prec), "1"], capture_output=True, text=True,
+                       timeout=120)
+    print(p.stdout, p.stderr)
+    assert p.returncode == 0 and "PASSED" in p.stdout and "moved=1" in p.stdout
