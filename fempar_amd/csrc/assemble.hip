@@ -505,6 +505,7 @@ __global__ __launch_bounds__(64 * kQ2Rows) void k_gather_q2(FaceTab t, int32_t n
 struct SymOut {
   double *v64;
   int64_t stride;
+  int32_t kp;      // stored diagonals
   int8_t kidx[32];
 };
 
@@ -539,7 +540,7 @@ __device__ __forceinline__ void finish_row(const VtxGrid &g, int32_t r, const in
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int d = so.kidx[k];
-      if (d >= 0) so.v64[d * so.stride + r] = sr[k] != 255 ? acc[k] : 0.0;
+      if (d >= 0) so.v64[sym_at(d, r, so.stride, so.kp)] = sr[k] != 255 ? acc[k] : 0.0;
     }
   }
 }
@@ -799,7 +800,7 @@ __global__ __launch_bounds__(kMfmaRows) void k_assemble_sym32_mfma(
       a = out[tid * 17 + d];
       if (sc) a = (float)((double)a * (sr * sc[min(r + (int64_t)so.off[d], (int64_t)n - 1)]));
     }
-    so.v32[(int64_t)d * so.stride + r] = a;
+    so.v32[sym_at(d, r, so.stride, so.nd)] = a;
   }
 }
 
@@ -1084,6 +1085,7 @@ void launch_gather(femhip_operator &op, const PhysDev &ph, const double *d_fixed
   if (op.sym_gather == 1 && op.sl.v64.p) {
     so.v64 = op.sl.v64.p;
     so.stride = op.sl.stride;
+    so.kp = op.sl.kplus;
     for (int k = 0; k < 32; ++k) so.kidx[k] = op.sym_kidx[k];
     op.sym_written = 1;
   }

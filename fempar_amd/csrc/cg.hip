@@ -258,19 +258,18 @@ __device__ __forceinline__ A sym27_row(const SlicedDev &m, const T *__restrict__
     xv[t][2] = lane == kSlice - 1 ? hr : r;
   }
   A acc = 0;
-  const T *vr = v + row;
   // upper part: k = 0 (column row), 1 (row + 1), k >= 2: triplet (k+1)/3, member (k+1)%3
   if (m.unit) acc += (A)xv[0][1];
-  else acc += (A)vr[0] * (A)xv[0][1];
-  acc += (A)vr[m.stride] * (A)xv[0][2];
+  else acc += (A)v[sym_at(0, row, m.stride, 14)] * (A)xv[0][1];
+  acc += (A)v[sym_at(1, row, m.stride, 14)] * (A)xv[0][2];
 #pragma unroll
-  for (int k = 2; k < 14; ++k) acc += (A)vr[(int64_t)k * m.stride] * (A)xv[(k + 1) / 3][(k + 1) % 3];
+  for (int k = 2; k < 14; ++k) acc += (A)v[sym_at(k, row, m.stride, 14)] * (A)xv[(k + 1) / 3][(k + 1) % 3];
   // lower part: column row - o_k: k = 1 -> row - 1; k >= 2 -> the mirrored triplet
 #pragma unroll
   for (int k = 1; k < 14; ++k) {
     const int rr = row - m.soffs[k];
     const int rc = min(max(rr, 0), hi);
-    A a = (A)v[(int64_t)k * m.stride + rc];
+    A a = (A)v[sym_at(k, rc, m.stride, 14)];
     if (rr < 0) a = 0;
     const X xx = k == 1 ? xv[0][0] : xv[(k + 1) / 3 + 4][2 - (k + 1) % 3];
     acc += a * (A)xx;
@@ -284,12 +283,11 @@ __device__ __forceinline__ A sym_rows(const SlicedDev &m, const T *__restrict__ 
   const int row = s * kSlice + lane;
   const int hi = m.n - 1;
   A acc = 0;
-  const T *vr = v + row;
   if (m.kplus == 14) {   // 27-point stencils: every load of the row in flight at once
 #pragma unroll
     for (int k = 0; k < 14; ++k) {
       const int o = m.soffs[k];
-      const A a = (k == 0 && m.unit) ? (A)1 : (A)vr[(int64_t)k * m.stride];
+      const A a = (k == 0 && m.unit) ? (A)1 : (A)v[sym_at(k, row, m.stride, 14)];
       const int c = min(row + o, hi);
       acc += a * (A)g(c);
     }
@@ -298,7 +296,7 @@ __device__ __forceinline__ A sym_rows(const SlicedDev &m, const T *__restrict__ 
       const int o = m.soffs[k];
       const int rr = row - o;
       const int rc = min(max(rr, 0), hi);
-      A a = (A)v[(int64_t)k * m.stride + rc];
+      A a = (A)v[sym_at(k, rc, m.stride, 14)];
       if (rr < 0) a = 0;
       acc += a * (A)g(rc);
     }
@@ -306,7 +304,7 @@ __device__ __forceinline__ A sym_rows(const SlicedDev &m, const T *__restrict__ 
   }
   for (int k = 0; k < m.kplus; ++k) {   // diagonal and upper part
     const int o = m.soffs[k];
-    const A a = (k == 0 && m.unit) ? (A)1 : (A)vr[k * m.stride];
+    const A a = (k == 0 && m.unit) ? (A)1 : (A)v[sym_at(k, row, m.stride, m.kplus)];
     const int c = min(row + o, hi);
     acc += a * (A)g(c);
   }
@@ -314,7 +312,7 @@ __device__ __forceinline__ A sym_rows(const SlicedDev &m, const T *__restrict__ 
     const int o = m.soffs[k];
     const int rr = row - o;
     const int rc = min(max(rr, 0), hi);
-    A a = (A)v[k * m.stride + rc];
+    A a = (A)v[sym_at(k, rc, m.stride, m.kplus)];
     if (rr < 0) a = 0;
     acc += a * (A)g(rc);
   }
@@ -1197,18 +1195,18 @@ __global__ __launch_bounds__(kFillRows) void k_fill_sym(int32_t n, const int32_t
   for (int k = 0; k < kplus; ++k) {
     double a = 0.0;
     if (j < e && sc[j] - (int32_t)r == soffs[k]) a = sv[j++];
-    if (v64) v64[(int64_t)k * stride + r] = a;
-    if (v32) v32[(int64_t)k * stride + r] = (float)a;
+    if (v64) v64[sym_at(k, r, stride, kplus)] = a;
+    if (v32) v32[sym_at(k, r, stride, kplus)] = (float)a;
   }
 }
 
 // Jacobi setup from the symmetric-diagonal copy: diagonal 0 is offset 0.
 // A zero pivot raises the flag (mapped host memory, read after the solve).
-__global__ void k_jacobi_sym(int32_t n, const double *__restrict__ v, double *__restrict__ out, int root,
-                             int *flag) {
+__global__ void k_jacobi_sym(int32_t n, const double *__restrict__ v, int64_t stride, int kplus,
+                             double *__restrict__ out, int root, int *flag) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  jacobi_store(v[i], root, out, i, flag);
+  jacobi_store(v[sym_at(0, i, stride, kplus)], root, out, i, flag);
 }
 
 // FP32 copy of the solver matrix, of S A S when sc (= S) is set:
@@ -1224,7 +1222,7 @@ __global__ void k_scale32(SlicedDev m, const double *__restrict__ v64, const dou
   if (m.sym) {
     const int row = s * kSlice + lane;
     for (int k = 0; k < m.kplus; ++k) {
-      const int64_t e = (int64_t)k * m.stride + row;
+      const int64_t e = sym_at(k, row, m.stride, m.kplus);
       double a = row < m.n ? v64[e] : 0.0;
       if (sc && row < m.n) a *= sc[row] * sc[min(row + m.soffs[k], hi)];
       v32[e] = (float)a;
@@ -1469,7 +1467,8 @@ void prepare(femhip_solver &s) {
     const dim3 G((n + 255) / 256), B(256);
     const int root = mx ? 1 : 0;
     if (op.sl.sym)
-      hipLaunchKernelGGL(k_jacobi_sym, G, B, 0, st, n, op.sl.v64.p, s.invd.p, root, s.d_flag);
+      hipLaunchKernelGGL(k_jacobi_sym, G, B, 0, st, n, op.sl.v64.p, op.sl.stride, op.sl.kplus, s.invd.p, root,
+                         s.d_flag);
     else if (op.structured || op.sl.permuted)   // no CSR in the solver's order: the diagonal of the sliced copy
       hipLaunchKernelGGL(k_jacobi_sliced, dim3((unsigned)(((int64_t)op.sl.ns * kSlice + 255) / 256)), B, 0, st,
                          sliced_dev(op), op.sl.v64.p, s.invd.p, root, s.d_flag);
