@@ -131,7 +131,8 @@ class ServeResult(C.Structure):
 class ServeStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
                 ("queued", C.c_int32), ("active", C.c_int32), ("peak_active", C.c_int32),
-                ("running_peak", C.c_int32), ("hbm_in_use", C.c_double), ("hbm_peak", C.c_double)]
+                ("running_peak", C.c_int32), ("hbm_in_use", C.c_double), ("hbm_peak", C.c_double),
+                ("arena_bytes", C.c_double), ("arena_fallbacks", C.c_int64)]
 
 
 _lib = None

@@ -7,6 +7,10 @@
 namespace femhip {
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
+DevArena *&current_arena() {
+  static thread_local DevArena *a = nullptr;
+  return a;
+}
 }  // namespace femhip
 
 using namespace femhip;

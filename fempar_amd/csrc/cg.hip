@@ -1320,14 +1320,17 @@ struct femhip_solver {
   hipGraphExec_t graphs[2][2] = {};            // [chunk size][result slot]
   int graph_kind = -1;
   bool stream_launches = false;  // never build chunk graphs (set by the request server)
+  bool borrowed_host = false;    // h_res, h_flag and ev belong to the caller (solver_use_host_slots)
   bool prepared = false;
   bool have_solution = false;
   ~femhip_solver() {
-    for (auto *h : h_res)
-      if (h) (void)hipHostFree(h);
-    if (h_flag) (void)hipHostFree(h_flag);
-    for (auto e : ev)
-      if (e) (void)hipEventDestroy(e);
+    if (!borrowed_host) {
+      for (auto *h : h_res)
+        if (h) (void)hipHostFree(h);
+      if (h_flag) (void)hipHostFree(h_flag);
+      for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    }
     for (auto &row : graphs)
       for (auto g : row)
         if (g) (void)hipGraphExecDestroy(g);
@@ -1434,14 +1437,17 @@ void prepare(femhip_solver &s) {
       s.r32.alloc(n); s.p32.alloc(n); s.q32.alloc(n); s.d32.alloc(n);
     }
     for (int k = 0; k < 2; ++k) {   // written by the GPU, read by the host: mapped, coherent
-      FH_HIP(hipHostMalloc((void **)&s.h_res[k], sizeof(CgResult), hipHostMallocMapped | hipHostMallocCoherent));
+      if (!s.borrowed_host)
+        FH_HIP(hipHostMalloc((void **)&s.h_res[k], sizeof(CgResult), hipHostMallocMapped | hipHostMallocCoherent));
       std::memset(s.h_res[k], 0, sizeof(CgResult));
       FH_HIP(hipHostGetDevicePointer((void **)&s.d_res_map[k], s.h_res[k], 0));
     }
-    FH_HIP(hipHostMalloc((void **)&s.h_flag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    if (!s.borrowed_host)
+      FH_HIP(hipHostMalloc((void **)&s.h_flag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
     *s.h_flag = 0;
     FH_HIP(hipHostGetDevicePointer((void **)&s.d_flag, s.h_flag, 0));
-    for (auto &e : s.ev) FH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (!s.borrowed_host)
+      for (auto &e : s.ev) FH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     s.prepared = true;
   }
   // matrix-dependent work, redone whenever the operator was recomputed
@@ -1842,6 +1848,18 @@ double true_residual(femhip_solver &s) {
 int solver_chunk() { return kChunk; }
 
 void solver_stream_launches(femhip_solver &s) { s.stream_launches = true; }
+
+void solver_use_host_slots(femhip_solver &s, const SolverHostSlots &hs) {
+  FH_CHECK(!s.prepared, FEMHIP_ERR_STATE, "host slots must be set before the solver is prepared");
+  for (int k = 0; k < 2; ++k) {
+    s.h_res[k] = static_cast<CgResult *>(hs.h_res[k]);
+    s.ev[k] = hs.ev[k];
+  }
+  s.h_flag = hs.h_flag;
+  s.borrowed_host = true;
+}
+
+size_t solver_result_bytes() { return sizeof(CgResult); }
 
 void solver_start(femhip_solver &s) {
   prepare(s);

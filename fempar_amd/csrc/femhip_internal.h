@@ -32,10 +32,32 @@ struct Error {
   } while (0)
 
 // ------------------------------------------------------- device buffers ---
+// A block of device memory handed out by a bump pointer.  While a thread has
+// one installed (ArenaScope), DevBuf::alloc carves its buffers from it
+// (256-byte aligned) and release gives nothing back: the block's owner reuses
+// it as a whole.  An allocation that does not fit falls back to hipMalloc and
+// is counted.  The request server installs each request's block while it sets
+// the request up, so no hipMalloc or hipFree runs while other requests'
+// kernels are launched.
+struct DevArena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+  size_t misses = 0;
+};
+DevArena *&current_arena();   // per thread
+struct ArenaScope {
+  DevArena *prev;
+  explicit ArenaScope(DevArena *a) : prev(current_arena()) { current_arena() = a; }
+  ~ArenaScope() { current_arena() = prev; }
+  ArenaScope(const ArenaScope &) = delete;
+  ArenaScope &operator=(const ArenaScope &) = delete;
+};
+
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  bool owned = true;   // false: carved from an arena
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
@@ -43,17 +65,31 @@ struct DevBuf {
   void alloc(size_t count) {
     release();
     if (count == 0) count = 1;
-    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    const size_t bytes = count * sizeof(T);
+    if (DevArena *a = current_arena()) {
+      const size_t off = (a->used + 255) & ~(size_t)255;
+      if (off + bytes <= a->size) {
+        p = reinterpret_cast<T *>(a->base + off);
+        a->used = off + bytes;
+        n = count;
+        owned = false;
+        return;
+      }
+      ++a->misses;
+    }
+    hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
       p = nullptr;
-      throw Error{FEMHIP_ERR_OOM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+      throw Error{FEMHIP_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
     }
     n = count;
+    owned = true;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    owned = true;
   }
   size_t bytes() const { return n * sizeof(T); }
 };
@@ -340,6 +376,16 @@ void error_norms(femhip_operator &op, const double *d_x, int exact, double out[4
 // launches instead of hipGraphs (set before solver_start).
 int solver_chunk();
 void solver_stream_launches(femhip_solver &s);
+// Pinned, device-mapped result slots and completion events a solver may use
+// instead of allocating its own (the request server keeps one set per slot
+// and reuses it for every request of that slot).
+struct SolverHostSlots {
+  void *h_res[2] = {nullptr, nullptr};   // CgResult each
+  int *h_flag = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+void solver_use_host_slots(femhip_solver &s, const SolverHostSlots &hs);
+size_t solver_result_bytes();
 void solver_start(femhip_solver &s);
 void solver_push(femhip_solver &s, int slot);
 int solver_poll(femhip_solver &s, int slot);

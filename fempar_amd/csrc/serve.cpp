@@ -14,9 +14,16 @@
 //  * setup workers turn an admitted request into device state on its slot's
 //    context (structured operator built and assembled on the device, Jacobi,
 //    initial residual) and retire finished requests (result,
-//    true residual, solution to the host, device memory freed).
+//    true residual, solution to the host, device block returned).
 // Each slot owns a femhip context with its own non-blocking stream, so a
 // request's kernels, reductions and launch shapes are those of a solve alone.
+// Device memory: the server reserves its HBM budget once, at creation, as one
+// arena; admission hands a request a contiguous block of its estimate (first
+// fit), its buffers are carved from that block while it is set up, and the
+// block goes back to the arena when the request retires.  Each slot also owns
+// the pinned result slots and events its requests' solvers use.  So no
+// hipMalloc, hipFree, hipHostMalloc or hipHostFree runs while requests
+// iterate; they run only at server creation and destruction.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +32,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -54,6 +62,8 @@ struct Req {
   femhip_serve_result_t res{};
   std::vector<double> x;
   std::string error;   // message of a failed setup or retire
+  size_t blk_off = 0, blk_size = 0;   // the request's block of the server's arena (size 0: none)
+  DevArena arena;                     // bump allocator over that block
 };
 using ReqP = std::shared_ptr<Req>;
 
@@ -76,6 +86,12 @@ struct femhip_server {
   bool stop = false;
   std::thread sched;
   std::vector<std::thread> workers;
+  // device arena (the HBM budget, reserved once) and its free ranges by offset
+  char *arena = nullptr;
+  size_t arena_size = 0;
+  std::map<size_t, size_t> free_blocks;
+  int64_t arena_fallbacks = 0;                // allocations that did not fit a request's block
+  std::vector<SolverHostSlots> slot_host;     // per slot: pinned result slots and events
 
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - epoch).count();
@@ -95,6 +111,39 @@ double estimate_bytes(int32_t n, int32_t maxit) {
   const double slices = rows / 64.0;
   const double vecs = 8.0 * rows + 48.0 * rows + 20.0 * rows + 8.0 * std::max(1, maxit);
   return values + 24.0 * slices + vecs + (4 << 20);
+}
+
+constexpr size_t kBlockAlign = 256;
+
+// first-fit block of `bytes` from the arena's free ranges (false: none fits)
+bool arena_take(femhip_server &S, size_t bytes, size_t *off) {
+  for (auto it = S.free_blocks.begin(); it != S.free_blocks.end(); ++it) {
+    if (it->second < bytes) continue;
+    *off = it->first;
+    const size_t rest = it->second - bytes;
+    const size_t at = it->first + bytes;
+    S.free_blocks.erase(it);
+    if (rest > 0) S.free_blocks[at] = rest;
+    return true;
+  }
+  return false;
+}
+
+// give a block back, merged with its free neighbours
+void arena_give(femhip_server &S, size_t off, size_t bytes) {
+  auto next = S.free_blocks.lower_bound(off);
+  if (next != S.free_blocks.end() && off + bytes == next->first) {
+    bytes += next->second;
+    next = S.free_blocks.erase(next);
+  }
+  if (next != S.free_blocks.begin()) {
+    auto prev = std::prev(next);
+    if (prev->first + prev->second == off) {
+      prev->second += bytes;
+      return;
+    }
+  }
+  S.free_blocks[off] = bytes;
 }
 
 void free_request_objects(Req &r) {
@@ -120,10 +169,18 @@ void setup_request(femhip_server &S, Req &r) {
   ph.E = r.rq.E;
   ph.nu = r.rq.nu;
   for (int k = 0; k < 3; ++k) ph.body[k] = r.rq.body[k];
-  r.op = create_structured_elasticity(c, r.rq.n, ph);
-  check_rc(femhip_solver_create(c, r.op, &S.prm.cg, &r.solver));
-  solver_stream_launches(*r.solver);
-  solver_start(*r.solver);
+  {
+    ArenaScope scope(r.blk_size ? &r.arena : nullptr);   // the request's buffers come from its block
+    r.op = create_structured_elasticity(c, r.rq.n, ph);
+    check_rc(femhip_solver_create(c, r.op, &S.prm.cg, &r.solver));
+    solver_stream_launches(*r.solver);
+    solver_use_host_slots(*r.solver, S.slot_host[(size_t)r.slot]);
+    solver_start(*r.solver);
+  }
+  if (r.arena.misses) {
+    std::lock_guard<std::mutex> lk(S.mu);
+    S.arena_fallbacks += (int64_t)r.arena.misses;
+  }
   FH_HIP(hipStreamSynchronize(c->stream));
   r.res.hbm_bytes = (double)(operator_device_bytes(*r.op) + solver_device_bytes(*r.solver));
   r.res.num_dofs = 3 * r.op->nodes;
@@ -163,6 +220,8 @@ void finish_locked(femhip_server &S, Req &r, int status_if_failed) {
   r.state = DONE;
   ++S.completed;
   S.free_slots.push_back(r.slot);
+  if (r.blk_size) arena_give(S, r.blk_off, r.blk_size);   // the slot's stream is idle (worker_loop)
+  r.blk_size = 0;
   S.hbm_in_use -= r.estimate;
   --S.active;
   S.cv_done.notify_all();
@@ -211,7 +270,8 @@ void worker_loop(femhip_server &S) {
       msg = "host allocation failed";
     }
     if (retire || status != FEMHIP_OK) {
-      if (status != FEMHIP_OK && r->solver) (void)hipStreamSynchronize(S.slot_ctx[(size_t)r->slot]->stream);
+      // nothing of the request may still run on its slot before its block is reused
+      (void)hipStreamSynchronize(S.slot_ctx[(size_t)r->slot]->stream);
       free_request_objects(*r);
       std::lock_guard<std::mutex> lk(S.mu);
       if (status != FEMHIP_OK && r->error.empty()) r->error = std::string(retire ? "retire: " : "setup: ") + msg;
@@ -257,6 +317,14 @@ void scheduler_loop(femhip_server &S) {
         if (it == S.pending.end()) break;
         ReqP r = it->second;
         if (S.free_slots.empty() || S.hbm_in_use + r->estimate > S.prm.hbm_budget_bytes) break;
+        if (S.arena) {   // a contiguous block of the estimate, else wait as for the budget
+          const size_t need = ((size_t)r->estimate + kBlockAlign - 1) / kBlockAlign * kBlockAlign;
+          size_t off = 0;
+          if (!arena_take(S, need, &off)) break;
+          r->blk_off = off;
+          r->blk_size = need;
+          r->arena = DevArena{S.arena + off, need, 0, 0};
+        }
         S.pending.erase(it);
         r->slot = S.free_slots.back();
         S.free_slots.pop_back();
@@ -385,11 +453,33 @@ int femhip_serve_create(femhip_ctx_t *ctx, const femhip_serve_params_t *params, 
     S->prm = p;
     S->epoch = std::chrono::steady_clock::now();
     try {
+      // the arena: the budget, or 90 % of the free device memory if that is less
+      // (the budget is then capped at the arena)
+      size_t free_b = 0, total_b = 0;
+      FH_HIP(hipMemGetInfo(&free_b, &total_b));
+      const size_t gran = 2u << 20, cap = (size_t)(0.9 * (double)free_b) / gran * gran;
+      size_t want = (size_t)std::ceil(std::min(p.hbm_budget_bytes, (double)cap) / (double)gran) * gran;
+      want = std::min(want, cap);
+      if (want >= gran && hipMalloc((void **)&S->arena, want) == hipSuccess) {
+        S->arena_size = want;
+        S->free_blocks[0] = want;
+        S->prm.hbm_budget_bytes = std::min(S->prm.hbm_budget_bytes, (double)want);
+      } else {
+        S->arena = nullptr;   // requests then allocate their buffers one by one
+        (void)hipGetLastError();
+      }
       for (int i = 0; i < p.max_active; ++i) {
         femhip_ctx_t *c = nullptr;
         check_rc(femhip_ctx_create(ctx->device, &c));
         S->slot_ctx.push_back(c);
         S->free_slots.push_back(p.max_active - 1 - i);
+        SolverHostSlots hs;
+        for (int k = 0; k < 2; ++k) {
+          FH_HIP(hipHostMalloc(&hs.h_res[k], solver_result_bytes(), hipHostMallocMapped | hipHostMallocCoherent));
+          FH_HIP(hipEventCreateWithFlags(&hs.ev[k], hipEventDisableTiming));
+        }
+        FH_HIP(hipHostMalloc((void **)&hs.h_flag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+        S->slot_host.push_back(hs);
       }
       S->sched = std::thread(scheduler_loop, std::ref(*S));
       for (int i = 0; i < p.setup_threads; ++i) S->workers.emplace_back(worker_loop, std::ref(*S));
@@ -507,6 +597,8 @@ int femhip_serve_stats(femhip_server_t *S, femhip_serve_stats_t *st) {
     st->running_peak = S->running_peak;
     st->hbm_in_use = S->hbm_in_use;
     st->hbm_peak = S->hbm_peak;
+    st->arena_bytes = (double)S->arena_size;
+    st->arena_fallbacks = S->arena_fallbacks;
     return FEMHIP_OK;
   } catch (const Error &e) {
     set_error(e.msg);
@@ -537,6 +629,14 @@ int femhip_serve_destroy(femhip_server_t *S) {
   for (auto &w : S->workers)
     if (w.joinable()) w.join();
   for (auto *c : S->slot_ctx) femhip_ctx_destroy(c);
+  for (auto &hs : S->slot_host) {   // every request retired, every thread joined
+    for (int k = 0; k < 2; ++k) {
+      if (hs.h_res[k]) (void)hipHostFree(hs.h_res[k]);
+      if (hs.ev[k]) (void)hipEventDestroy(hs.ev[k]);
+    }
+    if (hs.h_flag) (void)hipHostFree(hs.h_flag);
+  }
+  if (S->arena) (void)hipFree(S->arena);
   ctx_release(S->ctx);
   delete S;
   return FEMHIP_OK;

@@ -442,6 +442,8 @@ typedef struct {
   int64_t submitted, completed, failed;
   int32_t queued, active, peak_active, running_peak;  /* running_peak: most requests iterating at once */
   double hbm_in_use, hbm_peak;   /* admitted estimates (the budget's currency) */
+  double arena_bytes;            /* device memory reserved at creation for requests (0: none) */
+  int64_t arena_fallbacks;       /* request buffers that did not fit their block (allocated apart) */
 } femhip_serve_stats_t;
 
 typedef struct femhip_server femhip_server_t;

@@ -96,6 +96,9 @@ def test_serve_alone_vs_mixed(env):
         print(f"[serve] stats {st}", flush=True)
         assert st["completed"] == 2 * len(cases) and st["failed"] == 0
         assert st["running_peak"] >= 2              # requests iterated together
+        # every request's buffers came from its block of the arena reserved at
+        # creation: no device allocation while requests iterated
+        assert st["arena_bytes"] > 0 and st["arena_fallbacks"] == 0
     finally:
         srv.free()
 
@@ -117,6 +120,7 @@ def test_serve_admission_budget(env):
         st = srv.stats()
         assert st["peak_active"] == 1 and st["hbm_peak"] <= budget
         assert st["hbm_in_use"] == 0 and st["active"] == 0
+        assert st["arena_bytes"] >= budget and st["arena_fallbacks"] == 0
     finally:
         srv.free()
 
