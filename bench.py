@@ -454,15 +454,15 @@ def main():
     if os.path.exists(tpath):   # HBM bytes per launch from the committed rocprofv3 --pmc runs
         traffic = json.load(open(tpath)).get("FP64", {}).get("hbm_bytes_per_launch")
 
-    mixed = None
-    if not args.no_mixed:
-        s32 = make_solver(F, env, op, "FP32_REFINED", args.rtol, args.delta)
+    def run_mixed(precision):
+        s32 = make_solver(F, env, op, precision, args.rtol, args.delta)
         for _ in range(args.warmup):
             run_steps(op, s32, 1)
         ta32, tc32, wall32, r32 = run_steps(op, s32, args.steps)
         x32 = s32.get_solution()
         sp32_ms, sp32_bytes = s32.profile_spmv(50)
-        mixed = {
+        res = {
+            "precision": precision,
             "value": fes.num_free_dofs / float(np.mean(ta32 + tc32)),
             "ms_per_step": float(np.mean(wall32) * 1e3),
             "t_assemble_ms": float(np.mean(ta32) * 1e3), "t_cg_ms": float(np.mean(tc32) * 1e3),
@@ -473,6 +473,13 @@ def main():
             "spmv_fp32_GBs": sp32_bytes / (sp32_ms * 1e-3) / 1e9,
             "error_vs_u": s32.error_norms("sin"),
         }
+        s32.free()
+        return res
+
+    mixed = None if args.no_mixed else run_mixed("FP32_REFINED")
+    # the FP32 matrix integrated in FP32 on the matrix cores and summed in FP32,
+    # refined against the same FP64 residual
+    mixed_strict = None if args.no_mixed else run_mixed("FP32_STRICT")
 
     batched = None if args.no_batched else run_batched(F, env, args.steps)
     serving = None if args.no_serving else run_serving(F, env, args.serve_requests)
@@ -508,6 +515,7 @@ def main():
                      "kernel_ms": spmv_ms, "algorithmic_bytes": spmv_bytes,
                      "kernel_ms_in_solve_events": insolve_ms, "in_solve_launches": insolve_its},
         "mixed": mixed,
+        "mixed_strict": mixed_strict,
         "batched": batched,
         "serving": serving,
         "configs": configs,

@@ -21,7 +21,7 @@ K_CONST, K_SIN3, K_CELL, K_QPOINT = 0, 1, 2, 3
 F_CONST, F_MANUFACTURED, F_CELL_SIN, F_QPOINT = 0, 1, 2, 3
 G_ZERO, G_LINEAR = 0, 1
 PREC_IDENTITY, PREC_JACOBI = 0, 1
-PRECISION_FP64, PRECISION_FP32_REFINED = 0, 1
+PRECISION_FP64, PRECISION_FP32_REFINED, PRECISION_FP32_STRICT = 0, 1, 2
 
 # iterative_solver_parameters.f90:37-46, in the C-ABI's numbering
 STOPPING_CRITERIA = ["RES_NRMGIVEN_RHS_NRMGIVEN", "RES_NRMGIVEN_RES_NRMGIVEN", "DELTA_RHS", "DELTA_DELTA",

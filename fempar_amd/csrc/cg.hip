@@ -705,6 +705,9 @@ __global__ __launch_bounds__(kThreads) void k_ru_init_fin(const double *part, in
 // must stay resident for the FP32 iterations (4 waves per SIMD: at most 128
 // VGPRs; the second bound is waves per execution unit); the rare
 // FP64 update branch, which alone would take more, absorbs the limit.
+// A: the type the FP32 matrix rows are summed in (double by default: 259
+// inner iterations at 128^3 against 281 with float; float for FP32_STRICT).
+template <typename A>
 __global__ __launch_bounds__(kThreads, 4) void k_ru_spmv(SlicedDev m, SlicedDev m32, const float *__restrict__ v32,
                                                       const double *__restrict__ v64, const float *__restrict__ p,
                                                       float *__restrict__ q, const double *__restrict__ b,
@@ -722,7 +725,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_ru_spmv(SlicedDev m, SlicedDev 
     for (int s = s0; s < s1; s += sstep) {
       // FP32 values and vectors, the row sum held in FP64 registers: 8 % fewer
       // inner iterations at 128^3 (259 against 281), no extra memory traffic
-      const float a = (float)slice_row<float, double>(m32, v32, p, s, lane);
+      const float a = (float)slice_row<float, A>(m32, v32, p, s, lane);
       const int row = slice_row_index(m, s, lane);
       if (row < m.n) {
         q[row] = a;
@@ -932,6 +935,7 @@ __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restric
 // State parity: slot A reads S[par] and writes S[par ^ 1]; slot B reads
 // S[par ^ 1], its block 0 stores alpha there for the next slot A.
 
+template <typename A>
 __global__ __launch_bounds__(kThreads, 4) void k_rf_a(SlicedDev m, SlicedDev m32, const float *__restrict__ v32,
                                                    const float *__restrict__ r, float *__restrict__ p,
                                                    float *__restrict__ q, float *__restrict__ d,
@@ -1030,7 +1034,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_rf_a(SlicedDev m, SlicedDev m32
   float pq = 0.f, pp = 0.f;
   for (int s = s0; s < s1; s += sstep) {
     // FP32 values and vectors, the row sum held in FP64 registers
-    const float w = (float)slice_row<float, double>(m32, v32, r, s, lane);
+    const float w = (float)slice_row<float, A>(m32, v32, r, s, lane);
     const int row = slice_row_index(m, s, lane);
     if (row < m.n) {
       const float ri = r[row];
@@ -1371,7 +1375,10 @@ SlicedDev sliced_dev(femhip_operator &op) {
   return m;
 }
 
-bool mixed(const femhip_solver &s) { return s.prm.precision == FEMHIP_PRECISION_FP32_REFINED; }
+bool mixed(const femhip_solver &s) { return s.prm.precision != FEMHIP_PRECISION_FP64; }
+// FP32_STRICT: FP32 matrix integrated in FP32 on the matrix cores where that
+// applies, FP32 row sums in the inner iterations
+bool strict(const femhip_solver &s) { return s.prm.precision == FEMHIP_PRECISION_FP32_STRICT; }
 
 // The FP32 matrix as the kernels see it: with Jacobi it holds S A S, whose unit
 // diagonal the symmetric layout does not read.
@@ -1483,11 +1490,12 @@ void prepare(femhip_solver &s) {
     FH_HIP(hipGetLastError());
   }
   // FP32 copy: of S A S with Jacobi, of A without
-  if (mx && (op.sl.filled_version32 != op.version || op.sl.scaled32 != jac)) {
+  const bool int32 = strict(s) || fp32_integration();
+  if (mx && (op.sl.filled_version32 != op.version || op.sl.scaled32 != jac || op.sl.wants_int32 != int32)) {
     if (!op.sl.v32.p) op.sl.v32.alloc((size_t)std::max<int64_t>(1, op.sl.entries));
     // rounded from the FP64 copy, or integrated in FP32 on the matrix cores
     // where asked for and supported (scalar Q1 on the symmetric-diagonal copy)
-    if (!fp32_integration() || !assemble_sym32(op, jac ? s.invd.p : nullptr, op.sl.v32.p)) {
+    if (!int32 || !assemble_sym32(op, jac ? s.invd.p : nullptr, op.sl.v32.p)) {
       const int64_t threads = (int64_t)op.sl.ns * kSlice;
       hipLaunchKernelGGL(k_scale32, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, sliced_dev(op),
                          op.sl.v64.p, jac ? s.invd.p : nullptr, op.sl.v32.p);
@@ -1495,6 +1503,7 @@ void prepare(femhip_solver &s) {
     }
     op.sl.filled_version32 = op.version;
     op.sl.scaled32 = jac;
+    op.sl.wants_int32 = int32;
   }
 }
 
@@ -1568,16 +1577,24 @@ void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = 
   } else if (!three_kernel_refined(s)) {
     const double *sc = jac ? s.invd.p : nullptr;   // S = diag^(-1/2)
     const SlicedDev m32 = fp32_view(s, m);
-    L(k_rf_a, GM, B, m, m32, op.sl.v32.p, s.r32.p, s.p32.p, s.q32.p, s.d32.p, s.x.p, sc, s.st.p, par, s.part_u.p,
-      s.g_mat, s.part_s.p, s.res.p, rh);
+    if (strict(s))
+      L(k_rf_a<float>, GM, B, m, m32, op.sl.v32.p, s.r32.p, s.p32.p, s.q32.p, s.d32.p, s.x.p, sc, s.st.p, par,
+        s.part_u.p, s.g_mat, s.part_s.p, s.res.p, rh);
+    else
+      L(k_rf_a<double>, GM, B, m, m32, op.sl.v32.p, s.r32.p, s.p32.p, s.q32.p, s.d32.p, s.x.p, sc, s.st.p, par,
+        s.part_u.p, s.g_mat, s.part_s.p, s.res.p, rh);
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
     L(k_rf_b, GM, B, m, op.sl.v64.p, s.b.p, s.x.p, s.r32.p, s.q32.p, sc, s.st.p, par, s.part_s.p, s.g_mat,
       s.part_u.p);
   } else {
     const double *sc = jac ? s.invd.p : nullptr;   // S = diag^(-1/2)
     const SlicedDev m32 = fp32_view(s, m);
-    L(k_ru_spmv, GM, B, m, m32, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.r32.p, sc, s.st.p, par,
-      s.part_s.p);
+    if (strict(s))
+      L(k_ru_spmv<float>, GM, B, m, m32, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.r32.p, sc,
+        s.st.p, par, s.part_s.p);
+    else
+      L(k_ru_spmv<double>, GM, B, m, m32, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.r32.p, sc,
+        s.st.p, par, s.part_s.p);
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
     L(k_ru_update, GV, B, n, s.r32.p, s.p32.p, s.q32.p, s.st.p, par, s.part_s.p, s.g_mat, s.part_u.p);
     L(k_ru_dir, GV, B, n, s.d32.p, s.p32.p, s.r32.p, s.x.p, sc, s.st.p, par, s.part_u.p, s.g_vec, s.res.p, rh);
@@ -1914,7 +1931,8 @@ int femhip_solver_create(femhip_ctx_t *ctx, femhip_operator_t *op, const femhip_
              "unknown stopping criterion");
     FH_CHECK(p.preconditioner == FEMHIP_PREC_IDENTITY || p.preconditioner == FEMHIP_PREC_JACOBI,
              FEMHIP_ERR_INVALID, "unknown preconditioner");
-    FH_CHECK(p.precision == FEMHIP_PRECISION_FP64 || p.precision == FEMHIP_PRECISION_FP32_REFINED,
+    FH_CHECK(p.precision == FEMHIP_PRECISION_FP64 || p.precision == FEMHIP_PRECISION_FP32_REFINED ||
+                 p.precision == FEMHIP_PRECISION_FP32_STRICT,
              FEMHIP_ERR_INVALID, "unknown precision");
     FH_CHECK(p.max_num_iterations >= 1, FEMHIP_ERR_INVALID, "max_num_iterations must be >= 1");
     FH_CHECK(p.rtol >= 0 && p.atol >= 0, FEMHIP_ERR_INVALID, "negative tolerance");

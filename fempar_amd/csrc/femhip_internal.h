@@ -211,6 +211,7 @@ struct Sliced {
   DevBuf<float> v32;
   int64_t filled_version = -1, filled_version32 = -1;
   bool scaled32 = false;          // v32 holds S A S (S = diag^(-1/2)) rather than A
+  bool wants_int32 = false;       // v32 was asked for integrated in FP32 (FP32_STRICT)
   bool built = false;
   // Solver rows: the operator's rows, or (permuted) the class-blocked order of
   // a structured scalar space of order >= 2, padded with identity rows; perm

@@ -38,7 +38,7 @@ ILS_ATOL = "ILS_ATOL"
 ILS_STOPPING_CRITERIUM = "ILS_STOPPING_CRITERIUM"
 ILS_MAX_NUM_ITERATIONS = "ILS_MAX_NUM_ITERATIONS"
 # keys of this implementation's refined mode (no reference counterpart)
-ILS_PRECISION = "FEMHIP_PRECISION"          # "FP64" | "FP32_REFINED"
+ILS_PRECISION = "FEMHIP_PRECISION"          # "FP64" | "FP32_REFINED" | "FP32_STRICT"
 ILS_INNER_RTOL = "FEMHIP_INNER_RTOL"
 ILS_MAX_OUTER = "FEMHIP_MAX_OUTER"
 
@@ -398,8 +398,8 @@ class iterative_linear_solver_t:
             p.max_num_iterations = int(self.params[ILS_MAX_NUM_ITERATIONS])
             p.stopping_criterion = N.STOPPING_CRITERIA.index(self.params[ILS_STOPPING_CRITERIUM])
             p.preconditioner = getattr(self.M, "kind", N.PREC_IDENTITY)
-            p.precision = {"FP64": N.PRECISION_FP64, "FP32_REFINED": N.PRECISION_FP32_REFINED}[
-                self.params[ILS_PRECISION]]
+            p.precision = {"FP64": N.PRECISION_FP64, "FP32_REFINED": N.PRECISION_FP32_REFINED,
+                           "FP32_STRICT": N.PRECISION_FP32_STRICT}[self.params[ILS_PRECISION]]
             p.inner_rtol = float(self.params[ILS_INNER_RTOL])
             p.max_outer = int(self.params[ILS_MAX_OUTER])
             h = C.c_void_p()
@@ -413,7 +413,8 @@ class iterative_linear_solver_t:
         p.max_num_iterations = int(self.params[ILS_MAX_NUM_ITERATIONS])
         p.stopping_criterion = N.STOPPING_CRITERIA.index(self.params[ILS_STOPPING_CRITERIUM])
         p.preconditioner = getattr(self.M, "kind", N.PREC_IDENTITY)
-        p.precision = {"FP64": N.PRECISION_FP64, "FP32_REFINED": N.PRECISION_FP32_REFINED}[self.params[ILS_PRECISION]]
+        p.precision = {"FP64": N.PRECISION_FP64, "FP32_REFINED": N.PRECISION_FP32_REFINED,
+                       "FP32_STRICT": N.PRECISION_FP32_STRICT}[self.params[ILS_PRECISION]]
         p.inner_rtol = float(self.params[ILS_INNER_RTOL])
         p.max_outer = int(self.params[ILS_MAX_OUTER])
         return p

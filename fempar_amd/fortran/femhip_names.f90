@@ -24,7 +24,8 @@ module femhip_names
   integer(c_int32_t), parameter, public :: FEMHIP_G_ZERO = 0, FEMHIP_G_LINEAR = 1
   integer(c_int32_t), parameter, public :: FEMHIP_RES_RES = 4, FEMHIP_RES_RHS = 5
   integer(c_int32_t), parameter, public :: FEMHIP_PREC_IDENTITY = 0, FEMHIP_PREC_JACOBI = 1
-  integer(c_int32_t), parameter, public :: FEMHIP_PRECISION_FP64 = 0, FEMHIP_PRECISION_FP32_REFINED = 1
+  integer(c_int32_t), parameter, public :: FEMHIP_PRECISION_FP64 = 0, FEMHIP_PRECISION_FP32_REFINED = 1, &
+                                            FEMHIP_PRECISION_FP32_STRICT = 2
 
   type, bind(c), public :: femhip_mesh_desc_t
     integer(c_int32_t) :: dim = 3

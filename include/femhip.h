@@ -82,6 +82,9 @@ extern "C" {
 
 #define FEMHIP_PRECISION_FP64        0   /* plain drop-in path */
 #define FEMHIP_PRECISION_FP32_REFINED 1  /* FP32 inner PCG inside FP64 iterative refinement */
+#define FEMHIP_PRECISION_FP32_STRICT  2  /* FP32_REFINED with the FP32 matrix integrated in FP32 on the
+                                            matrix cores (scalar Q1, symmetric-diagonal layout; rounded
+                                            from FP64 elsewhere) and FP32 row sums in the inner products */
 
 /* ---- opaque handles ---- */
 typedef struct femhip_ctx femhip_ctx_t;

@@ -247,6 +247,36 @@ def test_fp32_integration_on_matrix_cores(env, name, monkeypatch):
     assert its["rounded"] - 2 <= its["mfma"] <= its["rounded"] * 1.3 + 2, its
 
 
+@pytest.mark.parametrize("name", ["2d_q1_64_f1", "3d_q1_20_sin3", "3d_q1_cellk", "3d_q2_8_sin3"])
+def test_fp32_strict_reaches_fp64_accuracy(env, name):
+    """FEMHIP_PRECISION = FP32_STRICT: the FP32 matrix integrated in FP32 on
+    the matrix cores (scalar Q1; rounded from FP64 for Q2) and its rows summed
+    in FP32, inside the same FP64 refinement.  Solved on one operator by turns
+    with an FP32_REFINED solver, so that each solve must make the operator's
+    FP32 copy its own way again: both reach the oracle's FP64 solution within
+    1e-8 with a true FP64 residual within rtol ||b||, and the strict solve takes
+    at most 40 % more inner iterations (FP32 row sums of 7 or 27 terms add a
+    few ulps per product; a wrong accumulator or a stale copy of the other
+    kind costs far more or fails to converge)."""
+    print(f"[fp32-strict] {name}", flush=True)
+    osp, oph, fes, op, di = _build(env, name)
+    op.compute()
+    irp, ja, val, rhs = osp.assemble(oph, True)
+    x_ref, info = O.cg(irp, ja, val, rhs, True, precond="jacobi", rtol=1e-12, maxit=20000)
+    bn = np.linalg.norm(rhs)
+    solvers = {p: _solver(env, op, p) for p in ("FP32_STRICT", "FP32_REFINED")}
+    its = {}
+    for p in ("FP32_STRICT", "FP32_REFINED", "FP32_STRICT"):
+        x = solvers[p].solve(rhs)
+        r = solvers[p].result
+        assert r["did_converge"] and r["true_residual"] <= 1.0001e-10 * bn, p
+        assert np.linalg.norm(x - x_ref) <= 1e-8 * np.linalg.norm(x_ref), p
+        assert its.setdefault(p, r["num_iterations"]) == r["num_iterations"], p   # the second strict solve repeats
+    for s in solvers.values():
+        s.free()
+    assert its["FP32_STRICT"] <= its["FP32_REFINED"] * 1.4 + 2, its
+
+
 def _solver(env, op, precision):
     s = F.iterative_linear_solver_t().create(env)
     s.set_type_from_string("CG")
