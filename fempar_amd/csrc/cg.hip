@@ -935,6 +935,8 @@ __global__ __launch_bounds__(kThreads) void k_ru_dir(int32_t n, float *__restric
 // State parity: slot A reads S[par] and writes S[par ^ 1]; slot B reads
 // S[par ^ 1], its block 0 stores alpha there for the next slot A.
 
+constexpr int kRfbGroup = 8;   // slices of a wave whose r and q loads k_rf_b keeps in flight together
+
 template <typename A>
 __global__ __launch_bounds__(kThreads, 4) void k_rf_a(SlicedDev m, SlicedDev m32, const float *__restrict__ v32,
                                                    const float *__restrict__ r, float *__restrict__ p,
@@ -1055,6 +1057,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_rf_a(SlicedDev m, SlicedDev m32
   write_partials<2>(out, part);
 }
 
+// SYM: the symmetric-diagonal copy, whose slice s holds rows 64 s .. 64 s + 63
+// (no row index is loaded, so nothing but r and q is waited for)
+template <bool SYM>
 __global__ __launch_bounds__(kThreads, 4) void k_rf_b(SlicedDev m, const double *__restrict__ v64,
                                                    const double *__restrict__ b, const double *__restrict__ x,
                                                    float *__restrict__ r, const float *__restrict__ q,
@@ -1081,6 +1086,25 @@ __global__ __launch_bounds__(kThreads, 4) void k_rf_b(SlicedDev m, const double 
     write_partials<2>(acc, part);
     return;
   }
+  // the wave's slices kRfbGroup at a time, all their loads of r and q issued
+  // before the first is used; the first group's are in flight while the
+  // partials are reduced
+  int rows[kRfbGroup];
+  float rv[kRfbGroup], qv[kRfbGroup];
+  auto load = [&](int sb) {
+#pragma unroll
+    for (int k = 0; k < kRfbGroup; ++k) {
+      const int s = sb + k * sstep;
+      rows[k] = s >= s1 ? m.n : SYM ? s * kSlice + lane : slice_row_index(m, s, lane);
+      rv[k] = 0.0f;
+      qv[k] = 0.0f;
+      if (rows[k] < m.n) {
+        rv[k] = r[rows[k]];
+        qv[k] = q[rows[k]];
+      }
+    }
+  };
+  load(s0);
   double red[2];
   reduce_partials<2>(part_a, ga, red);
   const double alpha = (red[0] != 0.0) ? st.rz / red[0] : 0.0;   // cg.f90:195-199
@@ -1091,13 +1115,17 @@ __global__ __launch_bounds__(kThreads, 4) void k_rf_b(SlicedDev m, const double 
   }
   const float a = (float)alpha;
   float rr = 0.f;
-  for (int s = s0; s < s1; s += sstep) {
-    const int row = slice_row_index(m, s, lane);
-    if (row < m.n) {
-      const float ri = r[row] + (-a) * q[row];
-      r[row] = ri;
-      rr += ri * ri;
-    }
+  for (int sb = s0; sb < s1; sb += kRfbGroup * sstep) {
+    if (sb != s0) load(sb);
+    // every new value before the first store, so the stores wait for nothing
+#pragma unroll
+    for (int k = 0; k < kRfbGroup; ++k) rv[k] = rv[k] + (-a) * qv[k];
+#pragma unroll
+    for (int k = 0; k < kRfbGroup; ++k)
+      if (rows[k] < m.n) {
+        r[rows[k]] = rv[k];
+        rr += rv[k] * rv[k];
+      }
   }
   acc[0] = rr;
   write_partials<2>(acc, part);
@@ -1379,6 +1407,13 @@ bool mixed(const femhip_solver &s) { return s.prm.precision != FEMHIP_PRECISION_
 // FP32_STRICT: FP32 matrix integrated in FP32 on the matrix cores where that
 // applies, FP32 row sums in the inner iterations
 bool strict(const femhip_solver &s) { return s.prm.precision == FEMHIP_PRECISION_FP32_STRICT; }
+// FP32 row sums in the refined iterations: always for FP32_STRICT, for
+// FP32_REFINED when FEMHIP_RF_SUM32=1 (faster slots, more inner iterations at
+// 128^3 Q1, fewer milliseconds on large high-contrast fields).  Read per call.
+bool sum32(const femhip_solver &s) {
+  const char *e = getenv("FEMHIP_RF_SUM32");
+  return strict(s) || (e && atoi(e) == 1);
+}
 
 // The FP32 matrix as the kernels see it: with Jacobi it holds S A S, whose unit
 // diagonal the symmetric layout does not read.
@@ -1577,19 +1612,19 @@ void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = 
   } else if (!three_kernel_refined(s)) {
     const double *sc = jac ? s.invd.p : nullptr;   // S = diag^(-1/2)
     const SlicedDev m32 = fp32_view(s, m);
-    if (strict(s))
+    if (sum32(s))
       L(k_rf_a<float>, GM, B, m, m32, op.sl.v32.p, s.r32.p, s.p32.p, s.q32.p, s.d32.p, s.x.p, sc, s.st.p, par,
         s.part_u.p, s.g_mat, s.part_s.p, s.res.p, rh);
     else
       L(k_rf_a<double>, GM, B, m, m32, op.sl.v32.p, s.r32.p, s.p32.p, s.q32.p, s.d32.p, s.x.p, sc, s.st.p, par,
         s.part_u.p, s.g_mat, s.part_s.p, s.res.p, rh);
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
-    L(k_rf_b, GM, B, m, op.sl.v64.p, s.b.p, s.x.p, s.r32.p, s.q32.p, sc, s.st.p, par, s.part_s.p, s.g_mat,
+    L(m.sym ? k_rf_b<true> : k_rf_b<false>, GM, B, m, op.sl.v64.p, s.b.p, s.x.p, s.r32.p, s.q32.p, sc, s.st.p, par, s.part_s.p, s.g_mat,
       s.part_u.p);
   } else {
     const double *sc = jac ? s.invd.p : nullptr;   // S = diag^(-1/2)
     const SlicedDev m32 = fp32_view(s, m);
-    if (strict(s))
+    if (sum32(s))
       L(k_ru_spmv<float>, GM, B, m, m32, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.r32.p, sc,
         s.st.p, par, s.part_s.p);
     else
