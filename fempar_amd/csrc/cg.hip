@@ -1093,9 +1093,12 @@ __global__ __launch_bounds__(kThreads, 4) void k_rf_b(SlicedDev m, const double 
   float rv[kRfbGroup], qv[kRfbGroup];
   auto load = [&](int sb) {
 #pragma unroll
-    for (int k = 0; k < kRfbGroup; ++k) {
+    for (int k = 0; k < kRfbGroup; ++k) {   // row indices first: on the general layouts they are loads
       const int s = sb + k * sstep;
       rows[k] = s >= s1 ? m.n : SYM ? s * kSlice + lane : slice_row_index(m, s, lane);
+    }
+#pragma unroll
+    for (int k = 0; k < kRfbGroup; ++k) {
       rv[k] = 0.0f;
       qv[k] = 0.0f;
       if (rows[k] < m.n) {
@@ -1385,7 +1388,7 @@ SlicedDev sliced_dev(femhip_operator &op) {
   m.kplus = op.sl.kplus;
   m.stride = op.sl.stride;
   m.soffs = op.sl.soffs.p;
-  m.srow = op.sl.srow.p;
+  m.srow = op.sl.srow.n > 1 ? op.sl.srow.p : nullptr;   // null: no permuted slice, rows need no load
   // symmetric layout: pencils of one plane (the largest stored offset) per step
   // for order 2; measured inside CG solves order 2 and order 1 run the SpMV
   // equally fast and order 1 gives the faster FP32 iterations, so 1 is default
