@@ -162,6 +162,7 @@ __device__ __forceinline__ void bsum(double (&v)[NV], double *sm) {
   __syncthreads();
   for (int j = 0; j < NV; ++j) {
     double s = 0.0;
+#pragma unroll 4   // a full unroll at 16 waves holds 16 NV values and spills the 1024-thread solve
     for (int i = 0; i < BS / 64; ++i) s += sm[j * (BS / 64) + i];
     v[j] = s;
   }

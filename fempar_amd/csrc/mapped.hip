@@ -302,6 +302,7 @@ __global__ void k_qpoints(MapMesh m, int64_t ncells, const double *__restrict__ 
 // its reference position xi = (2 / order) m - 1 (sbm_serial_fe_space.i90:
 // 997-1176); taken from the last cell touching the node, as the reference's
 // cell loop leaves the last write
+template <int DIM>
 __global__ void k_interp_mapped(FaceTab t, MapMesh m, int64_t ncells, const int32_t *__restrict__ dofs,
                                 const double *__restrict__ vc, double *__restrict__ out) {
 #pragma clang fp contract(off)
@@ -309,18 +310,16 @@ __global__ void k_interp_mapped(FaceTab t, MapMesh m, int64_t ncells, const int3
   if (c >= ncells) return;
   int ijk[3];
   cell_ijk(t, c, ijk);
-  double xv[8][3];
-  const int nv = 1 << t.dim;
-  for (int a = 0; a < nv; ++a) {
-    if (t.dim == 2) cell_vertex<2>(m, vc, c, a, xv[a]);
-    else cell_vertex<3>(m, vc, c, a, xv[a]);
-  }
+  constexpr int nv = 1 << DIM;   // the vertex coordinates stay in registers
+  double xv[nv][3];
+#pragma unroll
+  for (int a = 0; a < nv; ++a) cell_vertex<DIM>(m, vc, c, a, xv[a]);
   for (int l = 0; l < t.nd; ++l) {
     const int32_t d = dofs[c * t.nd + l];
     if (d >= 0) continue;
     const int f = t.face_of[l];
     bool last = true;
-    for (int k = 0; k < t.dim; ++k) {
+    for (int k = 0; k < DIM; ++k) {
       if ((t.type[f] >> k) & 1) continue;
       const int w = ijk[k] + ((t.anchor[f] >> k) & 1);
       last = last && ijk[k] == min(w, t.n[k] - 1);
@@ -328,19 +327,24 @@ __global__ void k_interp_mapped(FaceTab t, MapMesh m, int64_t ncells, const int3
     if (!last) continue;
     int rem = l % t.nnode;
     double xi[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < t.dim; ++k) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
       const int mm = rem % t.np1;
       rem /= t.np1;
       xi[k] = (2.0 / t.order) * mm - 1.0;
     }
     double x[3] = {0.0, 0.0, 0.0};
+#pragma unroll
     for (int a = 0; a < nv; ++a) {
       double N = 1.0;
-      for (int k = 0; k < t.dim; ++k) N *= ((a >> k) & 1) ? 0.5 * (1.0 + xi[k]) : 0.5 * (1.0 - xi[k]);
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) N *= ((a >> k) & 1) ? 0.5 * (1.0 + xi[k]) : 0.5 * (1.0 - xi[k]);
+#pragma unroll
       for (int k = 0; k < 3; ++k) x[k] += xv[a][k] * N;
     }
     double v = 0.0;
-    for (int k = 0; k < t.dim; ++k) v += x[k];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) v += x[k];
     out[-d - 1] = v;
   }
 }
@@ -513,8 +517,8 @@ void interpolate_dirichlet_mapped(const femhip_space &s, double *out_host) {
   hipStream_t st = s.ctx->stream;
   DevBuf<double> out;
   out.alloc((size_t)s.n_fixed);
-  hipLaunchKernelGGL(k_interp_mapped, dim3(grid_of(s.ncells, 256)), dim3(256), 0, st, face_tab(s), map_mesh(s),
-                     s.ncells, s.d_dofs.p, s.mapped ? s.vcoords.p : nullptr, out.p);
+  hipLaunchKernelGGL(s.fe.dim == 2 ? k_interp_mapped<2> : k_interp_mapped<3>, dim3(grid_of(s.ncells, 256)), dim3(256),
+                     0, st, face_tab(s), map_mesh(s), s.ncells, s.d_dofs.p, s.mapped ? s.vcoords.p : nullptr, out.p);
   FH_HIP(hipGetLastError());
   FH_HIP(hipMemcpyAsync(out_host, out.p, sizeof(double) * (size_t)s.n_fixed, hipMemcpyDeviceToHost, st));
   FH_HIP(hipStreamSynchronize(st));
