@@ -87,6 +87,7 @@ struct SlicedDev {
   int32_t pstride;             // order 2: slices per step of a wave (about one plane)
   int32_t trip;                // symmetric 27-point offsets in column triplets (sym27_row)
   int32_t unit;                // symmetric layout: the diagonal is 1 and is not read
+  int32_t alt;                 // CG SpMVs: odd iterations sweep the slices in reverse
 };
 
 // Row of lane `lane` of slice s (meta bit 28: the slice's rows are a sorted
@@ -466,13 +467,15 @@ __global__ __launch_bounds__(kThreads) void k_init_fin(const double *part, int g
 // q = A p ; partials <p,q>, ||p||^2
 template <typename T, typename A>
 __global__ __launch_bounds__(kThreads) void k_spmv(SlicedDev m, const T *__restrict__ v, const T *__restrict__ p,
-                                                   T *__restrict__ q, const CgState *S, int par, double *part) {
+                                                   T *__restrict__ q, const CgState *S, int par, double *part,
+                                                   int rev) {
   if (S[par].done) return;
   const int lane = threadIdx.x & 63;
   const WaveRange wr = wave_slices(m.ns, m.order, m.pstride);
   const int s0 = wr.s0, s1 = wr.s1, sstep = wr.step;
   A pq = 0, pp = 0;
-  for (int s = s0; s < s1; s += sstep) {
+  for (int t = s0; t < s1; t += sstep) {
+    const int s = rev ? m.ns - 1 - t : t;
     const A acc = slice_row<T, A, 25>(m, v, p, s, lane);
     const int row = slice_row_index(m, s, lane);
     if (row < m.n) {
@@ -713,7 +716,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_ru_spmv(SlicedDev m, SlicedDev 
                                                       float *__restrict__ q, const double *__restrict__ b,
                                                       const double *__restrict__ x, float *__restrict__ r32,
                                                       const double *__restrict__ sc, const CgState *S, int par,
-                                                      double *part) {
+                                                      double *part, int rev) {
   const CgState &cur = S[par];
   if (cur.done) return;
   const int lane = threadIdx.x & 63;
@@ -722,7 +725,8 @@ __global__ __launch_bounds__(kThreads, 4) void k_ru_spmv(SlicedDev m, SlicedDev 
   double acc[3] = {0, 0, 0};
   if (!cur.ru) {
     float pq = 0.f, pp = 0.f;
-    for (int s = s0; s < s1; s += sstep) {
+    for (int t = s0; t < s1; t += sstep) {
+      const int s = rev ? m.ns - 1 - t : t;
       // FP32 values and vectors, the row sum held in FP64 registers: 8 % fewer
       // inner iterations at 128^3 (259 against 281), no extra memory traffic
       const float a = (float)slice_row<float, A>(m32, v32, p, s, lane);
@@ -943,7 +947,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_rf_a(SlicedDev m, SlicedDev m32
                                                    float *__restrict__ q, float *__restrict__ d,
                                                    double *__restrict__ x, const double *__restrict__ sc,
                                                    CgState *S, int par, const double *part_b, int gb, double *part,
-                                                   CgResult *R, CgResult *Rh) {
+                                                   CgResult *R, CgResult *Rh, int rev) {
   const CgState &cur = S[par];
   if (cur.done) {   // keep `done` sticky in both parity slots
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1034,7 +1038,8 @@ __global__ __launch_bounds__(kThreads, 4) void k_rf_a(SlicedDev m, SlicedDev m32
   const float bt = (float)beta;
   const bool first = cur.first;
   float pq = 0.f, pp = 0.f;
-  for (int s = s0; s < s1; s += sstep) {
+  for (int t = s0; t < s1; t += sstep) {
+    const int s = rev ? m.ns - 1 - t : t;
     // FP32 values and vectors, the row sum held in FP64 registers
     const float w = (float)slice_row<float, A>(m32, v32, r, s, lane);
     const int row = slice_row_index(m, s, lane);
@@ -1389,6 +1394,13 @@ SlicedDev sliced_dev(femhip_operator &op) {
   m.stride = op.sl.stride;
   m.soffs = op.sl.soffs.p;
   m.srow = op.sl.srow.n > 1 ? op.sl.srow.p : nullptr;   // null: no permuted slice, rows need no load
+  // the CG SpMVs of odd iterations sweep the slices in reverse, so each starts
+  // on the part of the matrix the previous one read last, which is the part
+  // still in the Infinity Cache (FEMHIP_SPMV_ALT=0: every sweep forward).
+  // FP32-refined CG: 141.5 -> 134.8 ms at 192^3 (random field), 85.4 -> 81.9 ms
+  // at 64^3 Q2
+  static const char *alt = getenv("FEMHIP_SPMV_ALT");
+  m.alt = alt ? atoi(alt) : 1;
   // symmetric layout: pencils of one plane (the largest stored offset) per step
   // for order 2; measured inside CG solves order 2 and order 1 run the SpMV
   // equally fast and order 1 gives the faster FP32 iterations, so 1 is default
@@ -1608,31 +1620,37 @@ void enqueue_one(femhip_solver &s, int par, Launcher &L, const hipEvent_t *ev = 
   if (ev) FH_HIP(hipEventRecord(ev[0], L.stream));
   if (!mixed(s)) {
     const double *invd = jac ? s.invd.p : nullptr;
-    L(k_spmv<double, double>, GM, B, m, op.sl.v64.p, s.p.p, s.q.p, s.st.p, par, s.part_s.p);
+    // reversed odd sweeps: CG 21.35 -> 21.13 ms at 128^3 Q1 and 270.3 -> 268.0 ms at
+    // 192^3 on the symmetric copy, but 124.3 -> 125.7 ms at 64^3 Q2 on the
+    // class-blocked one, which keeps forward sweeps
+    L(k_spmv<double, double>, GM, B, m, op.sl.v64.p, s.p.p, s.q.p, s.st.p, par, s.part_s.p,
+      m.alt && m.sym ? par & 1 : 0);
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
     L(k_update<double, double>, GV, B, n, s.r.p, s.q.p, invd, s.st.p, par, s.part_s.p, s.g_mat, s.part_u.p);
     L(k_dir<double>, GV, B, n, s.x.p, s.p.p, s.r.p, invd, s.st.p, par, s.part_u.p, s.g_vec, s.hist.p, s.res.p, rh);
   } else if (!three_kernel_refined(s)) {
     const double *sc = jac ? s.invd.p : nullptr;   // S = diag^(-1/2)
     const SlicedDev m32 = fp32_view(s, m);
+    const int rev = m.alt ? par & 1 : 0;
     if (sum32(s))
       L(k_rf_a<float>, GM, B, m, m32, op.sl.v32.p, s.r32.p, s.p32.p, s.q32.p, s.d32.p, s.x.p, sc, s.st.p, par,
-        s.part_u.p, s.g_mat, s.part_s.p, s.res.p, rh);
+        s.part_u.p, s.g_mat, s.part_s.p, s.res.p, rh, rev);
     else
       L(k_rf_a<double>, GM, B, m, m32, op.sl.v32.p, s.r32.p, s.p32.p, s.q32.p, s.d32.p, s.x.p, sc, s.st.p, par,
-        s.part_u.p, s.g_mat, s.part_s.p, s.res.p, rh);
+        s.part_u.p, s.g_mat, s.part_s.p, s.res.p, rh, rev);
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
     L(m.sym ? k_rf_b<true> : k_rf_b<false>, GM, B, m, op.sl.v64.p, s.b.p, s.x.p, s.r32.p, s.q32.p, sc, s.st.p, par, s.part_s.p, s.g_mat,
       s.part_u.p);
   } else {
     const double *sc = jac ? s.invd.p : nullptr;   // S = diag^(-1/2)
     const SlicedDev m32 = fp32_view(s, m);
+    const int rev = m.alt ? par & 1 : 0;
     if (sum32(s))
       L(k_ru_spmv<float>, GM, B, m, m32, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.r32.p, sc,
-        s.st.p, par, s.part_s.p);
+        s.st.p, par, s.part_s.p, rev);
     else
       L(k_ru_spmv<double>, GM, B, m, m32, op.sl.v32.p, op.sl.v64.p, s.p32.p, s.q32.p, s.b.p, s.x.p, s.r32.p, sc,
-        s.st.p, par, s.part_s.p);
+        s.st.p, par, s.part_s.p, rev);
     if (ev) FH_HIP(hipEventRecord(ev[1], L.stream));
     L(k_ru_update, GV, B, n, s.r32.p, s.p32.p, s.q32.p, s.st.p, par, s.part_s.p, s.g_mat, s.part_u.p);
     L(k_ru_dir, GV, B, n, s.d32.p, s.p32.p, s.r32.p, s.x.p, sc, s.st.p, par, s.part_u.p, s.g_vec, s.res.p, rh);
@@ -2046,20 +2064,22 @@ int femhip_solver_profile_spmv(femhip_solver_t *s, int32_t reps, double *ms_per_
     if (mx) hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, st, (int64_t)s->n, s->p.p, s->p32.p);
     const SlicedDev m = sliced_dev(op);
     const SlicedDev m32 = fp32_view(*s, m);
+    int i = 0;   // launches alternate their sweep direction as in CG when the layout asks for it
     auto launch = [&] {
+      const int rev = m.alt && (mx || m.sym) ? (i++ & 1) : 0;
       if (mx)
         hipLaunchKernelGGL((k_spmv<float, float>), dim3(s->g_mat), dim3(kThreads), 0, st, m32, op.sl.v32.p, s->p32.p,
-                           s->q32.p, s->st.p, 0, s->part_s.p);
+                           s->q32.p, s->st.p, 0, s->part_s.p, rev);
       else
         hipLaunchKernelGGL((k_spmv<double, double>), dim3(s->g_mat), dim3(kThreads), 0, st, m, op.sl.v64.p, s->p.p,
-                           s->q.p, s->st.p, 0, s->part_s.p);
+                           s->q.p, s->st.p, 0, s->part_s.p, rev);
     };
-    for (int i = 0; i < 3; ++i) launch();
+    for (int w = 0; w < 3; ++w) launch();
     hipEvent_t e0, e1;
     FH_HIP(hipEventCreate(&e0));
     FH_HIP(hipEventCreate(&e1));
     FH_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < reps; ++i) launch();
+    for (int k = 0; k < reps; ++k) launch();
     FH_HIP(hipEventRecord(e1, st));
     FH_HIP(hipEventSynchronize(e1));
     FH_HIP(hipGetLastError());
