@@ -246,25 +246,48 @@ __global__ void k_block_rhs(FaceTab t, int32_t nrows, int32_t row_off, const int
   rhs[row_off + fr] = acc;
 }
 
-// y = A x (add = 0) or y += A x, full-storage CSR, kSpmvLanes lanes per row
+// y = A x (add = 0) or y += A x, full-storage CSR, LANES lanes per row
 // (rows of 27 to 125 entries): the lanes stride over the row's entries, so
-// the value and column loads of a row are contiguous across the lanes, and a
-// fixed shuffle tree sums them; the blocks of a block row are applied in block
-// order (a block_sparse_matrix_t apply).
-constexpr int kSpmvLanes = 16, kSpmvThreads = 256;
+// the value and column loads of a row are contiguous across the lanes; each
+// lane issues the loads of 4 of its entries before it uses the first, so a
+// long row is not a chain of dependent round trips; a fixed shuffle tree sums
+// the lanes; the blocks of a block row are applied in block order (a
+// block_sparse_matrix_t apply).
+constexpr int kSpmvThreads = 256;
+template <int LANES>
 __global__ __launch_bounds__(kSpmvThreads) void k_block_spmv(int32_t n, const int32_t *__restrict__ rowptr,
                                                              const int32_t *__restrict__ col,
                                                              const double *__restrict__ val,
                                                              const double *__restrict__ x, double *__restrict__ y,
                                                              int add) {
-  const int sub = threadIdx.x % kSpmvLanes;
-  const int64_t r = ((int64_t)blockIdx.x * kSpmvThreads + threadIdx.x) / kSpmvLanes;
+  const int sub = threadIdx.x % LANES;
+  const int64_t r = ((int64_t)blockIdx.x * kSpmvThreads + threadIdx.x) / LANES;
   double acc = 0.0;
-  if (r < n)
-    for (int32_t k = rowptr[r] + sub; k < rowptr[r + 1]; k += kSpmvLanes) acc += val[k] * x[col[k]];
-  for (int o = kSpmvLanes / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, kSpmvLanes);
+  if (r < n) {
+    int32_t k = rowptr[r] + sub;
+    const int32_t e = rowptr[r + 1];
+    for (; k + 3 * LANES < e; k += 4 * LANES) {
+      int32_t c[4];
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        c[u] = col[k + u * LANES];
+        v[u] = val[k + u * LANES];
+      }
+      double g[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) g[u] = x[c[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc += v[u] * g[u];
+    }
+    for (; k < e; k += LANES) acc += val[k] * x[col[k]];
+  }
+  for (int o = LANES / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, LANES);
   if (r < n && sub == 0) y[r] = add ? y[r] + acc : acc;
 }
+
+// 16 lanes a row: 8 take the same time, 32 are 20 % slower (48^3 Q2 + Q1 block CG)
+constexpr int kSpmvLanes = 16;
 
 // dot products in a fixed order: kDotBlocks partial sums (grid-strided rows,
 // block tree), then one block sums them in order
@@ -530,8 +553,9 @@ void block_apply(femhip_block_operator &B, const double *x, double *y) {
     for (int jb = 0; jb < B.nb; ++jb) {
       auto &K = B.blk[ib][jb];
       if (!K.on) continue;
-      hipLaunchKernelGGL(k_block_spmv, dim3(nblocks((int64_t)n * kSpmvLanes, kSpmvThreads)), dim3(kSpmvThreads), 0,
-                         st, n, K.rowptr.p, K.col.p, K.val.p, x + B.boff[jb], y + B.boff[ib], first ? 0 : 1);
+      hipLaunchKernelGGL(k_block_spmv<kSpmvLanes>, dim3(nblocks((int64_t)n * kSpmvLanes, kSpmvThreads)),
+                         dim3(kSpmvThreads), 0, st, n, K.rowptr.p, K.col.p, K.val.p, x + B.boff[jb], y + B.boff[ib],
+                         first ? 0 : 1);
       FH_HIP(hipGetLastError());
       first = 0;
     }
