@@ -259,7 +259,8 @@ __global__ __launch_bounds__(kSpmvThreads) void k_block_spmv(int32_t n, const in
                                                              const int32_t *__restrict__ col,
                                                              const double *__restrict__ val,
                                                              const double *__restrict__ x, double *__restrict__ y,
-                                                             int add) {
+                                                             int add, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;   // the solve has ended (see k_cg_test)
   const int sub = threadIdx.x % LANES;
   const int64_t r = ((int64_t)blockIdx.x * kSpmvThreads + threadIdx.x) / LANES;
   double acc = 0.0;
@@ -303,28 +304,54 @@ __device__ __forceinline__ double block_sum(double v) {
   return red[0];
 }
 __global__ __launch_bounds__(kDotThreads) void k_dot_part(int64_t n, const double *__restrict__ a,
-                                                          const double *__restrict__ b, double *__restrict__ part) {
+                                                          const double *__restrict__ b, double *__restrict__ part,
+                                                          const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kDotThreads + threadIdx.x; i < n; i += (int64_t)kDotBlocks * kDotThreads)
     acc += a[i] * b[i];
   const double s = block_sum(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-__global__ __launch_bounds__(kDotThreads) void k_dot_final(const double *__restrict__ part, double *__restrict__ out) {
+__global__ __launch_bounds__(kDotThreads) void k_dot_final(const double *__restrict__ part, double *__restrict__ out,
+                                                           const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;
   const double s = block_sum(part[threadIdx.x]);
   if (threadIdx.x == 0) *out = s;
 }
 
 // Device scalars of the block CG: kRz = <r,z> of the current direction,
-// kQp = <q,p>, kRzNew = the next <r,z>, kRr = ||r||^2 (the only one the host
-// reads each iteration, for the stopping test).
-enum { kRz = 0, kQp = 1, kRzNew = 2, kRr = 3, kDot = 4, kScalars = 5 };
+// kQp = <q,p>, kRzNew = the next <r,z>, kRr = ||r||^2; the stopping test
+// (k_cg_test) keeps the iteration count, the estimate ||r||, whether it
+// converged and kDone, which every kernel of an iteration reads at entry, so
+// iterations queued after the end return at once.
+enum { kRz = 0, kQp = 1, kRzNew = 2, kRr = 3, kDot = 4, kIt = 5, kEst = 6, kConv = 7, kDone = 8, kScalars = 9 };
+
+// it += 1 ; est = ||r|| ; converged = est <= rhs_test (cg.f90:323-407, RES_RES
+// and RES_RHS) ; done = converged or it >= maxit
+__global__ void k_cg_test(double *__restrict__ sc, double rhs_test, int maxit) {
+  if (sc[kDone] != 0.0) return;
+  const double it = sc[kIt] + 1.0;
+  const double est = sqrt(sc[kRr]);
+  const bool conv = est <= rhs_test;
+  sc[kIt] = it;
+  sc[kEst] = est;
+  sc[kConv] = conv ? 1.0 : 0.0;
+  sc[kDone] = (conv || it >= (double)maxit) ? 1.0 : 0.0;
+}
+// a new solve: no iteration made, not done
+__global__ void k_cg_reset(double *__restrict__ sc) {
+  sc[kIt] = 0.0;
+  sc[kEst] = 0.0;
+  sc[kConv] = 0.0;
+  sc[kDone] = 0.0;
+}
 
 // alpha = <r,z> / <q,p> (0 when <q,p> = 0, cg.f90:195-199); x += alpha p, r -= alpha q
 __global__ void k_cg_xr(int64_t n, const double *__restrict__ sc, double *__restrict__ x,
                         const double *__restrict__ p, double *__restrict__ r, const double *__restrict__ q) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || sc[kDone] != 0.0) return;
   const double qp = sc[kQp];
   const double alpha = qp != 0.0 ? sc[kRz] / qp : 0.0;
   x[i] = x[i] + alpha * p[i];
@@ -332,22 +359,24 @@ __global__ void k_cg_xr(int64_t n, const double *__restrict__ sc, double *__rest
 }
 // z = invd o r (identity: z = r); p = beta p + z as scal then daxpy
 __global__ void k_cg_z(int64_t n, const double *__restrict__ invd, const double *__restrict__ r,
-                       double *__restrict__ z) {
+                       double *__restrict__ z, const double *__restrict__ stop) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (stop && *stop != 0.0)) return;
   z[i] = invd ? invd[i] * r[i] : r[i];
 }
 // beta = (1 / <r,z>_old) <r,z>_new (cg.f90's order); p = beta p + z
 __global__ void k_cg_p(int64_t n, const double *__restrict__ sc, const double *__restrict__ z,
                        double *__restrict__ p) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || sc[kDone] != 0.0) return;
   const double beta = (1.0 / sc[kRz]) * sc[kRzNew];
   double v = beta * p[i];
   p[i] = v + 1.0 * z[i];
 }
 // the new <r,z> becomes the current one (after every k_cg_p block read both)
-__global__ void k_cg_shift(double *__restrict__ sc) { sc[kRz] = sc[kRzNew]; }
+__global__ void k_cg_shift(double *__restrict__ sc) {
+  if (sc[kDone] == 0.0) sc[kRz] = sc[kRzNew];
+}
 // b - A x0 into r
 __global__ void k_sub(int64_t n, const double *__restrict__ b, double *__restrict__ r) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -544,7 +573,7 @@ void block_compute(femhip_block_operator &B, double *ms) {
 }
 
 // y = A x over the block vectors (device pointers)
-void block_apply(femhip_block_operator &B, const double *x, double *y) {
+void block_apply(femhip_block_operator &B, const double *x, double *y, const double *stop = nullptr) {
   hipStream_t st = B.ctx->stream;
   for (int ib = 0; ib < B.nb; ++ib) {
     int first = 1;
@@ -555,7 +584,7 @@ void block_apply(femhip_block_operator &B, const double *x, double *y) {
       if (!K.on) continue;
       hipLaunchKernelGGL(k_block_spmv<kSpmvLanes>, dim3(nblocks((int64_t)n * kSpmvLanes, kSpmvThreads)),
                          dim3(kSpmvThreads), 0, st, n, K.rowptr.p, K.col.p, K.val.p, x + B.boff[jb], y + B.boff[ib],
-                         first ? 0 : 1);
+                         first ? 0 : 1, stop);
       FH_HIP(hipGetLastError());
       first = 0;
     }
@@ -564,10 +593,11 @@ void block_apply(femhip_block_operator &B, const double *x, double *y) {
 }
 
 // <a, b> into the device scalar `slot` (no host wait)
-void block_dot_dev(femhip_block_operator &B, const double *a, const double *b, int slot) {
+void block_dot_dev(femhip_block_operator &B, const double *a, const double *b, int slot,
+                   const double *stop = nullptr) {
   hipStream_t st = B.ctx->stream;
-  hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(kDotThreads), 0, st, B.ntot, a, b, B.part.p);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, st, B.part.p, B.scal.p + slot);
+  hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(kDotThreads), 0, st, B.ntot, a, b, B.part.p, stop);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, st, B.part.p, B.scal.p + slot, stop);
   FH_HIP(hipGetLastError());
 }
 
@@ -582,6 +612,8 @@ double block_dot(femhip_block_operator &B, const double *a, const double *b) {
   block_dot_dev(B, a, b, kDot);
   return read_scalar(B, kDot);
 }
+
+constexpr int kBlockChunk = 8;   // iterations queued between two reads of the device state
 
 // cg_solve_body (cg.f90:101-238) on the block operator with the RES_RES or
 // RES_RHS criterion (cg.f90:279-407), identity or Jacobi (the inverse
@@ -615,32 +647,40 @@ void block_cg(femhip_block_operator &B, const femhip_cg_params_t &prm, const dou
   const double *invd = jac ? B.invd.p : nullptr;
   block_apply(B, B.x.p, B.r.p);
   hipLaunchKernelGGL(k_sub, G, T, 0, st, n, b, B.r.p);
-  hipLaunchKernelGGL(k_cg_z, G, T, 0, st, n, invd, B.r.p, B.z.p);
+  hipLaunchKernelGGL(k_cg_z, G, T, 0, st, n, invd, B.r.p, B.z.p, nullptr);
   FH_HIP(hipGetLastError());
   block_dot_dev(B, B.r.p, B.z.p, kRz);
   FH_HIP(hipMemcpyAsync(B.p.p, B.z.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
   const double rhs_test =
       prm.stopping_criterion == FEMHIP_RES_RHS ? prm.rtol * std::sqrt(block_dot(B, b, b)) + prm.atol
                                                : prm.rtol * std::sqrt(block_dot(B, B.r.p, B.r.p)) + prm.atol;
-  int it = 0;
-  bool conv = false;
-  double est = 0.0;
-  for (;;) {   // alpha and beta stay on the device; the host reads ||r|| once per iteration
-    ++it;
-    block_apply(B, B.p.p, B.q.p);
-    block_dot_dev(B, B.q.p, B.p.p, kQp);
-    hipLaunchKernelGGL(k_cg_xr, G, T, 0, st, n, B.scal.p, B.x.p, B.p.p, B.r.p, B.q.p);
-    FH_HIP(hipGetLastError());
-    block_dot_dev(B, B.r.p, B.r.p, kRr);
-    est = std::sqrt(read_scalar(B, kRr));
-    conv = est <= rhs_test;
-    if (conv || it >= prm.max_num_iterations) break;
-    hipLaunchKernelGGL(k_cg_z, G, T, 0, st, n, invd, B.r.p, B.z.p);
-    block_dot_dev(B, B.r.p, B.z.p, kRzNew);
-    hipLaunchKernelGGL(k_cg_p, G, T, 0, st, n, B.scal.p, B.z.p, B.p.p);
-    hipLaunchKernelGGL(k_cg_shift, dim3(1), dim3(1), 0, st, B.scal.p);
-    FH_HIP(hipGetLastError());
+  // alpha, beta and the stopping test stay on the device: the host queues
+  // kBlockChunk iterations and reads the state once per chunk; the kernels of
+  // iterations queued past the end return at entry (kDone)
+  hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(1), 0, st, B.scal.p);
+  const double *stop = B.scal.p + kDone;
+  double h[4] = {0.0, 0.0, 0.0, 0.0};   // kIt, kEst, kConv, kDone
+  static_assert(kEst == kIt + 1 && kConv == kIt + 2 && kDone == kIt + 3, "state slots are contiguous");
+  for (;;) {
+    for (int c = 0; c < kBlockChunk; ++c) {
+      block_apply(B, B.p.p, B.q.p, stop);
+      block_dot_dev(B, B.q.p, B.p.p, kQp, stop);
+      hipLaunchKernelGGL(k_cg_xr, G, T, 0, st, n, B.scal.p, B.x.p, B.p.p, B.r.p, B.q.p);
+      block_dot_dev(B, B.r.p, B.r.p, kRr, stop);
+      hipLaunchKernelGGL(k_cg_test, dim3(1), dim3(1), 0, st, B.scal.p, rhs_test, prm.max_num_iterations);
+      hipLaunchKernelGGL(k_cg_z, G, T, 0, st, n, invd, B.r.p, B.z.p, stop);
+      block_dot_dev(B, B.r.p, B.z.p, kRzNew, stop);
+      hipLaunchKernelGGL(k_cg_p, G, T, 0, st, n, B.scal.p, B.z.p, B.p.p);
+      hipLaunchKernelGGL(k_cg_shift, dim3(1), dim3(1), 0, st, B.scal.p);
+      FH_HIP(hipGetLastError());
+    }
+    FH_HIP(hipMemcpyAsync(h, B.scal.p + kIt, sizeof(h), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    if (h[3] != 0.0) break;
   }
+  const int it = (int)h[0];
+  const bool conv = h[2] != 0.0;
+  const double est = h[1];
   FH_HIP(hipEventRecord(e1, st));
   FH_HIP(hipEventSynchronize(e1));
   float t = 0.f;
