@@ -1424,10 +1424,13 @@ bool mixed(const femhip_solver &s) { return s.prm.precision != FEMHIP_PRECISION_
 bool strict(const femhip_solver &s) { return s.prm.precision == FEMHIP_PRECISION_FP32_STRICT; }
 // FP32 row sums in the refined iterations: always for FP32_STRICT, for
 // FP32_REFINED when FEMHIP_RF_SUM32=1 (faster slots, more inner iterations at
-// 128^3 Q1, fewer milliseconds on large high-contrast fields).  Read per call.
+// 128^3 Q1, fewer milliseconds on large high-contrast fields).  Read once.
 bool sum32(const femhip_solver &s) {
-  const char *e = getenv("FEMHIP_RF_SUM32");
-  return strict(s) || (e && atoi(e) == 1);
+  static const bool env = [] {
+    const char *e = getenv("FEMHIP_RF_SUM32");
+    return e && atoi(e) == 1;
+  }();
+  return strict(s) || env;
 }
 
 // The FP32 matrix as the kernels see it: with Jacobi it holds S A S, whose unit
