@@ -77,8 +77,9 @@ def grf_k_cell(n, sigma=1.5, ell=0.1, seed=20261202):
     return np.exp(c).ravel()
 
 
-def measure_config(F, env, n, order, di, rtol, steps, warmup, with_mixed=True, exact=None):
-    """Assembly + Jacobi-PCG DOFs/s of one configuration in FP64 and FP32-refined."""
+def measure_config(F, env, n, order, di, rtol, steps, warmup, with_mixed=True, exact=None, strict=False):
+    """Assembly + Jacobi-PCG DOFs/s of one configuration in FP64 and FP32-refined
+    (and FP32_STRICT: the FP32 matrix integrated and summed in FP32)."""
     tria = F.serial_triangulation_t().create(env, {
         "STRUCT_HEX_MESH_GENERATOR_NUM_DIMS": 3, "STRUCT_HEX_MESH_GENERATOR_NUM_CELLS_X_DIM": [n, n, n],
         "STRUCT_HEX_MESH_GENERATOR_DOMAIN_LIMITS": [0, 1, 0, 1, 0, 1]})
@@ -88,7 +89,8 @@ def measure_config(F, env, n, order, di, rtol, steps, warmup, with_mixed=True, e
     op.create("CSR", [True], fes, di)
     out = {"n_free_dofs": fes.num_free_dofs, "nnz_full": op.nnz_full, "setup_s": time.perf_counter() - t0}
     xs, solvers = {}, []
-    for prec in ("FP64", "FP32_REFINED") if with_mixed else ("FP64",):
+    precs = ("FP64", "FP32_REFINED") if with_mixed else ("FP64",)
+    for prec in precs + (("FP32_STRICT",) if with_mixed and strict else ()):
         s = make_solver(F, env, op, prec, rtol)
         solvers.append(s)
         first = None
@@ -112,6 +114,10 @@ def measure_config(F, env, n, order, di, rtol, steps, warmup, with_mixed=True, e
         out["speedup_mixed_vs_fp64"] = out["FP32_REFINED"]["value"] / out["FP64"]["value"]
         out["rel_err_mixed_vs_fp64"] = float(np.linalg.norm(xs["FP32_REFINED"] - xs["FP64"]) /
                                              np.linalg.norm(xs["FP64"]))
+    if "FP32_STRICT" in out:
+        out["speedup_strict_vs_fp64"] = out["FP32_STRICT"]["value"] / out["FP64"]["value"]
+        out["rel_err_strict_vs_fp64"] = float(np.linalg.norm(xs["FP32_STRICT"] - xs["FP64"]) /
+                                              np.linalg.norm(xs["FP64"]))
     for s in solvers:
         s.free()
     op.free()
@@ -392,7 +398,7 @@ def main():
             t0 = time.perf_counter()
             kc = grf_k_cell(192)
             di = F.poisson_discrete_integration_t(k="cell", k_cell=kc, f="const", f0=1.0)
-            r = measure_config(F, env, 192, 1, di, args.rtol, 1, 1)
+            r = measure_config(F, env, 192, 1, di, args.rtol, 1, 1, strict=True)
             r["k_contrast"] = float(kc.max() / kc.min())
             r["field_s"] = time.perf_counter() - t0 - r["setup_s"]
             res["poisson3d_q1_192_grf"] = r

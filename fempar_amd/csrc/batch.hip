@@ -349,8 +349,10 @@ __global__ void k_batch_start(int *counter, long long *t0) {
 // branch that encloses or follows a barrier is wave-uniform: the queue fetch is
 // made by the whole of wave 0 (lane 0 adds 1, the others 0) and the fetched
 // index is read back through readfirstlane.
+// BS threads (BS / 64 waves) per problem; 1024 / BS workgroups per CU, each
+// held to 128 VGPRs (four waves per SIMD)
 template <int BS>
-__global__ __launch_bounds__(BS) void k_batch_solve(const BProb *__restrict__ probs, const int *__restrict__ order,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_solve(const BProb *__restrict__ probs, const int *__restrict__ order,
                                                     int count, int *counter, BPools pl, double rtol, double delta,
                                                     int maxit, BOut *out) {
   __shared__ double sm[3 * (BS / 64)];
@@ -409,11 +411,19 @@ BPools pools(femhip_batch &B) {
 
 void launch_solve(femhip_batch &B, const int *order, int count, int threads) {
   hipStream_t st = B.ctx->stream;
-  const int grid = std::max(1, B.ctx->num_cus * std::max(1, (threads == 256 ? 4 : threads == 512 ? 2 : 1)));
+  const int grid = std::max(1, B.ctx->num_cus * std::max(1, 1024 / threads));   // 16 waves a CU
   const double delta = B.prm.inner_rtol > 0 ? B.prm.inner_rtol : 0.1;
   hipLaunchKernelGGL(k_batch_start, dim3(1), dim3(64), 0, st, B.counter.p, B.t0.p);
   BPools pl = pools(B);
   switch (threads) {
+    case 64:
+      hipLaunchKernelGGL(k_batch_solve<64>, dim3(grid), dim3(64), 0, st, B.d_probs.p, order, count, B.counter.p, pl,
+                         B.prm.rtol, delta, B.prm.max_num_iterations, B.out.p);
+      break;
+    case 128:
+      hipLaunchKernelGGL(k_batch_solve<128>, dim3(grid), dim3(128), 0, st, B.d_probs.p, order, count, B.counter.p,
+                         pl, B.prm.rtol, delta, B.prm.max_num_iterations, B.out.p);
+      break;
     case 256:
       hipLaunchKernelGGL(k_batch_solve<256>, dim3(grid), dim3(256), 0, st, B.d_probs.p, order, count, B.counter.p,
                          pl, B.prm.rtol, delta, B.prm.max_num_iterations, B.out.p);
@@ -543,7 +553,12 @@ int femhip_batch_create(femhip_ctx_t *ctx, int32_t count, const femhip_batch_pro
       for (auto *buf : {&B->r, &B->p, &B->q, &B->d, &B->invd}) buf->alloc((size_t)B->vec);
       B->b.alloc((size_t)B->vec);
       B->x.alloc((size_t)B->vec);
-      if (const char *t = getenv("FEMHIP_BATCH_THREADS")) B->threads = atoi(t);
+      if (const char *t = getenv("FEMHIP_BATCH_THREADS")) {
+        const int v = atoi(t);
+        FH_CHECK(v == 64 || v == 128 || v == 256 || v == 512 || v == 1024, FEMHIP_ERR_INVALID,
+                 "FEMHIP_BATCH_THREADS must be 64, 128, 256, 512 or 1024");
+        B->threads = v;
+      }
     } catch (...) {
       femhip_batch_destroy(B);
       throw;
@@ -599,7 +614,7 @@ int femhip_batch_autotune(femhip_batch_t *B, int32_t *threads) {
     FH_HIP(hipMemcpy(B->d_sample.p, sample.data(), sizeof(int) * sample.size(), hipMemcpyHostToDevice));
     float best = 1e30f;
     int best_t = B->threads;
-    for (int t : {256, 512, 1024}) {
+    for (int t : {64, 128, 256, 512, 1024}) {   // waves per problem: 1 to 16
       timed_solve(*B, B->d_sample.p, (int)sample.size(), t);   // warm-up
       const float ms = timed_solve(*B, B->d_sample.p, (int)sample.size(), t);
       if (ms < best) {

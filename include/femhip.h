@@ -373,8 +373,10 @@ int femhip_batch_get_solution(const femhip_batch_t *batch, int32_t i, double *x)
 int femhip_batch_get_problem_matrix(const femhip_batch_t *batch, int32_t i, int32_t *irp, int32_t *ja,
                                     double *val, double *rhs, int64_t *nnz);
 /* Autotuning: time the persistent solve on a sample of the batch for each
- * block shape and keep the fastest (also used by femhip_batch_solve when
- * FEMHIP_AUTOTUNE is set).  Chosen threads-per-block in *threads. */
+ * block shape (64, 128, 256, 512 or 1024 threads: 1 to 16 wavefronts per
+ * problem, 16 wavefronts per CU) and keep the fastest (also used by
+ * femhip_batch_solve when FEMHIP_AUTOTUNE is set).  Chosen threads-per-block
+ * in *threads. */
 int femhip_batch_autotune(femhip_batch_t *batch, int32_t *threads);
 int femhip_batch_destroy(femhip_batch_t *batch);
 
