@@ -365,7 +365,7 @@ def test_error_norms_converge_at_the_theoretical_rates(order, l2_rate, h1_rate):
 
 
 # ------------------------------------------- variable-coefficient branches -----
-# The 3-D BASELINE configurations run the oracle's k(x_q) and manufactured-source
+# The 3-D benchmark configurations (128^3 Q1, 64^3 Q2) run the oracle's k(x_q) and manufactured-source
 # code (k = 1 + 0.5 sin(2 pi x) sin(2 pi y) sin(2 pi z), f = -div(k grad u)); the
 # pins below check those branches without the oracle's own formulas.
 
