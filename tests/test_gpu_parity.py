@@ -247,13 +247,14 @@ def test_fp32_integration_on_matrix_cores(env, name, monkeypatch):
     assert its["rounded"] - 2 <= its["mfma"] <= its["rounded"] * 1.3 + 2, its
 
 
-@pytest.mark.parametrize("name", ["2d_q1_64_f1", "3d_q1_20_sin3", "3d_q1_cellk", "3d_q2_8_sin3"])
+@pytest.mark.parametrize("name", ["2d_q1_64_f1", "3d_q1_20_sin3", "3d_q1_cellk", "3d_q2_8_sin3", "elasticity_8"])
 def test_fp32_strict_reaches_fp64_accuracy(env, name):
     """FEMHIP_PRECISION = FP32_STRICT: the FP32 matrix integrated in FP32 on
     the matrix cores (scalar Q1; rounded from FP64 for Q2) and its rows summed
     in FP32, inside the same FP64 refinement.  Solved on one operator by turns
     with an FP32_REFINED solver, so that each solve must make the operator's
-    FP32 copy its own way again: both reach the oracle's FP64 solution within
+    FP32 copy its own way again (vector Q1 and Q2 run the three-kernel slots
+    on the class-blocked layout): both reach the oracle's FP64 solution within
     1e-8 with a true FP64 residual within rtol ||b||, and the strict solve takes
     at most 40 % more inner iterations (FP32 row sums of 7 or 27 terms add a
     few ulps per product; a wrong accumulator or a stale copy of the other
